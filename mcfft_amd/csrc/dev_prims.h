@@ -1,0 +1,65 @@
+// dev_prims.h -- the handful of device primitives the FFT kernels use.
+//
+// The kernels are written once against this header.  Built by hipcc for
+// gfx950 the primitives are the CDNA4 instructions themselves (MFMA
+// 32x32x16 f16/bf16, LDS, wave64 lane id, workgroup barrier).  Built by a
+// host compiler with MCFFT_EMU defined (test infrastructure under tests/emu/)
+// each workgroup runs as host threads: LDS and global accesses are
+// bounds-checked and the MFMA is evaluated from the documented operand
+// layout, so indexing mistakes surface on the CPU before any GPU run.
+#pragma once
+#include <stdint.h>
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+#ifndef MCFFT_EMU
+#include <hip/hip_runtime.h>
+
+#define MCFFT_DEV __device__ __forceinline__
+#define MCFFT_KERNEL(nt) extern "C" __global__ __launch_bounds__(nt)
+#define MCFFT_TKERNEL(nt) __global__ __launch_bounds__(nt)
+#define MCFFT_DYN_LDS(type, name) \
+  extern __shared__ __attribute__((aligned(16))) unsigned char mcfft_lds_raw[]; \
+  type* name = (type*)mcfft_lds_raw
+
+namespace mcfft {
+MCFFT_DEV int lane_id() { return __lane_id(); }
+MCFFT_DEV int thread_id() { return threadIdx.x; }
+MCFFT_DEV int block_id() { return blockIdx.x; }
+MCFFT_DEV void block_barrier() { __syncthreads(); }
+
+template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
+template <class T> MCFFT_DEV void lds_st(T* p, int i, T v) { p[i] = v; }
+template <class T> MCFFT_DEV T g_ld(const T* p, long long i) { return p[i]; }
+template <class T> MCFFT_DEV void g_st(T* p, long long i, T v) { p[i] = v; }
+
+// D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
+// Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and
+// D[8*(v/4) + 4*(l/32) + v%4][l%32] in accumulator v = 0..15.
+MCFFT_DEV f32x16 mfma_f16(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+MCFFT_DEV f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+}  // namespace mcfft
+
+#else  // ------------------------------ CPU emulation ---------------------------
+#include "emu_runtime.h"
+#endif
+
+namespace mcfft {
+// ---- scalar format conversions (RNE) ----
+MCFFT_DEV uint32_t f16_bits(float x) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x); }
+MCFFT_DEV float f16_val(uint32_t bits) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits); }
+MCFFT_DEV uint32_t bf16_bits(float x) { return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x); }
+MCFFT_DEV float bf16_val(uint32_t bits) { return (float)__builtin_bit_cast(__bf16, (uint16_t)bits); }
+MCFFT_DEV uint32_t pack2(uint32_t lo, uint32_t hi) { return (lo & 0xffffu) | (hi << 16); }
+// pack the low halves / high halves of two packed pairs into one pair
+MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
+MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }
+}  // namespace mcfft

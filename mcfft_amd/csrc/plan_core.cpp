@@ -1,0 +1,276 @@
+// plan_core.cpp -- see plan_core.h
+#include "plan_core.h"
+
+#include <cstdio>
+#include <sstream>
+
+namespace mcfft {
+
+int elem_bytes(int fmt) { return fmt == 2 ? 8 : 4; }
+
+int compute_mode(int in_fmt, int out_fmt) {
+  if (in_fmt == 0 && out_fmt == 0) return 0;        // FP16 -> FP16
+  if (in_fmt == 1 && out_fmt == 1) return 1;        // BF16 -> BF16
+  if ((in_fmt == 0 || in_fmt == 2) && out_fmt == 2) return 2;  // FP16 or split -> split
+  return -1;
+}
+
+static int ilog2(long long v) {
+  int r = 0;
+  while ((1LL << r) < v) ++r;
+  return ((1LL << r) == v) ? r : -1;
+}
+
+// L = 16^S * T with S >= 1, T in {1,2,4,8}
+static bool split_len(long long L, int& S, int& T) {
+  int lg = ilog2(L);
+  if (lg < 4 || lg > 12) return false;
+  S = lg / 4;
+  T = 1 << (lg % 4);
+  return true;
+}
+
+static const KernelInfo* find_kernel(int S, int T, int cm, int qfast, int staged, int fourstep) {
+  const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
+  const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  for (int l = 0; l < 4; ++l)
+    for (int i = 0; i < counts[l]; ++i) {
+      const KernelInfo& k = lists[l][i];
+      if (k.S == S && k.T == T && k.cm == cm && k.qfast == qfast && k.staged == staged && k.fourstep == fourstep)
+        return &k;
+    }
+  return nullptr;
+}
+
+static int stage_shift(int radix, int scaling) {
+  int lg = ilog2(radix);
+  if (scaling == SCALE_FULL) return lg;
+  if (scaling == SCALE_BALANCED) return lg / 2;
+  return 0;
+}
+
+static float pow2(int e) {  // 2^e, exact
+  float s = 1.f;
+  for (int i = 0; i < e; ++i) s *= 2.f;
+  for (int i = 0; i > e; --i) s *= 0.5f;
+  return s;
+}
+
+// Append the stage post-twiddle tables of one pass of length L (stages 16^S then T)
+// and return the per-stage shifts.  Table of stage P, entry rr*(L/16)+j, is the
+// input twiddle of stage P+1 for the element stage P writes at
+// m' = (j/Ns)*Ns*16 + j%Ns + rr*Ns, times 2^-shift_P.
+static std::vector<int> add_stage_tables(PlanCore& p, PassPlan& pp, int S, int T, int scaling) {
+  const long long L = (1LL << (4 * S)) * T;
+  std::vector<int> radix;
+  for (int i = 0; i < S; ++i) radix.push_back(16);
+  if (T > 1) radix.push_back(T);
+  std::vector<int> shifts;
+  for (int r : radix) shifts.push_back(stage_shift(r, scaling));
+  pp.stage_tw_off = (long long)p.tw.size();
+  long long ns = 1;
+  for (size_t P = 0; P + 1 < radix.size(); ++P) {
+    const int R = radix[P];  // always 16 (the tail is last)
+    const long long nsn = ns * R;
+    const int Rn = radix[P + 1];
+    for (long long rr = 0; rr < R; ++rr)
+      for (long long j = 0; j < L / R; ++j) {
+        const long long mo = (j / ns) * ns * R + (j % ns) + rr * ns;
+        const long long r2 = mo / (L / Rn), j2 = mo % (L / Rn), k2 = j2 % nsn;
+        p.tw.push_back({(r2 * k2) % (nsn * Rn), nsn * Rn, pow2(-shifts[P])});
+      }
+    ns = nsn;
+  }
+  return shifts;
+}
+
+int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
+               long long ws_limit_bytes) {
+  p = PlanCore();
+  if (rank < 1 || rank > 3 || batch < 1 || !dims) return ST_INVALID_VALUE;
+  if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2) return ST_INVALID_VALUE;
+  if (scaling < 0 || scaling > 2) return ST_INVALID_VALUE;
+  const int cm = compute_mode(in_fmt, out_fmt);
+  if (cm < 0) return ST_UNSUPPORTED;
+  p.rank = rank;
+  p.batch = batch;
+  p.in_fmt = in_fmt;
+  p.out_fmt = out_fmt;
+  p.cm = cm;
+  p.scaling = scaling;
+  p.n = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (dims[d] < 1) return ST_INVALID_VALUE;
+    if (ilog2(dims[d]) < 0) return ST_UNSUPPORTED;  // powers of two only
+    p.dims[d] = dims[d];
+    p.n *= dims[d];
+  }
+  const int ws_fmt = (cm == 2) ? 2 : in_fmt;
+  int total_shift = 0;
+
+  auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt) -> int {
+    int S, T;
+    if (!split_len(L, S, T)) return ST_UNSUPPORTED;
+    const KernelInfo* k = find_kernel(S, T, cm, 0, 0, 0);
+    if (!k) return ST_UNSUPPORTED;
+    PassPlan pp;
+    pp.k = k;
+    pp.src = src;
+    pp.dst = dst;
+    pp.src_fmt = src_fmt;
+    pp.in_sq = pp.out_sq = L;
+    pp.in_sm = pp.out_sm = 1;
+    pp.in_sg = pp.out_sg = 0;
+    pp.in_st = pp.out_st = (long long)k->Q * L;
+    pp.n_inst = n_rows;
+    pp.tiles_per_group = (int)((n_rows + k->Q - 1) / k->Q);
+    pp.groups = 1;
+    pp.shifts = add_stage_tables(p, pp, S, T, scaling);
+    pp.out_scale = pow2(-pp.shifts.back());
+    for (int s : pp.shifts) total_shift += s;
+    p.passes.push_back(pp);
+    return ST_OK;
+  };
+  // strided columns: L points at stride `stride`, `ncols` consecutive columns per
+  // group, groups `groups` apart by `gstride` elements
+  auto add_col = [&](long long L, long long stride, long long ncols, long long groups, long long gstride, int src,
+                     int dst, int src_fmt) -> int {
+    int S, T;
+    if (!split_len(L, S, T)) return ST_UNSUPPORTED;
+    const KernelInfo* k = find_kernel(S, T, cm, 1, 0, 0);
+    if (!k || ncols % k->Q != 0) return ST_UNSUPPORTED;
+    PassPlan pp;
+    pp.k = k;
+    pp.src = src;
+    pp.dst = dst;
+    pp.src_fmt = src_fmt;
+    pp.in_sq = pp.out_sq = 1;
+    pp.in_sm = pp.out_sm = stride;
+    pp.in_sg = pp.out_sg = gstride;
+    pp.in_st = pp.out_st = k->Q;
+    pp.n_inst = ncols;
+    pp.tiles_per_group = (int)(ncols / k->Q);
+    pp.groups = groups;
+    pp.groups_per_transform = groups / batch;
+    pp.shifts = add_stage_tables(p, pp, S, T, scaling);
+    pp.out_scale = pow2(-pp.shifts.back());
+    for (int s : pp.shifts) total_shift += s;
+    p.passes.push_back(pp);
+    return ST_OK;
+  };
+
+  int st = ST_OK;
+  if (rank == 1) {
+    const long long N = p.dims[0];
+    if (N <= 4096) {
+      p.chunk = batch;
+      p.n_chunks = 1;
+      st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt);
+    } else {
+      // four-step: N = N1 * N2; columns of length N1 (stride N2) with twiddle
+      // W_N^(k1*n2), then rows of length N2 stored transposed X[k1 + N1*k2]
+      const int lg = ilog2(N);
+      int lg2 = (lg + 1) / 2;
+      if (lg2 > 11) lg2 = 11;
+      const int lg1 = lg - lg2;
+      if (lg1 > 12) return ST_UNSUPPORTED;
+      const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
+      int S1, T1, S2, T2;
+      if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
+      const KernelInfo* kc = find_kernel(S1, T1, cm, 1, 0, 1);
+      const KernelInfo* kr = find_kernel(S2, T2, cm, 0, 1, 0);
+      if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
+      long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (256LL << 20);
+      long long per = N * elem_bytes(ws_fmt);
+      long long chunk = limit / per;
+      if (chunk < 1) chunk = 1;
+      if (chunk > batch) chunk = batch;
+      p.chunk = chunk;
+      p.n_chunks = (batch + chunk - 1) / chunk;
+      p.ws_elems = chunk * N;
+      // pass A (columns + four-step twiddle): in -> ws
+      PassPlan a;
+      a.k = kc;
+      a.src = BUF_IN;
+      a.dst = BUF_WS;
+      a.src_fmt = in_fmt;
+      a.in_sq = a.out_sq = 1;
+      a.in_sm = a.out_sm = N2;
+      a.in_sg = a.out_sg = N;
+      a.in_st = a.out_st = kc->Q;
+      a.n_inst = N2;
+      a.tiles_per_group = (int)(N2 / kc->Q);
+      a.groups = chunk;
+      a.groups_per_transform = 1;
+      a.shifts = add_stage_tables(p, a, S1, T1, scaling);
+      a.fourstep = true;
+      a.fs_shift = (lg + 1) / 2;
+      a.fs_mask = (unsigned)((1LL << a.fs_shift) - 1);
+      a.fs_nmask = (unsigned long long)(N - 1);
+      a.fs_hi_off = (long long)p.tw.size();
+      for (long long i = 0; i < (N >> a.fs_shift); ++i) p.tw.push_back({i << a.fs_shift, N, 1.f});
+      a.fs_lo_off = (long long)p.tw.size();
+      const float sl = pow2(-a.shifts.back());
+      for (long long i = 0; i < (1LL << a.fs_shift); ++i) p.tw.push_back({i, N, sl});
+      a.out_scale = 1.f;
+      for (int s : a.shifts) total_shift += s;
+      p.passes.push_back(a);
+      // pass B (rows, transposed store): ws -> out
+      PassPlan b;
+      b.k = kr;
+      b.src = BUF_WS;
+      b.dst = BUF_OUT;
+      b.src_fmt = ws_fmt;
+      b.in_sq = N2;
+      b.in_sm = 1;
+      b.out_sq = 1;
+      b.out_sm = N1;
+      b.in_sg = b.out_sg = N;
+      b.in_st = (long long)kr->Q * N2;
+      b.out_st = kr->Q;
+      b.n_inst = N1;
+      b.tiles_per_group = (int)(N1 / kr->Q);
+      b.groups = chunk;
+      b.groups_per_transform = 1;
+      b.shifts = add_stage_tables(p, b, S2, T2, scaling);
+      b.out_scale = pow2(-b.shifts.back());
+      for (int s : b.shifts) total_shift += s;
+      p.passes.push_back(b);
+    }
+  } else if (rank == 2) {
+    const long long NX = p.dims[0], NY = p.dims[1];
+    p.chunk = batch;
+    p.n_chunks = 1;
+    st = add_row(NY, batch * NX, BUF_IN, BUF_OUT, in_fmt);
+    if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
+  } else {
+    const long long NX = p.dims[0], NY = p.dims[1], NZ = p.dims[2];
+    p.chunk = batch;
+    p.n_chunks = 1;
+    st = add_row(NZ, batch * NX * NY, BUF_IN, BUF_OUT, in_fmt);
+    if (st == ST_OK) st = add_col(NY, NZ, NZ, batch * NX, NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
+    if (st == ST_OK) st = add_col(NX, NY * NZ, NY * NZ, batch, NX * NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
+  }
+  if (st != ST_OK) return st;
+  p.out_exponent = total_shift;
+  return ST_OK;
+}
+
+std::string PlanCore::describe() const {
+  std::ostringstream o;
+  o << "rank=" << rank << " dims=" << dims[0];
+  for (int d = 1; d < rank; ++d) o << "x" << dims[d];
+  o << " batch=" << batch << " cm=" << cm << " in=" << in_fmt << " out=" << out_fmt << " exp=" << out_exponent
+    << " chunk=" << chunk << "x" << n_chunks << " ws=" << ws_elems << "\n";
+  for (const auto& pp : passes) {
+    o << "  pass " << pp.k->name << " src=" << pp.src << " dst=" << pp.dst << " groups=" << pp.groups
+      << " tiles/group=" << pp.tiles_per_group << " n_inst=" << pp.n_inst << " in(sq,sm)=(" << pp.in_sq << ","
+      << pp.in_sm << ") out(sq,sm)=(" << pp.out_sq << "," << pp.out_sm << ") lds=" << pp.k->lds_bytes
+      << (pp.fourstep ? " fourstep" : "") << " shifts=";
+    for (int s : pp.shifts) o << s << ",";
+    o << "\n";
+  }
+  return o.str();
+}
+
+}  // namespace mcfft

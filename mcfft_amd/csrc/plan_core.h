@@ -1,0 +1,83 @@
+// plan_core.h -- pure C++ planning (no HIP calls): pass decomposition, strides,
+// per-stage scale shifts and the twiddle-table specification of a plan.
+//
+// Reference counterpart: mcfftCreate's size switch
+// (source/mcfft_1d_device.cpp:680-790, source/mcfft_2d_device.cpp:625-696),
+// which hard-codes radices/mergings for ten 1D sizes and three 2D sides.  Here
+// any power-of-two length 16..2^23 (1D) or side 16..4096 (2D/3D) is decomposed
+// into passes of the kernel family in fft_pass.h.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "kernel_list.h"
+
+namespace mcfft {
+
+enum Buf { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
+enum Scaling { SCALE_NONE = 0, SCALE_BALANCED = 1, SCALE_FULL = 2 };
+
+// one twiddle-table entry: exp(sign * 2 pi i * e / M) * scale (sign = -1 forward)
+struct TwSpec {
+  long long e, M;
+  float scale;
+};
+
+struct PassPlan {
+  const KernelInfo* k = nullptr;
+  int src = BUF_IN, dst = BUF_OUT;
+  int src_fmt = 0;
+  long long in_sq = 0, in_sm = 0, out_sq = 0, out_sm = 0;
+  long long in_sg = 0, in_st = 0, out_sg = 0, out_st = 0;
+  long long n_inst = 0;         // valid instances per group
+  int tiles_per_group = 1;
+  long long groups = 1;         // groups when the whole batch runs as one chunk
+  long long groups_per_transform = 0;  // >0: groups = transforms in the chunk * this
+  float out_scale = 1.f;
+  long long stage_tw_off = 0;   // first entry of this pass's stage tables
+  bool fourstep = false;
+  long long fs_hi_off = 0, fs_lo_off = 0;
+  int fs_shift = 0;
+  unsigned fs_mask = 0;
+  unsigned long long fs_nmask = 0;
+  std::vector<int> shifts;      // per stage, for reporting
+};
+
+struct PlanCore {
+  int rank = 0;
+  long long dims[3] = {1, 1, 1};
+  long long batch = 0;
+  long long n = 0;              // elements per transform
+  int in_fmt = 0, out_fmt = 0, cm = 0;
+  int scaling = SCALE_BALANCED;
+  int out_exponent = 0;         // true value = stored * 2^out_exponent
+  long long chunk = 0;          // transforms per chunk
+  long long n_chunks = 0;
+  long long ws_elems = 0;       // workspace elements in the compute format
+  std::vector<PassPlan> passes;
+  std::vector<TwSpec> tw;       // twiddle entries (same spec for both directions)
+  std::string describe() const;
+};
+
+// status codes shared with the C ABI (include/mcfft.h)
+enum {
+  ST_OK = 0,
+  ST_INVALID_VALUE = 1,
+  ST_UNSUPPORTED = 2,
+  ST_ALLOC_FAILED = 3,
+  ST_HIP_ERROR = 4,
+  ST_MISALIGNED = 5,
+  ST_INVALID_PLAN = 6,
+  ST_CACHE_ERROR = 7,
+};
+
+int elem_bytes(int fmt);
+// compute mode from the input/output formats; -1 if the pair is not supported
+int compute_mode(int in_fmt, int out_fmt);
+// ws_limit_bytes: upper bound of the four-step workspace (0 = default)
+int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
+               long long ws_limit_bytes);
+
+}  // namespace mcfft

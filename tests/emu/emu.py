@@ -1,0 +1,90 @@
+"""ctypes binding of the CPU emulation build (tests/emu/_build/libmcfft_emu.so)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "libmcfft_emu.so")
+_lib = None
+
+FMT_F16, FMT_BF16, FMT_SPLIT = 0, 1, 2
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", _HERE, "-j8"], check=True)
+        L = ctypes.CDLL(_LIB)
+        L.emu_exec.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_longlong, ctypes.POINTER(ctypes.c_int)]
+        L.emu_exec.restype = ctypes.c_int
+        L.emu_plan_describe.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.emu_plan_describe.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def to_storage(x: np.ndarray, fmt: int) -> np.ndarray:
+    """complex128 -> storage array (uint16 pairs, or hi/lo pairs for split)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+    if fmt == FMT_F16:
+        v = np.empty(2 * x.size, dtype=np.float16)
+        v[0::2] = x.real
+        v[1::2] = x.imag
+        return v.view(np.uint16)
+    if fmt == FMT_BF16:
+        f = np.empty(2 * x.size, dtype=np.float32)
+        f[0::2] = x.real
+        f[1::2] = x.imag
+        u = f.view(np.uint32)
+        r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        return r
+    hi = to_storage(x, FMT_F16).view(np.float16)
+    re = x.real - hi[0::2].astype(np.float64)
+    im = x.imag - hi[1::2].astype(np.float64)
+    lo = np.empty(2 * x.size, dtype=np.float16)
+    lo[0::2] = re * 2048.0
+    lo[1::2] = im * 2048.0
+    out = np.empty((x.size, 4), dtype=np.uint16)
+    out[:, 0:2] = hi.view(np.uint16).reshape(-1, 2)
+    out[:, 2:4] = lo.view(np.uint16).reshape(-1, 2)
+    return out.reshape(-1)
+
+
+def from_storage(v: np.ndarray, fmt: int) -> np.ndarray:
+    v = np.asarray(v).view(np.uint16)
+    if fmt == FMT_F16:
+        h = v.view(np.float16).astype(np.float64)
+        return h[0::2] + 1j * h[1::2]
+    if fmt == FMT_BF16:
+        f = (v.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        return f[0::2] + 1j * f[1::2]
+    q = v.reshape(-1, 4).view(np.float16).astype(np.float64)
+    return (q[:, 0] + q[:, 2] / 2048.0) + 1j * (q[:, 1] + q[:, 3] / 2048.0)
+
+
+def exec_fft(x, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, ws_limit=0):
+    """Run the emulated plan; returns (complex128 output * 2^exponent, exponent)."""
+    src = to_storage(x, in_fmt)
+    n = int(np.prod(dims)) * batch
+    out = np.zeros(n * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    e = ctypes.c_int(0)
+    rc = lib().emu_exec(len(dims), d, batch, in_fmt, out_fmt, scaling, int(inverse), src.ctypes.data, out.ctypes.data,
+                        ws_limit, ctypes.byref(e))
+    if rc != 0:
+        raise RuntimeError(f"emu_exec status {rc}")
+    return from_storage(out, out_fmt) * (2.0 ** e.value), e.value
+
+
+def describe(dims, batch, in_fmt, out_fmt, scaling=1):
+    buf = ctypes.create_string_buffer(8192)
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    rc = lib().emu_plan_describe(len(dims), d, batch, in_fmt, out_fmt, scaling, buf, 8192)
+    if rc != 0:
+        raise RuntimeError(f"plan status {rc}")
+    return buf.value.decode()

@@ -1,0 +1,121 @@
+// emu_harness.cpp -- runs the library's planner and pass kernels on the CPU
+// (test infrastructure).  The kernel sources are compiled unchanged with
+// MCFFT_EMU: every LDS and global access is bounds-checked, so a wrong index
+// aborts here instead of faulting a GPU.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "fft_pass.h"
+#include "plan_core.h"
+
+using namespace mcfft;
+
+static void twiddle_host(const TwSpec& s, int sign, f32x2& out) {
+  long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)s.e / (long double)s.M;
+  long double c = cosl(a), sn = sinl(a);
+  if ((4 * s.e) % s.M == 0) {
+    long long q = ((4 * s.e) / s.M) % 4;
+    c = (q == 0) ? 1 : (q == 2) ? -1 : 0;
+    sn = (q == 1) ? 1 : (q == 3) ? -1 : 0;
+  }
+  out.x = (float)((double)c * s.scale);
+  out.y = (float)((double)(sign * sn) * s.scale);
+}
+
+static uint16_t f16b(double v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+static uint16_t bf16b(double v) { return __builtin_bit_cast(uint16_t, (__bf16)(float)v); }
+
+static void dft_table_host(int sign, uint16_t* t) {
+  for (int e = 0; e < 16; ++e) {
+    TwSpec s{e, 16, 1.f};
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * e / 16.0L;
+    double re = (double)cosl(a), im = (double)(sign * sinl(a));
+    if ((4 * e) % 16 == 0) {
+      int q = (4 * e) / 16;
+      re = (q == 0) ? 1 : (q == 2) ? -1 : 0;
+      im = sign * ((q == 1) ? 1 : (q == 3) ? -1 : 0);
+    }
+    (void)s;
+    uint16_t rh = f16b(re), ih = f16b(im);
+    t[e] = rh;
+    t[16 + e] = ih;
+    t[32 + e] = f16b((re - (double)__builtin_bit_cast(_Float16, rh)) * 2048.0);
+    t[48 + e] = f16b((im - (double)__builtin_bit_cast(_Float16, ih)) * 2048.0);
+    t[64 + e] = bf16b(re);
+    t[80 + e] = bf16b(im);
+  }
+}
+
+extern "C" int emu_plan_describe(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                 int scaling, char* buf, int buflen) {
+  PlanCore p;
+  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, 0);
+  if (st) return st;
+  std::string s = p.describe();
+  snprintf(buf, buflen, "%s", s.c_str());
+  return 0;
+}
+
+// Run a whole plan on host buffers.  ws_limit bounds the four-step workspace.
+extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
+                        int inverse, const void* in, void* out, long long ws_limit, int* out_exponent) {
+  PlanCore p;
+  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit);
+  if (st) return st;
+  if (out_exponent) *out_exponent = p.out_exponent;
+  const int sign = inverse ? 1 : -1;
+  std::vector<f32x2> tw(p.tw.size() + 1);
+  for (size_t i = 0; i < p.tw.size(); ++i) twiddle_host(p.tw[i], sign, tw[i]);
+  std::vector<uint16_t> dft(kDftTableEntries);
+  dft_table_host(sign, dft.data());
+  const int ws_fmt = (p.cm == 2) ? 2 : in_fmt;
+  std::vector<unsigned char> ws((size_t)(p.ws_elems * elem_bytes(ws_fmt)) + 16);
+  const size_t in_bytes = (size_t)(p.n * batch * elem_bytes(in_fmt));
+  const size_t out_bytes = (size_t)(p.n * batch * elem_bytes(out_fmt));
+  emu::clear_ranges();
+  emu::register_range(in, in_bytes);
+  emu::register_range(out, out_bytes);
+  emu::register_range(ws.data(), (size_t)(p.ws_elems * elem_bytes(ws_fmt)));
+  emu::register_range(tw.data(), p.tw.size() * sizeof(f32x2));
+  emu::register_range(dft.data(), dft.size() * 2);
+  for (long long c = 0; c < p.n_chunks; ++c) {
+    const long long t0 = c * p.chunk;
+    const long long nt = (t0 + p.chunk <= batch) ? p.chunk : batch - t0;
+    for (const PassPlan& pp : p.passes) {
+      PassArgs a;
+      memset(&a, 0, sizeof(a));
+      auto bufptr = [&](int b, int fmt) -> char* {
+        if (b == BUF_IN) return (char*)in + (size_t)(t0 * p.n * elem_bytes(fmt));
+        if (b == BUF_OUT) return (char*)out + (size_t)(t0 * p.n * elem_bytes(fmt));
+        return (char*)ws.data();
+      };
+      const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
+      a.in = bufptr(pp.src, pp.src_fmt);
+      a.out = bufptr(pp.dst, dst_fmt);
+      a.in_sq = pp.in_sq; a.in_sm = pp.in_sm; a.out_sq = pp.out_sq; a.out_sm = pp.out_sm;
+      a.in_sg = pp.in_sg; a.in_st = pp.in_st; a.out_sg = pp.out_sg; a.out_st = pp.out_st;
+      a.n_inst = pp.n_inst;
+      a.tiles_per_group = pp.tiles_per_group;
+      a.in_fmt = pp.src_fmt;
+      a.inverse = inverse;
+      a.out_scale = pp.out_scale;
+      a.stage_tw = tw.data() + pp.stage_tw_off;
+      a.dft = dft.data();
+      if (pp.fourstep) {
+        a.fs_hi = tw.data() + pp.fs_hi_off;
+        a.fs_lo = tw.data() + pp.fs_lo_off;
+        a.fs_shift = pp.fs_shift;
+        a.fs_mask = pp.fs_mask;
+        a.fs_nmask = pp.fs_nmask;
+      }
+      const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
+      const long long grid = groups * pp.tiles_per_group;
+      void (*fn)(PassArgs) = (void (*)(PassArgs))pp.k->fn;
+      emu::launch((int)grid, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); });
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,129 @@
+// emu_runtime.h -- CPU emulation of the device primitives used by the FFT
+// kernels (test infrastructure; included by mcfft_amd/csrc/dev_prims.h only
+// when MCFFT_EMU is defined).  Each workgroup runs as blockDim host threads;
+// LDS and global accesses are bounds-checked; the MFMA is evaluated from the
+// operand layout documented in dev_prims.h.
+#pragma once
+#include <barrier>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define MCFFT_DEV inline
+#define MCFFT_KERNEL(nt) extern "C"
+#define MCFFT_TKERNEL(nt)
+#define MCFFT_DYN_LDS(type, name) type* name = (type*)::emu::lds_base()
+#define __restrict__
+#define __device__
+#define __global__
+#define __host__
+
+namespace emu {
+struct Range { const char* lo; const char* hi; };
+inline std::vector<Range>& ranges() { static std::vector<Range> r; return r; }
+inline void register_range(const void* p, size_t bytes) { ranges().push_back({(const char*)p, (const char*)p + bytes}); }
+inline void clear_ranges() { ranges().clear(); }
+[[noreturn]] inline void fail(const char* what, long long i) {
+  fprintf(stderr, "EMU BOUNDS ERROR: %s index %lld\n", what, i);
+  fflush(stderr);
+  abort();
+}
+inline void check_global(const void* p, size_t bytes, long long i) {
+  const char* c = (const char*)p;
+  for (auto& r : ranges()) if (c >= r.lo && c + bytes <= r.hi) return;
+  fail("global", i);
+}
+
+struct Block {
+  int nt = 0, bid = 0;
+  std::vector<unsigned char> lds;
+  std::barrier<>* bar = nullptr;
+  std::vector<std::barrier<>*> wbar;
+  // per-wave MFMA exchange buffers: [wave][lane][8] for A and B
+  std::vector<float> xa, xb;
+};
+inline thread_local Block* tl_blk = nullptr;
+inline thread_local int tl_tid = 0;
+
+inline void* lds_base() { return tl_blk->lds.data(); }
+inline size_t lds_bytes() { return tl_blk->lds.size(); }
+
+// launch `body` as a grid of nblocks workgroups of nt threads with `lds` bytes of LDS
+inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>& body) {
+  for (int b = 0; b < nblocks; ++b) {
+    Block blk;
+    blk.nt = nt;
+    blk.bid = b;
+    blk.lds.assign(lds, 0xCD);
+    std::barrier<> bar(nt);
+    blk.bar = &bar;
+    int nw = nt / 64;
+    std::vector<std::barrier<>*> wb;
+    for (int w = 0; w < nw; ++w) wb.push_back(new std::barrier<>(64));
+    blk.wbar = wb;
+    blk.xa.assign((size_t)nw * 64 * 8, 0.f);
+    blk.xb.assign((size_t)nw * 64 * 8, 0.f);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t] { tl_blk = &blk; tl_tid = t; body(); });
+    for (auto& x : th) x.join();
+    for (auto* p : wb) delete p;
+  }
+}
+}  // namespace emu
+
+namespace mcfft {
+inline int lane_id() { return emu::tl_tid % 64; }
+inline int thread_id() { return emu::tl_tid; }
+inline int block_id() { return emu::tl_blk->bid; }
+inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
+
+template <class T> inline T lds_ld(const T* p, int i) {
+  const char* base = (const char*)emu::lds_base();
+  const char* a = (const char*)(p + i);
+  if (i < 0 || a < base || a + sizeof(T) > base + emu::lds_bytes()) emu::fail("lds load", i);
+  return p[i];
+}
+template <class T> inline void lds_st(T* p, int i, T v) {
+  const char* base = (const char*)emu::lds_base();
+  const char* a = (const char*)(p + i);
+  if (i < 0 || a < base || a + sizeof(T) > base + emu::lds_bytes()) emu::fail("lds store", i);
+  p[i] = v;
+}
+template <class T> inline T g_ld(const T* p, long long i) { emu::check_global(p + i, sizeof(T), i); return p[i]; }
+template <class T> inline void g_st(T* p, long long i, T v) { emu::check_global(p + i, sizeof(T), i); p[i] = v; }
+
+template <class Dec>
+inline f32x16 emu_mfma(u32x4 a, u32x4 b, f32x16 c, Dec dec) {
+  emu::Block* blk = emu::tl_blk;
+  int l = emu::tl_tid % 64, w = emu::tl_tid / 64;
+  float* xa = &blk->xa[(size_t)w * 512];
+  float* xb = &blk->xb[(size_t)w * 512];
+  for (int t = 0; t < 8; ++t) {
+    uint32_t wa = a[t / 2], wbv = b[t / 2];
+    xa[l * 8 + t] = dec((t & 1) ? (wa >> 16) : (wa & 0xffff));
+    xb[l * 8 + t] = dec((t & 1) ? (wbv >> 16) : (wbv & 0xffff));
+  }
+  blk->wbar[w]->arrive_and_wait();
+  f32x16 d;
+  int h = l / 32, col = l % 32;
+  for (int v = 0; v < 16; ++v) {
+    int row = 8 * (v / 4) + 4 * h + v % 4;
+    float s = c[v];
+    for (int k = 0; k < 16; ++k) s += xa[(row + 32 * (k / 8)) * 8 + k % 8] * xb[(col + 32 * (k / 8)) * 8 + k % 8];
+    d[v] = s;
+  }
+  blk->wbar[w]->arrive_and_wait();
+  return d;
+}
+inline f32x16 mfma_f16(u32x4 a, u32x4 b, f32x16 c) {
+  return emu_mfma(a, b, c, [](uint32_t x) { return (float)__builtin_bit_cast(_Float16, (uint16_t)x); });
+}
+inline f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 c) {
+  return emu_mfma(a, b, c, [](uint32_t x) { return (float)__builtin_bit_cast(__bf16, (uint16_t)x); });
+}
+}  // namespace mcfft
