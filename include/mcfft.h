@@ -1,0 +1,106 @@
+/*
+ * mcfft.h -- C ABI of the MI355X batched complex-to-complex FFT library.
+ *
+ * Plain pointers and sizes only: device buffers are HIP device pointers passed
+ * as void*, the stream is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Data layout: interleaved complex, transforms stored back to back, row-major
+ * dimensions with the last dimension contiguous ([b][x][y] for 2D, as in the
+ * reference's source/mcfft_2d_utils.cpp:49-50).  Input and output are in natural
+ * order; in-place (in == out) and out-of-place execution are both supported.
+ *
+ * Element formats:
+ *   MCFFT_FMT_FP16        4 B: binary16 re, binary16 im
+ *   MCFFT_FMT_BF16        4 B: bfloat16 re, bfloat16 im
+ *   MCFFT_FMT_FP16_SPLIT  8 B: binary16 (re_hi, im_hi, re_lo, im_lo) with
+ *                         value = hi + lo * 2^-11 (about 22 significant bits)
+ * Supported (input, output) pairs: FP16->FP16, BF16->BF16, FP16->SPLIT,
+ * SPLIT->SPLIT.  FP16/BF16 compute on MFMA with FP32 accumulation; SPLIT forms
+ * the compensated products hi*hi + 2^-11 (hi*lo + lo*hi) on FP16 MFMA.
+ *
+ * Scaling: each radix stage multiplies by a power of two chosen by the plan's
+ * scaling mode, so the stored result is  true_result * 2^-E  where E is
+ * reported by mcfft_get_output_exponent().  NONE keeps E = 0 (unnormalised,
+ * FFTW convention, may overflow FP16), BALANCED keeps the RMS of random data
+ * roughly constant (E ~ log2(N)/2), FULL divides by N (E = log2 N; never
+ * overflows for inputs that fit the format).
+ *
+ * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D sides
+ * 16..4096 (the number of columns an outer axis has must be a multiple of the
+ * column tile, 32 or less).
+ *
+ * Reference interface replaced: mcfftCreate / mcfftExec of
+ * include/mcfft_1d_device.h:37-38, include/mcfft_2d_device.h:37-38 and
+ * include/mcfft_3d_device.h:37-38 (void functions, digit-reversed in-place
+ * input, no status, null stream).  The reference's exact signatures are kept by
+ * the compatibility headers mcfft_{1,2,3}d_device.h / mcfft_{1,2,3}d_utils.h.
+ */
+#ifndef MCFFT_H
+#define MCFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mcfftStatus_t {
+  MCFFT_SUCCESS = 0,
+  MCFFT_INVALID_VALUE = 1,   /* bad size, batch, rank, pointer or enum */
+  MCFFT_UNSUPPORTED = 2,     /* valid request this build does not implement */
+  MCFFT_ALLOC_FAILED = 3,
+  MCFFT_HIP_ERROR = 4,       /* a HIP runtime call failed */
+  MCFFT_MISALIGNED = 5,      /* buffer not aligned to its element size */
+  MCFFT_INVALID_PLAN = 6,
+  MCFFT_CACHE_ERROR = 7      /* tuning cache unreadable or of another version */
+} mcfftStatus;
+
+typedef enum mcfftFormat_t {
+  MCFFT_FMT_FP16 = 0,
+  MCFFT_FMT_BF16 = 1,
+  MCFFT_FMT_FP16_SPLIT = 2
+} mcfftFormat;
+
+typedef enum mcfftDirection_t { MCFFT_FORWARD = -1, MCFFT_INVERSE = 1 } mcfftDirection;
+
+typedef enum mcfftScaling_t {
+  MCFFT_SCALE_NONE = 0,
+  MCFFT_SCALE_BALANCED = 1,
+  MCFFT_SCALE_FULL = 2
+} mcfftScaling;
+
+typedef struct mcfftPlan_st* mcfftPlan;
+
+/* Create a plan for `batch` transforms of shape dims[0..rank-1] (rank 1..3).
+ * Allocates the plan's twiddle tables (generated on the GPU in FP64) and, for
+ * 1D lengths above 4096, a workspace; both live on the current device.
+ * Replaces mcfftCreate (source/mcfft_1d_device.cpp:680-790). */
+mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                            mcfftFormat out_fmt, mcfftScaling scaling);
+/* Shorthands with MCFFT_SCALE_BALANCED. */
+mcfftStatus mcfft_plan_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt);
+mcfftStatus mcfft_plan_2d(mcfftPlan* plan, long long nx, long long ny, long long batch, mcfftFormat in_fmt,
+                          mcfftFormat out_fmt);
+mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long nz, long long batch,
+                          mcfftFormat in_fmt, mcfftFormat out_fmt);
+
+/* Execute on `stream` (hipStream_t or NULL); asynchronous, no host sync.
+ * direction: MCFFT_FORWARD (exp(-2 pi i kn/N)) or MCFFT_INVERSE (exp(+...),
+ * unnormalised apart from the plan's scaling).  Capturable into a hipGraph.
+ * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
+mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
+
+/* E such that true result = stored * 2^E (same for both directions). */
+mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent);
+/* Workspace the plan uses (bytes; 0 when none). */
+mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes);
+/* Human-readable pass list (for logs and tests). */
+mcfftStatus mcfft_plan_describe(mcfftPlan plan, char* buf, size_t len);
+mcfftStatus mcfft_destroy(mcfftPlan plan);
+const char* mcfft_status_string(mcfftStatus status);
+int mcfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCFFT_H */

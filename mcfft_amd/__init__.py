@@ -1,0 +1,186 @@
+"""Python host binding of the MI355X batched C2C FFT library (include/mcfft.h).
+
+The compute path is ``mcfft_amd/_lib/libmcfft.so`` (hand-written gfx950 HIP
+kernels behind a C ABI); this module only wraps plan/exec/destroy with ctypes
+and converts between complex arrays and the device storage formats.  There is
+no CPU fallback: if the shared library is missing or HIP fails, calls raise.
+
+Storage formats as torch tensors (last axis = components of one element):
+  "fp16"   torch.float16  (..., 2)  re, im
+  "bf16"   torch.bfloat16 (..., 2)  re, im
+  "split"  torch.float16  (..., 4)  re_hi, im_hi, re_lo, im_lo; value = hi + lo * 2^-11
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmcfft.so")
+
+FORMATS = {"fp16": 0, "bf16": 1, "split": 2}
+SCALINGS = {"none": 0, "balanced": 1, "full": 2}
+FORWARD, INVERSE = -1, 1
+STATUS = {0: "success", 1: "invalid value", 2: "unsupported", 3: "allocation failed", 4: "HIP error",
+          5: "misaligned buffer", 6: "invalid plan", 7: "tuning cache error"}
+
+_lib = None
+
+
+class McfftError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(status, status)} (status {status})")
+        self.status = status
+
+
+def lib() -> ctypes.CDLL:
+    """Load libmcfft.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ll, i = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int
+        L.mcfft_plan_many.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(ll), ll, i, i, i]
+        L.mcfft_plan_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i]
+        L.mcfft_plan_2d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, i, i]
+        L.mcfft_plan_3d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, ll, i, i]
+        L.mcfft_exec.argtypes = [vp, vp, vp, i, vp]
+        L.mcfft_get_output_exponent.argtypes = [vp, ctypes.POINTER(i)]
+        L.mcfft_get_workspace_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.mcfft_plan_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.mcfft_destroy.argtypes = [vp]
+        L.mcfft_status_string.argtypes = [i]
+        L.mcfft_status_string.restype = ctypes.c_char_p
+        L.mcfft_version.restype = i
+        for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
+                  "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
+            getattr(L, f).restype = i
+        _lib = L
+    return _lib
+
+
+def _check(st: int, what: str):
+    if st != 0:
+        raise McfftError(st, what)
+
+
+class Plan:
+    """A plan for `batch` transforms of shape `dims` (1 to 3 dims, last contiguous)."""
+
+    def __init__(self, dims, batch: int, in_fmt: str = "fp16", out_fmt: str | None = None,
+                 scaling: str = "balanced"):
+        dims = [int(d) for d in (dims if hasattr(dims, "__len__") else [dims])]
+        out_fmt = out_fmt or in_fmt
+        self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = dims, int(batch), in_fmt, out_fmt, scaling
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_longlong * len(dims))(*dims)
+        _check(lib().mcfft_plan_many(ctypes.byref(self._h), len(dims), arr, self.batch, FORMATS[in_fmt],
+                                     FORMATS[out_fmt], SCALINGS[scaling]), "mcfft_plan_many")
+        e = ctypes.c_int()
+        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
+        self.exponent = e.value
+        self.n = int(np.prod(dims))
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def workspace_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        _check(lib().mcfft_get_workspace_size(self._h, ctypes.byref(b)), "mcfft_get_workspace_size")
+        return b.value
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(16384)
+        _check(lib().mcfft_plan_describe(self._h, buf, 16384), "mcfft_plan_describe")
+        return buf.value.decode()
+
+    def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        _check(lib().mcfft_exec(self._h, in_ptr, out_ptr, direction, stream), "mcfft_exec")
+
+    def exec(self, inp, out, direction: int = FORWARD, stream=None):
+        """Run on torch device tensors (storage layout above) on `stream`
+        (default: torch's current stream)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(inp.device).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream)
+
+    def destroy(self):
+        if self._h and self._h.value:
+            lib().mcfft_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def components(fmt: str) -> int:
+    return 4 if fmt == "split" else 2
+
+
+def encode(x: np.ndarray, fmt: str) -> np.ndarray:
+    """complex array -> host storage array (float16 / bfloat16-as-uint16 / split)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+    if fmt == "fp16":
+        v = np.empty((x.size, 2), dtype=np.float16)
+        v[:, 0], v[:, 1] = x.real, x.imag
+        return v
+    if fmt == "bf16":
+        f = np.empty((x.size, 2), dtype=np.float32)
+        f[:, 0], f[:, 1] = x.real, x.imag
+        u = f.view(np.uint32)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    hi = encode(x, "fp16")
+    out = np.empty((x.size, 4), dtype=np.float16)
+    out[:, 0:2] = hi
+    out[:, 2] = (x.real - hi[:, 0].astype(np.float64)) * 2048.0
+    out[:, 3] = (x.imag - hi[:, 1].astype(np.float64)) * 2048.0
+    return out
+
+
+def decode(v: np.ndarray, fmt: str, exponent: int = 0) -> np.ndarray:
+    """host storage array -> complex128, times 2^exponent."""
+    s = 2.0 ** exponent
+    if fmt == "fp16":
+        h = np.asarray(v).view(np.float16).reshape(-1, 2).astype(np.float64)
+        return (h[:, 0] + 1j * h[:, 1]) * s
+    if fmt == "bf16":
+        u = np.asarray(v).view(np.uint16).reshape(-1, 2).astype(np.uint32) << 16
+        f = u.view(np.float32).astype(np.float64)
+        return (f[:, 0] + 1j * f[:, 1]) * s
+    q = np.asarray(v).view(np.float16).reshape(-1, 4).astype(np.float64)
+    return ((q[:, 0] + q[:, 2] / 2048.0) + 1j * (q[:, 1] + q[:, 3] / 2048.0)) * s
+
+
+def to_device(x: np.ndarray, fmt: str, device="cuda"):
+    """complex array -> torch device tensor in storage format."""
+    import torch
+    h = encode(x, fmt)
+    if fmt == "bf16":
+        return torch.from_numpy(h.view(np.int16)).to(device).view(torch.bfloat16)
+    return torch.from_numpy(h).to(device)
+
+
+def from_device(t, fmt: str, exponent: int = 0) -> np.ndarray:
+    import torch
+    if fmt == "bf16":
+        h = t.view(torch.int16).cpu().numpy().view(np.uint16)
+    else:
+        h = t.cpu().numpy()
+    return decode(h, fmt, exponent)
+
+
+def empty(n_elems: int, fmt: str, device="cuda"):
+    import torch
+    dt = torch.bfloat16 if fmt == "bf16" else torch.float16
+    return torch.empty((n_elems, components(fmt)), dtype=dt, device=device)

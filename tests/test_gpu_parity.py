@@ -1,0 +1,128 @@
+"""GPU parity: libmcfft.so (through its C ABI) against the FP64 oracle.
+
+Tolerances (rel-L2 against an FP64 FFT of the same quantised input):
+  fp16 plain   <= 1.5e-3 a-priori bound (~2^-11 per stage), and no worse than the
+               reference algorithm emulated at FP16 where the reference has a plan
+  bf16 plain   <= 1.2e-2 (~2^-8 per stage)
+  fp16 split   <= 1e-6
+"""
+import numpy as np
+import pytest
+
+import mcfft_amd as mc
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp16": 1.5e-3, "bf16": 1.2e-2, "split": 1e-6}
+
+
+def _input(n_total, seed, kind="normal"):
+    rng = np.random.default_rng(seed)
+    if kind == "uniform":
+        return rng.uniform(-1, 1, n_total) + 1j * rng.uniform(-1, 1, n_total)
+    return rng.standard_normal(n_total) + 1j * rng.standard_normal(n_total)
+
+
+def _quantised(x, in_fmt):
+    return mc.decode(mc.encode(x, in_fmt), in_fmt)
+
+
+def run(dims, batch, in_fmt, out_fmt, x, direction=mc.FORWARD, scaling="balanced", inplace=False):
+    import torch
+    p = mc.Plan(dims, batch, in_fmt, out_fmt, scaling)
+    din = mc.to_device(x, in_fmt)
+    if inplace:
+        assert in_fmt == out_fmt
+        dout = din
+    else:
+        dout = mc.empty(x.size, out_fmt)
+    p.exec(din, dout, direction)
+    torch.cuda.synchronize()
+    y = mc.from_device(dout, out_fmt, p.exponent)
+    p.destroy()
+    return y
+
+
+CASES_1D = [(16, 64), (32, 33), (64, 16), (128, 8), (256, 4096), (512, 16), (1024, 8), (2048, 4), (4096, 3),
+            (8192, 2), (65536, 2), (131072, 2), (1 << 20, 1)]
+
+
+@pytest.mark.parametrize("n,batch", CASES_1D)
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_1d_forward(torch_cuda, n, batch, fmt):
+    in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else (fmt, fmt)
+    x = _input(n * batch, seed=n + batch)
+    xq = _quantised(x, in_fmt)
+    y = run([n], batch, in_fmt, out_fmt, x)
+    ref = oracle.fft(xq, (n,))
+    err = oracle.rel_l2(y, ref)
+    print(f"1d n={n} b={batch} {fmt} rel_l2={err:.3e}")
+    assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("dims,batch", [((64, 32), 3), ((256, 256), 2), ((1024, 512), 1), ((4096, 4096), 1),
+                                        ((16, 16, 32), 2), ((64, 32, 128), 1), ((128, 128, 128), 1)])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_nd_forward(torch_cuda, dims, batch, fmt):
+    in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else (fmt, fmt)
+    n = int(np.prod(dims))
+    x = _input(n * batch, seed=n)
+    xq = _quantised(x, in_fmt)
+    y = run(list(dims), batch, in_fmt, out_fmt, x)
+    ref = oracle.fft(xq, dims)
+    err = oracle.rel_l2(y, ref)
+    print(f"nd dims={dims} b={batch} {fmt} rel_l2={err:.3e}")
+    assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("dims,batch", [((256,), 8), ((4096,), 2), ((131072,), 1), ((64, 128), 2), ((32, 32, 32), 1)])
+def test_inverse_split(torch_cuda, dims, batch):
+    n = int(np.prod(dims))
+    x = _input(n * batch, seed=7)
+    xq = _quantised(x, "fp16")
+    y = run(list(dims), batch, "fp16", "split", x, direction=mc.INVERSE)
+    ref = oracle.fft(xq, dims, sign=+1)
+    assert oracle.rel_l2(y, ref) <= TOL["split"]
+
+
+def test_inplace_matches_out_of_place(torch_cuda):
+    x = _input(65536 * 4, seed=3)
+    a = run([65536], 4, "fp16", "fp16", x)
+    b = run([65536], 4, "fp16", "fp16", x, inplace=True)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 131072])
+def test_plain_no_worse_than_reference_fp16(torch_cuda, n):
+    """One-sided bound: plain FP16 is at least as accurate as the reference's
+    algorithm run at MCTYPE=half (emulated by the oracle), on the same inputs."""
+    batch = max(1, 4096 // n)
+    x = _input(n * batch, seed=11)
+    xq = _quantised(x, "fp16")
+    ref = oracle.fft(xq, (n,))
+    y = run([n], batch, "fp16", "fp16", x)
+    ours = oracle.rel_l2(y, ref)
+    theirs = oracle.rel_l2(oracle.ref_fp16_exec(xq, n), ref)
+    print(f"n={n} ours={ours:.3e} reference-fp16={theirs:.3e}")
+    assert ours <= theirs
+
+
+def test_split_roundtrip_c2_full_size(torch_cuda):
+    """1D N=131072 x 256 forward then inverse (split): inv(fwd(x)) = N x."""
+    import torch
+    n, b = 131072, 256
+    x = _input(n * b, seed=2)
+    xq = _quantised(x, "fp16")
+    pf = mc.Plan([n], b, "fp16", "split")
+    pi = mc.Plan([n], b, "split", "split")
+    d0 = mc.to_device(x, "fp16")
+    d1 = mc.empty(n * b, "split")
+    d2 = mc.empty(n * b, "split")
+    pf.exec(d0, d1, mc.FORWARD)
+    pi.exec(d1, d2, mc.INVERSE)
+    torch.cuda.synchronize()
+    y = mc.from_device(d2, "split", pf.exponent + pi.exponent) / n
+    err = oracle.rel_l2(y, xq)
+    print(f"roundtrip rel_l2={err:.3e}")
+    assert err <= 2e-6
