@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched C2C FFT hot path on one MI355X.
+
+Workload (one "step"): 1D complex-to-complex FFT, N = 131072 (16^4 x 2, the
+radix-2 tail case), batch 256, forward then inverse, interleaved complex FP16,
+re/im iid N(0,1) from a seed -- plain FP16 (the `value` line) and split FP16
+hi+lo (reported beside it), each with its rel-L2 / max-abs error against an
+FP64 FFT of the same quantised input.
+
+`value` is GFLOP/s with the 5 N log2 N convention over the forward + inverse
+transforms of the whole batch, inputs resident in HBM.  Inputs rotate over
+several buffers (> 2x the 256 MB Infinity Cache) so the timed region reads HBM.
+
+`roofline` is for the dominant kernel of the plain path: algorithmic bytes per
+launch (8 B per complex element: one FP16 read + one FP16 write, for every
+element the launch covers) / its average duration, measured with HIP events on
+the stream the kernels run on.  `cpu_baseline` times the FP64 CPU oracle on the
+host cores (all OpenMP threads) on the same workload shape.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, BATCH = 131072, 256
+HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (scales rocprofv3 byte counters)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel_name: str):
+    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 PMC
+    summary (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        e = d.get("kernels", {}).get(kernel_name)
+        return None if e is None else float(e["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU scaling is not part "
+              "of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import mcfft_amd as mc
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": a.numel() * 4, "launches": 5}))
+        return
+    flops_step = 2 * 5.0 * N * math.log2(N) * BATCH   # forward + inverse
+    elems = N * BATCH
+
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(elems) + 1j * rng.standard_normal(elems)
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    # rotate inputs: each copy 128 MiB; together > 2x the Infinity Cache
+    n_rot = max(2, (2 * MALL_BYTES) // (elems * 4) + 1)
+    base = mc.to_device(x, "fp16")
+    ins = [base] + [base.clone() for _ in range(n_rot - 1)]
+    mid = mc.empty(elems, "fp16")
+    out = mc.empty(elems, "fp16")
+
+    pf = mc.Plan([N], BATCH, "fp16", "fp16")
+    stream = torch.cuda.current_stream()
+
+    def step(i):
+        pf.exec(ins[i % n_rot], mid, mc.FORWARD)
+        pf.exec(mid, out, mc.INVERSE)
+
+    def timed(fn, steps, warmup):
+        for i in range(warmup):
+            fn(i)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(steps):
+            fn(i)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_plain = timed(step, args.steps, args.warmup)
+
+    # errors (plain): forward output of the first 8 transforms vs FP64, and the round trip
+    pf.exec(ins[0], mid, mc.FORWARD)
+    pf.exec(mid, out, mc.INVERSE)
+    torch.cuda.synchronize()
+    import oracle
+    k = 8 * N
+    y_f = mc.from_device(mid[:k], "fp16", pf.exponent)
+    ref_f = oracle.fft(xq[:k], (N,))
+    y_rt = mc.from_device(out[:k], "fp16", 2 * pf.exponent) / N
+    plain_err = {"rel_l2_fwd": oracle.rel_l2(y_f, ref_f), "max_abs_fwd": oracle.max_abs(y_f, ref_f),
+                 "rel_l2_roundtrip": oracle.rel_l2(y_rt, xq[:k]), "checked_transforms": 8}
+
+    # per-kernel durations of the plain path (HIP events on the launch stream)
+    n_pass = pf.pass_count()
+    desc = pf.describe().strip().splitlines()
+    kernels = []
+    for p_i in range(n_pass):
+        def one(i, p_i=p_i):
+            pf.exec_passes(ins[i % n_rot], mid, mc.FORWARD, p_i, 1)
+        ms_k = timed(one, args.steps, 2)
+        name = desc[1 + p_i].split()[1]
+        alg_bytes = 8.0 * elems
+        kernels.append({"pass": p_i, "kernel": name, "ms": ms_k, "alg_gbs": alg_bytes / (ms_k * 1e-3) / 1e9})
+    dom = max(kernels, key=lambda r: r["ms"])
+    traffic = traffic_from_profiles(dom["kernel"])
+    roofline = {"bound": "hbm", "achieved": round(dom["alg_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["alg_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 4),
+                "alg_bytes_per_launch": 8 * elems}
+
+    # split precision leg: forward FP16 -> split, inverse split -> split
+    split = None
+    if not args.no_split:
+        ps_f = mc.Plan([N], BATCH, "fp16", "split")
+        ps_i = mc.Plan([N], BATCH, "split", "split")
+        smid = mc.empty(elems, "split")
+        sout = mc.empty(elems, "split")
+
+        def sstep(i):
+            ps_f.exec(ins[i % n_rot], smid, mc.FORWARD)
+            ps_i.exec(smid, sout, mc.INVERSE)
+
+        ms_split = timed(sstep, args.steps, args.warmup)
+        ps_f.exec(ins[0], smid, mc.FORWARD)
+        ps_i.exec(smid, sout, mc.INVERSE)
+        torch.cuda.synchronize()
+        ys = mc.from_device(smid[:k], "split", ps_f.exponent)
+        ys_rt = mc.from_device(sout[:k], "split", ps_f.exponent + ps_i.exponent) / N
+        split_bytes = elems * (12 + 16)   # fwd FP16 in + hi/lo out, inv hi/lo in + out
+        split = {"value": round(flops_step / (ms_split * 1e-3) / 1e9, 1), "ms_per_step": round(ms_split, 4),
+                 "hbm_frac_job": round(split_bytes / (ms_split * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "rel_l2_fwd": oracle.rel_l2(ys, ref_f), "max_abs_fwd": oracle.max_abs(ys, ref_f),
+                 "rel_l2_roundtrip": oracle.rel_l2(ys_rt, xq[:k])}
+        del smid, sout
+        ps_f.destroy()
+        ps_i.destroy()
+
+    # CPU baseline: the FP64 oracle on the host cores, same shape, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        n_tr = 16
+        sample = x[: n_tr * N]
+        t0 = time.perf_counter()
+        yy = oracle.fft(sample, (N,))
+        oracle.fft(yy, (N,), sign=+1)
+        t1 = time.perf_counter() - t0
+        reps = max(1, int(args.cpu_seconds / max(t1, 1e-3)))
+        reps = min(reps, 64)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            yy = oracle.fft(sample, (N,))
+            oracle.fft(yy, (N,), sign=+1)
+        tc = (time.perf_counter() - t0) / reps
+        cpu_gflops = 2 * 5.0 * N * math.log2(N) * n_tr / tc / 1e9
+        cpu = {"value": round(cpu_gflops, 2), "unit": "GFLOP/s", "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"FP64 forward+inverse of {n_tr} of the {BATCH} transforms (N={N}), {reps} repetitions, "
+                         f"{reps * tc:.1f} s"}
+
+    value = flops_step / (ms_plain * 1e-3) / 1e9
+    job_bytes = 2 * 8.0 * elems
+    line = {
+        "metric": "C2C GFLOP/s (5NlogN) & %HBM roofline, split vs plain FP16, rel-L2 vs FP64",
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_plain, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16",
+        "data": "synthetic",
+        "config": {"workload": "1D C2C forward+inverse, N=131072 (16^4 x 2), batch 256, interleaved complex FP16, "
+                               "re/im iid N(0,1) seed 2",
+                   "n": N, "batch": BATCH, "precision": "plain fp16 (value); split fp16 hi+lo in 'split'"},
+        "hbm_frac_job": round(job_bytes / (ms_plain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "plain": plain_err,
+        "split": split,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,14 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
  * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
 
+/* Profiling entry points (no reference counterpart): the number of kernel
+ * passes of a plan, and an exec that launches only passes [first, first+count)
+ * (for every batch chunk), so a caller can time each kernel on its own stream.
+ * Running a subset of passes leaves `out` incomplete. */
+mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count);
+mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
+                              int count);
+
 /* E such that true result = stored * 2^E (same for both directions). */
 mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent);
 /* Workspace the plan uses (bytes; 0 when none). */

@@ -49,6 +49,8 @@ def lib() -> ctypes.CDLL:
         L.mcfft_plan_2d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, i, i]
         L.mcfft_plan_3d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, ll, i, i]
         L.mcfft_exec.argtypes = [vp, vp, vp, i, vp]
+        L.mcfft_exec_passes.argtypes = [vp, vp, vp, i, vp, i, i]
+        L.mcfft_get_pass_count.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_output_exponent.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_workspace_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.mcfft_plan_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
@@ -57,6 +59,7 @@ def lib() -> ctypes.CDLL:
         L.mcfft_status_string.restype = ctypes.c_char_p
         L.mcfft_version.restype = i
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
+                  "mcfft_exec_passes", "mcfft_get_pass_count",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -111,6 +114,19 @@ class Plan:
         elif hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream)
+
+    def pass_count(self) -> int:
+        c = ctypes.c_int()
+        _check(lib().mcfft_get_pass_count(self._h, ctypes.byref(c)), "mcfft_get_pass_count")
+        return c.value
+
+    def exec_passes(self, inp, out, direction: int, first: int, count: int, stream=None):
+        """Launch only passes [first, first+count) (profiling; leaves `out` incomplete)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(inp.device).cuda_stream
+        _check(lib().mcfft_exec_passes(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, first, count),
+               "mcfft_exec_passes")
 
     def destroy(self):
         if self._h and self._h.value:

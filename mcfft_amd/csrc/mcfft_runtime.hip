@@ -156,6 +156,20 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
 
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
+  return mcfft_exec_passes(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
+}
+
+mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count) {
+  if (!plan) return MCFFT_INVALID_PLAN;
+  if (!count) return MCFFT_INVALID_VALUE;
+  *count = (int)plan->core.passes.size();
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
+                              int count) {
+  if (!plan) return MCFFT_INVALID_PLAN;
+  if (first < 0 || count < 0 || first + count > (int)plan->core.passes.size()) return MCFFT_INVALID_VALUE;
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
   const PlanCore& c = plan->core;
@@ -168,7 +182,8 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
   for (long long ch = 0; ch < c.n_chunks; ++ch) {
     const long long t0 = ch * c.chunk;
     const long long nt = (t0 + c.chunk <= c.batch) ? c.chunk : c.batch - t0;
-    for (const PassPlan& pp : c.passes) {
+    for (int pi = first; pi < first + count; ++pi) {
+      const PassPlan& pp = c.passes[pi];
       PassArgs a;
       memset(&a, 0, sizeof(a));
       auto bufptr = [&](int b, int fmt) -> char* {

@@ -1,0 +1,80 @@
+"""The library's kernels and planner, compiled for the CPU (tests/emu), against
+the FP64 oracle -- CPU only.  Every LDS and global access of the emulated
+kernels is bounds-checked (an out-of-range index aborts the test process), and
+the MFMA is evaluated from the operand layout the GPU layout test pins."""
+import numpy as np
+import pytest
+
+import emu
+import oracle
+
+TOL = {emu.FMT_F16: 1.5e-3, emu.FMT_BF16: 1.2e-2, emu.FMT_SPLIT: 1e-6}
+FMTS = [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_BF16, emu.FMT_BF16), (emu.FMT_F16, emu.FMT_SPLIT)]
+
+
+def check(dims, batch, fi, fo, inverse=False, scaling=1, ws_limit=0, seed=0):
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
+    xq = emu.from_storage(emu.to_storage(x, fi), fi)
+    y, e = emu.exec_fft(x, list(dims), batch, fi, fo, scaling=scaling, inverse=inverse, ws_limit=ws_limit)
+    ref = oracle.fft(xq, tuple(dims), sign=+1 if inverse else -1)
+    err = oracle.rel_l2(y, ref)
+    assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (dims, batch, fi, fo, err)
+    return err
+
+
+@pytest.mark.parametrize("n,batch", [(16, 128), (32, 64), (64, 32), (128, 16), (256, 8), (512, 4), (1024, 2),
+                                     (2048, 1), (4096, 1), (256, 5), (16, 3)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_single_pass(n, batch, fi, fo):
+    check([n], batch, fi, fo)
+
+
+@pytest.mark.parametrize("n,batch,ws", [(8192, 1, 0), (8192, 3, 8192 * 8), (65536, 1, 0)])
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_four_step(n, batch, ws, fi, fo):
+    check([n], batch, fi, fo, ws_limit=ws)
+
+
+@pytest.mark.parametrize("dims,batch", [((64, 32), 2), ((16, 256), 1), ((256, 64), 1), ((16, 16, 32), 1),
+                                        ((32, 64, 32), 1)])
+def test_multi_dim(dims, batch):
+    check(dims, batch, emu.FMT_F16, emu.FMT_F16)
+    check(dims, batch, emu.FMT_F16, emu.FMT_SPLIT)
+
+
+@pytest.mark.parametrize("dims", [(256,), (512,), (8192,), (64, 64)])
+def test_inverse(dims):
+    check(dims, 1, emu.FMT_F16, emu.FMT_SPLIT, inverse=True)
+
+
+@pytest.mark.parametrize("scaling", [0, 1, 2])
+def test_scaling_modes(scaling):
+    check([1024], 2, emu.FMT_F16, emu.FMT_SPLIT, scaling=scaling)
+    check([8192], 1, emu.FMT_F16, emu.FMT_F16, scaling=scaling)
+
+
+def test_split_input_roundtrip():
+    n = 2048
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    xq = emu.from_storage(emu.to_storage(x, emu.FMT_F16), emu.FMT_F16)
+    y, e1 = emu.exec_fft(xq, [n], 1, emu.FMT_F16, emu.FMT_SPLIT)
+    z, e2 = emu.exec_fft(y, [n], 1, emu.FMT_SPLIT, emu.FMT_SPLIT, inverse=True)
+    assert oracle.rel_l2(z / n, xq) < 1e-6
+
+
+def test_plan_describe_and_errors():
+    d = emu.describe([131072], 256, emu.FMT_F16, emu.FMT_F16)
+    assert "fourstep" in d and d.count("pass ") == 2
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([1000], 1, 0, 0)        # not a power of two
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([8], 1, 0, 0)           # below 16
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([16, 8192], 1, 0, 0)    # side above 4096
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([256], 1, 2, 0)         # split in, fp16 out
+    with pytest.raises(RuntimeError, match="status 1"):
+        emu.describe([256], 0, 0, 0)         # batch 0
