@@ -11,7 +11,9 @@
 #include <stdint.h>
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -30,12 +32,30 @@ namespace mcfft {
 MCFFT_DEV int lane_id() { return __lane_id(); }
 MCFFT_DEV int thread_id() { return threadIdx.x; }
 MCFFT_DEV int block_id() { return blockIdx.x; }
+MCFFT_DEV int grid_size() { return gridDim.x; }
 MCFFT_DEV void block_barrier() { __syncthreads(); }
 
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
 template <class T> MCFFT_DEV void lds_st(T* p, int i, T v) { p[i] = v; }
 template <class T> MCFFT_DEV T g_ld(const T* p, long long i) { return p[i]; }
 template <class T> MCFFT_DEV void g_st(T* p, long long i, T v) { p[i] = v; }
+
+// Buffer-resource addressing: a 64-bit base in scalar registers plus 32-bit byte
+// offsets per lane (no 64-bit address arithmetic per access).  Loads outside
+// [0, bytes) return 0 and stores there are dropped by the hardware.
+struct GBuf {
+  __amdgpu_buffer_rsrc_t r;
+};
+MCFFT_DEV GBuf make_gbuf(const void* p, unsigned bytes) {
+  GBuf b;
+  b.r = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+  return b;
+}
+MCFFT_DEV uint32_t buf_ld32(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_load_b32(b.r, off, 0, 0); }
+MCFFT_DEV u32x2 buf_ld64(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_load_b64(b.r, off, 0, 0); }
+MCFFT_DEV u32x4 buf_ld128(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_load_b128(b.r, off, 0, 0); }
+MCFFT_DEV void buf_st32(GBuf b, int off, uint32_t v) { __builtin_amdgcn_raw_buffer_store_b32(v, b.r, off, 0, 0); }
+MCFFT_DEV void buf_st64(GBuf b, int off, u32x2 v) { __builtin_amdgcn_raw_buffer_store_b64(v, b.r, off, 0, 0); }
 
 // D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
 // Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and

@@ -41,23 +41,33 @@ enum { FMT_F16 = 0, FMT_BF16 = 1, FMT_SPLIT = 2 }; // global element formats
 constexpr float kSplitLoScale = 2048.0f;
 constexpr float kSplitLoInv = 1.0f / 2048.0f;
 
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
+
+// Element (q, m) of a tile (instance q, position m) is at, in elements,
+//   ROW : in/out  base + q*L + m                  (contiguous rows)
+//   ROWT: in      base + q*L + m,  out  base + q + m*stride_out  (transposed store)
+//   COL, COLF: in base + q + m*stride_in, out base + q + m*stride_out (columns)
+// with base = group*sg + tile*st for the tile's group/tile indices.
 struct PassArgs {
   const void* in;
   void* out;
-  long long in_sq, in_sm, out_sq, out_sm;   // element strides: instance, position
-  long long in_sg, in_st, out_sg, out_st;   // block -> base: group stride, tile stride
-  long long n_inst;           // valid instances per group (partial last tile)
+  long long in_sg, in_st, out_sg, out_st;   // tile -> base: group stride, tile stride
+  int stride_in, stride_out;  // runtime strides of the strided axis (elements)
+  int n_inst;                 // valid instances per group (partial last tile)
   int tiles_per_group;
+  int n_tiles;                // total tiles; workgroups loop over them
   int in_fmt;                 // split mode only: FMT_F16 or FMT_SPLIT input
   int inverse;                // 0 forward (exp(-i...)), 1 inverse
   float out_scale;            // last-stage scale when there is no four-step table
-  const f32x2* stage_tw;      // (nstages-1) post-twiddle tables of L entries each
-  const uint16_t* dft;        // W16^e tables for this direction (see DftTable)
+  const f32x2* stage_tw;      // post-twiddle tables [stage][j][16] (see plan_core)
+  const u32x4* aops;          // DFT-matrix MFMA operands [64 lanes][4]: A1, A2, A1lo, A2lo
   const f32x2* fs_hi;         // four-step twiddle W_N^e = fs_hi[e>>fs_shift] * fs_lo[e&fs_mask]
   const f32x2* fs_lo;
   int fs_shift;
   unsigned fs_mask;
-  unsigned long long fs_nmask;  // N-1 (N a power of two)
+  unsigned fs_nmask;          // N-1 (N a power of two, N <= 2^31)
+  long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
+  int stage_tw_entries, fs_hi_entries, fs_lo_entries;
 };
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:
@@ -74,46 +84,45 @@ MCFFT_DEV f32x2 cmul(f32x2 a, f32x2 b) {
 }
 
 template <int CM> struct Traits;
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+// RNE conversions of a pair (one v_cvt_pk_{f16,bf16}_f32 on gfx950)
+MCFFT_DEV uint32_t pk_f16(f32x2 v) { return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2)); }
+MCFFT_DEV uint32_t pk_bf16(f32x2 v) { return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2)); }
+MCFFT_DEV f32x2 up_f16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(f16x2, e), f32x2); }
+MCFFT_DEV f32x2 up_bf16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(bf16x2, e), f32x2); }
+
 template <> struct Traits<CM_F16> {
   typedef uint32_t T;
-  static MCFFT_DEV T pack(f32x2 v) { return pack2(f16_bits(v.x), f16_bits(v.y)); }
-  static MCFFT_DEV f32x2 unpack(T e) { f32x2 r; r.x = f16_val(e & 0xffffu); r.y = f16_val(e >> 16); return r; }
-  static MCFFT_DEV T load(const void* p, long long i, int) { return g_ld((const uint32_t*)p, i); }
-  static MCFFT_DEV void store(void* p, long long i, T v) { g_st((uint32_t*)p, i, v); }
+  static MCFFT_DEV T pack(f32x2 v) { return pk_f16(v); }
+  static MCFFT_DEV f32x2 unpack(T e) { return up_f16(e); }
+  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
 };
 template <> struct Traits<CM_BF16> {
   typedef uint32_t T;
-  static MCFFT_DEV T pack(f32x2 v) { return pack2(bf16_bits(v.x), bf16_bits(v.y)); }
-  static MCFFT_DEV f32x2 unpack(T e) { f32x2 r; r.x = bf16_val(e & 0xffffu); r.y = bf16_val(e >> 16); return r; }
-  static MCFFT_DEV T load(const void* p, long long i, int) { return g_ld((const uint32_t*)p, i); }
-  static MCFFT_DEV void store(void* p, long long i, T v) { g_st((uint32_t*)p, i, v); }
+  static MCFFT_DEV T pack(f32x2 v) { return pk_bf16(v); }
+  static MCFFT_DEV f32x2 unpack(T e) { return up_bf16(e); }
+  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
 };
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 template <> struct Traits<CM_SPLIT> {
   typedef u32x2 T;  // .x = hi pair (re, im), .y = lo pair
   static MCFFT_DEV T pack(f32x2 v) {
-    uint32_t hr = f16_bits(v.x), hi = f16_bits(v.y);
-    uint32_t lr = f16_bits((v.x - f16_val(hr)) * kSplitLoScale);
-    uint32_t li = f16_bits((v.y - f16_val(hi)) * kSplitLoScale);
     T e;
-    e.x = pack2(hr, hi);
-    e.y = pack2(lr, li);
+    e.x = pk_f16(v);
+    e.y = pk_f16((v - up_f16(e.x)) * kSplitLoScale);
     return e;
   }
-  static MCFFT_DEV f32x2 unpack(T e) {
-    f32x2 r;
-    r.x = __builtin_fmaf(f16_val(e.y & 0xffffu), kSplitLoInv, f16_val(e.x & 0xffffu));
-    r.y = __builtin_fmaf(f16_val(e.y >> 16), kSplitLoInv, f16_val(e.x >> 16));
-    return r;
-  }
-  static MCFFT_DEV T load(const void* p, long long i, int fmt) {
-    if (fmt == FMT_SPLIT) return g_ld((const u32x2*)p, i);
+  static MCFFT_DEV f32x2 unpack(T e) { return up_f16(e.y) * kSplitLoInv + up_f16(e.x); }
+  static MCFFT_DEV T load(GBuf b, int i, int fmt) {
+    if (fmt == FMT_SPLIT) return buf_ld64(b, i * 8);
     T e;
-    e.x = g_ld((const uint32_t*)p, i);
+    e.x = buf_ld32(b, i * 4);
     e.y = 0u;
     return e;
   }
-  static MCFFT_DEV void store(void* p, long long i, T v) { g_st((u32x2*)p, i, v); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st64(b, i * 8, v); }
 };
 
 // DFT-matrix operands of one lane: A1 = [Fr; Fi], A2 = [-Fi; Fr] (hi and lo parts)
@@ -121,34 +130,37 @@ struct AOps {
   u32x4 a1, a2, a1lo, a2lo;
 };
 
-template <int CM>
-MCFFT_DEV AOps load_dft_operands(const uint16_t* tab, int lane) {
-  AOps A;
+// Build lane `lane`'s 16 operand words (A1, A2, A1lo, A2lo; 4 words each) from a
+// W16 table (layout above).  Lane l holds A[l%32][8*(l/32)+t], t = 0..7, two
+// halves per word; row i < 16 is the real part of DFT row i, row i >= 16 the
+// imaginary part of row i-16.
+template <class Fn>
+MCFFT_DEV void aops_words(const uint16_t* tab, int cm, int lane, Fn&& put) {
   const int i = lane & 31, h = lane >> 5, row = i & 15;
   const bool top = i < 16;
-  const int ore = (CM == CM_BF16) ? 64 : 0, oim = (CM == CM_BF16) ? 80 : 16;
-  uint32_t w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0}, w1l[4] = {0, 0, 0, 0}, w2l[4] = {0, 0, 0, 0};
+  const int ore = (cm == CM_BF16) ? 64 : 0, oim = (cm == CM_BF16) ? 80 : 16;
+  uint32_t w[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < 8; ++t) {
     const int k = 8 * h + t, e = (row * k) & 15;
-    uint32_t re = g_ld(tab, (long long)(ore + e)), im = g_ld(tab, (long long)(oim + e));
-    uint32_t v1 = top ? re : im;
-    uint32_t v2 = top ? (im ^ 0x8000u) : re;
-    w1[t >> 1] |= v1 << (16 * (t & 1));
-    w2[t >> 1] |= v2 << (16 * (t & 1));
-    if (CM == CM_SPLIT) {
-      uint32_t rl = g_ld(tab, (long long)(32 + e)), il = g_ld(tab, (long long)(48 + e));
-      uint32_t l1 = top ? rl : il;
-      uint32_t l2 = top ? (il ^ 0x8000u) : rl;
-      w1l[t >> 1] |= l1 << (16 * (t & 1));
-      w2l[t >> 1] |= l2 << (16 * (t & 1));
+    const uint32_t re = tab[ore + e], im = tab[oim + e];
+    const uint32_t rl = tab[32 + e], il = tab[48 + e];
+    const uint32_t sh = 16 * (t & 1);
+    w[0 + (t >> 1)] |= (top ? re : im) << sh;
+    w[4 + (t >> 1)] |= (top ? (im ^ 0x8000u) : re) << sh;
+    if (cm == CM_SPLIT) {
+      w[8 + (t >> 1)] |= (top ? rl : il) << sh;
+      w[12 + (t >> 1)] |= (top ? (il ^ 0x8000u) : rl) << sh;
     }
   }
-  for (int w = 0; w < 4; ++w) {
-    A.a1[w] = w1[w];
-    A.a2[w] = w2[w];
-    A.a1lo[w] = w1l[w];
-    A.a2lo[w] = w2l[w];
-  }
+  for (int k = 0; k < 16; ++k) put(k, w[k]);
+}
+
+MCFFT_DEV AOps load_aops(const u32x4* tab, int lane) {
+  AOps A;
+  A.a1 = g_ld(tab, (long long)(lane * 4 + 0));
+  A.a2 = g_ld(tab, (long long)(lane * 4 + 1));
+  A.a1lo = g_ld(tab, (long long)(lane * 4 + 2));
+  A.a2lo = g_ld(tab, (long long)(lane * 4 + 3));
   return A;
 }
 
@@ -245,19 +257,22 @@ template <int E> struct For<E, E> {
 // Pass configuration (compile time)
 //   S, T      L = 16^S * T
 //   Q         instances per workgroup tile
-//   QFAST     MFMA columns and LDS rows run over instances (strided columns)
-//             instead of positions (contiguous rows)
 //   NW        waves per workgroup
 //   CM        compute/storage mode
-//   STAGED    the last stage writes LDS and a copy loop stores instance-fastest
-//             (coalesced when out_sq == 1, e.g. the transposed four-step store)
-//   FOURSTEP  multiply outputs by W_N^(m * column) before storing
+//   KIND      ROW / ROWT / COL / COLF (addressing above).  COL and COLF run
+//             the MFMA columns and LDS rows over instances (q fastest); COLF
+//             multiplies its outputs by W_N^(m * column); ROWT's last stage writes
+//             LDS and a copy loop stores instance-fastest (coalesced for the
+//             transposed four-step store).
 // ---------------------------------------------------------------------------
-template <int S, int T, int Q, bool QFAST, int NW, int CM, bool STAGED, bool FOURSTEP>
+template <int S, int T, int Q, int NW, int CM, int KIND>
 struct PassCfg {
   static constexpr int L = ipow16(S) * T;
   static constexpr int NT = NW * 64;
   static constexpr int NSTAGE = S + (T > 1 ? 1 : 0);
+  static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF);
+  static constexpr bool STAGED = (KIND == KIND_ROWT);
+  static constexpr bool FOURSTEP = (KIND == KIND_COLF);
   static constexpr int ROW = L + L / 16 + 1;  // padded LDS row (contiguous-row layout)
   static constexpr int LDS_ELEMS = QFAST ? Q * L : Q * ROW;
   typedef typename Traits<CM>::T El;
@@ -275,134 +290,199 @@ struct PassCfg {
     if constexpr (QFAST) { q = c % Q; j = c / Q; }
     else { q = c / (L / R); j = c % (L / R); }
   }
+  static MCFFT_DEV int in_off(int q, int m, int stride) {
+    if constexpr (QFAST) return q + m * stride;
+    else return q * L + m;
+  }
+  static MCFFT_DEV int out_off(int q, int m, int stride) {
+    if constexpr (KIND == KIND_ROW) return q * L + m;
+    else return q + m * stride;
+  }
 };
 
-template <int S, int T, int Q, bool QFAST, int NW, int CM, bool STAGED, bool FOURSTEP>
+template <int S, int T, int Q, int NW, int CM, int KIND>
 MCFFT_DEV void fft_pass_body(const PassArgs& a) {
-  typedef PassCfg<S, T, Q, QFAST, NW, CM, STAGED, FOURSTEP> C;
+  typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
   constexpr int L = C::L;
+  constexpr bool STAGED = C::STAGED, FOURSTEP = C::FOURSTEP;
   MCFFT_DYN_LDS(El, lds);
 
   const int tid = thread_id(), lane = lane_id(), wave = tid >> 6, h = lane >> 5;
-  const int blk = block_id();
-  const int grp = blk / a.tiles_per_group, til = blk - grp * a.tiles_per_group;
-  const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
-  const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
   const bool inv = a.inverse != 0;
-  const long long inst0 = (long long)til * Q;
-  auto valid = [&](int q) -> bool { return inst0 + q < a.n_inst; };
-
-  const AOps A = load_dft_operands<CM>(a.dft, lane);
-
-  // output post-processing of the pass's last stage: four-step twiddle or scale
-  auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
-    if constexpr (FOURSTEP) {
-      const unsigned long long col = (unsigned long long)(inst0 + q);
-      const unsigned long long e = ((unsigned long long)m * col) & a.fs_nmask;
-      const f32x2 w = cmul(g_ld(a.fs_hi, (long long)(e >> a.fs_shift)), g_ld(a.fs_lo, (long long)(e & a.fs_mask)));
-      return cmul(v, w);
-    } else {
-      return v * a.out_scale;
-    }
-  };
-  auto store_out = [&](int q, int m, El e) {
-    if (valid(q)) Tr::store(a.out, ob + (long long)q * a.out_sq + (long long)m * a.out_sm, e);
+  const AOps A = load_aops(a.aops, lane);
+  const int sin_ = a.stride_in, sout = a.stride_out;
+  const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
+  constexpr int esz_out = (int)sizeof(El);
+  const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);
+  GBuf gfh, gfl;
+  if constexpr (FOURSTEP) {
+    gfh = make_gbuf(a.fs_hi, (unsigned)a.fs_hi_entries * 8u);
+    gfl = make_gbuf(a.fs_lo, (unsigned)a.fs_lo_entries * 8u);
+  }
+  auto view = [](const void* base, long long first, long long total, int esz) -> GBuf {
+    const long long rem = (total - first) * esz;
+    return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
   };
 
-  // ---------------- radix-16 MFMA stages ----------------
-  For<0, S>::run([&](auto pc) {
-    constexpr int P = decltype(pc)::value;
-    constexpr int NS = ipow16(P);
-    constexpr bool LAST = (P == C::NSTAGE - 1);
-    constexpr int NCOLT = Q * L / 512;             // 32-column MFMA tiles
-    constexpr int TPW = (NCOLT + NW - 1) / NW;     // tiles per wave
-    El x[TPW][8];
-    for (int k = 0; k < TPW; ++k) {
-      const int ti = wave + k * NW;
-      if (ti < NCOLT) {
-        int q, j;
-        C::template colmap<16>(ti * 32 + (lane & 31), q, j);
-        for (int t = 0; t < 8; ++t) {
-          const int m = j + (8 * h + t) * (L / 16);
-          if constexpr (P == 0) {
-            if (valid(q)) x[k][t] = Tr::load(a.in, ib + (long long)q * a.in_sq + (long long)m * a.in_sm, a.in_fmt);
-            else x[k][t] = El{};
-          } else
-            x[k][t] = lds_ld(lds, C::lidx(q, m));
-        }
+  for (int tile = block_id(); tile < a.n_tiles; tile += grid_size()) {
+    const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
+    const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
+    const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
+    const GBuf gin = view(a.in, ib, a.in_elems, esz_in);
+    const GBuf gout = view(a.out, ob, a.out_elems, esz_out);
+    const int inst0 = til * Q;
+    auto valid = [&](int q) -> bool { return inst0 + q < a.n_inst; };
+
+    // output post-processing of the pass's last stage: four-step twiddle or scale
+    auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
+      if constexpr (FOURSTEP) {
+        const unsigned e = ((unsigned)m * (unsigned)(inst0 + q)) & a.fs_nmask;
+        const f32x2 th = __builtin_bit_cast(f32x2, buf_ld64(gfh, (int)((e >> a.fs_shift) * 8u)));
+        const f32x2 tl = __builtin_bit_cast(f32x2, buf_ld64(gfl, (int)((e & a.fs_mask) * 8u)));
+        return cmul(v, cmul(th, tl));
+      } else {
+        return v * a.out_scale;
       }
-    }
-    // P > 0: all reads of LDS before any overwrite.  A single-stage pass
-    // (L = 16) reads and writes global memory in one stage: the barrier keeps an
-    // in-place pass from overwriting elements another wave has not read yet.
-    if constexpr (P > 0 || (LAST && !STAGED)) block_barrier();
-    for (int k = 0; k < TPW; ++k) {
-      const int ti = wave + k * NW;
-      if (ti < NCOLT) {
-        int q, j;
-        C::template colmap<16>(ti * 32 + (lane & 31), q, j);
-        const f32x16 y = dft16<CM>(x[k], A);
-        for (int s = 0; s < 8; ++s) {
-          const int rr = 4 * h + (s & 3) + 8 * (s >> 2);
-          const int mo = (j / NS) * NS * 16 + (j % NS) + rr * NS;
-          f32x2 v;
-          v.x = y[s];
-          v.y = y[8 + s];
-          if constexpr (!LAST) {
-            v = cmul(v, g_ld(a.stage_tw, (long long)(P * L + rr * (L / 16) + j)));
-            lds_st(lds, C::lidx(q, mo), Tr::pack(v));
+    };
+    auto store_out = [&](int q, int m, El e) {
+      if (valid(q)) Tr::store(gout, C::out_off(q, m, sout), e);
+    };
+
+    // ---------------- radix-16 MFMA stages ----------------
+    For<0, S>::run([&](auto pc) {
+      constexpr int P = decltype(pc)::value;
+      constexpr int NS = ipow16(P);
+      constexpr bool LAST = (P == C::NSTAGE - 1);
+      constexpr int NCOLT = Q * L / 512;             // 32-column MFMA tiles
+      constexpr int TPW = (NCOLT + NW - 1) / NW;     // tiles per wave
+      El x[TPW][8];
+      for (int k = 0; k < TPW; ++k) {
+        const int ti = wave + k * NW;
+        if (ti < NCOLT) {
+          int q, j;
+          C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+          if constexpr (P == 0) {
+            const bool ok = valid(q);
+            const int o0 = C::in_off(q, j + 8 * h * (L / 16), sin_);
+            const int ostep = C::in_off(0, L / 16, sin_);
+            for (int t = 0; t < 8; ++t) x[k][t] = ok ? Tr::load(gin, o0 + t * ostep, a.in_fmt) : El{};
           } else {
-            v = finish(v, q, mo);
-            if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(v));
-            else store_out(q, mo, Tr::pack(v));
+            for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, C::lidx(q, j + (8 * h + t) * (L / 16)));
           }
         }
       }
-    }
-    if constexpr (!LAST || STAGED) block_barrier();
-  });
-
-  // ---------------- radix-T tail stage (VALU, FP32) ----------------
-  if constexpr (T > 1) {
-    constexpr int NCOL = Q * L / T;
-    constexpr int CPT = (NCOL + C::NT - 1) / C::NT;
-    El x[CPT][T];
-    for (int k = 0; k < CPT; ++k) {
-      const int c = tid + k * C::NT;
-      if (c < NCOL) {
-        int q, j;
-        C::template colmap<T>(c, q, j);
-        for (int r = 0; r < T; ++r) x[k][r] = lds_ld(lds, C::lidx(q, j + r * (L / T)));
-      }
-    }
-    if constexpr (STAGED) block_barrier();
-    for (int k = 0; k < CPT; ++k) {
-      const int c = tid + k * C::NT;
-      if (c < NCOL) {
-        int q, j;
-        C::template colmap<T>(c, q, j);
-        f32x2 v[T];
-        for (int r = 0; r < T; ++r) v[r] = Tr::unpack(x[k][r]);
-        dft_small<T>(v, inv);
-        for (int r = 0; r < T; ++r) {
-          const int mo = j + r * (L / T);
-          const f32x2 w = finish(v[r], q, mo);
-          if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(w));
-          else store_out(q, mo, Tr::pack(w));
+      // P > 0: all reads of LDS before any overwrite.  A single-stage pass
+      // (L = 16) reads and writes global memory in one stage: the barrier keeps an
+      // in-place pass from overwriting elements another wave has not read yet.
+      if constexpr (P > 0 || (LAST && !STAGED)) block_barrier();
+      for (int k = 0; k < TPW; ++k) {
+        const int ti = wave + k * NW;
+        if (ti < NCOLT) {
+          int q, j;
+          C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+          const f32x16 y = dft16<CM>(x[k], A);
+          const int jb = (j / NS) * NS * 16 + (j % NS);
+          for (int g = 0; g < 2; ++g) {
+            f32x2 tw[4];
+            if constexpr (!LAST) {
+              // entries rr = 4h + 8g + 0..3 of row j: two 16-byte loads
+              const int toff = (P * L + j * 16 + 4 * h + 8 * g) * 8;
+              const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
+              const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
+              tw[0] = t01.xy;
+              tw[1] = t01.zw;
+              tw[2] = t23.xy;
+              tw[3] = t23.zw;
+            }
+            const int mo0 = jb + (4 * h + 8 * g) * NS;
+            for (int u = 0; u < 4; ++u) {
+              const int s = 4 * g + u;
+              const int mo = mo0 + u * NS;
+              f32x2 v;
+              v.x = y[s];
+              v.y = y[8 + s];
+              if constexpr (!LAST) {
+                lds_st(lds, C::lidx(q, mo), Tr::pack(cmul(v, tw[u])));
+              } else if constexpr (STAGED) {
+                lds_st(lds, C::lidx(q, mo), Tr::pack(finish(v, q, mo)));
+              }
+            }
+            if constexpr (LAST && !STAGED) {
+              // direct store: offsets and four-step exponents advance linearly in u
+              if (valid(q)) {
+                const int o0 = C::out_off(q, mo0, sout), ostep = C::out_off(0, NS, sout);
+                unsigned e0 = 0, estep = 0;
+                if constexpr (FOURSTEP) {
+                  const unsigned col = (unsigned)(inst0 + q);
+                  e0 = (unsigned)mo0 * col;
+                  estep = (unsigned)NS * col;
+                }
+                for (int u = 0; u < 4; ++u) {
+                  const int s = 4 * g + u;
+                  f32x2 v;
+                  v.x = y[s];
+                  v.y = y[8 + s];
+                  if constexpr (FOURSTEP) {
+                    const unsigned e = (e0 + (unsigned)u * estep) & a.fs_nmask;
+                    const f32x2 th = __builtin_bit_cast(f32x2, buf_ld64(gfh, (int)((e >> a.fs_shift) * 8u)));
+                    const f32x2 tl = __builtin_bit_cast(f32x2, buf_ld64(gfl, (int)((e & a.fs_mask) * 8u)));
+                    v = cmul(v, cmul(th, tl));
+                  } else {
+                    v = v * a.out_scale;
+                  }
+                  Tr::store(gout, o0 + u * ostep, Tr::pack(v));
+                }
+              }
+            }
+          }
         }
       }
-    }
-    if constexpr (STAGED) block_barrier();
-  }
+      if constexpr (!LAST || STAGED) block_barrier();
+    });
 
-  // ---------------- staged store: instance-fastest order ----------------
-  if constexpr (STAGED) {
-    for (int e = tid; e < Q * L; e += C::NT) {
-      const int q = e % Q, m = e / Q;
-      store_out(q, m, lds_ld(lds, C::lidx(q, m)));
+    // ---------------- radix-T tail stage (VALU, FP32) ----------------
+    if constexpr (T > 1) {
+      constexpr int NCOL = Q * L / T;
+      constexpr int CPT = (NCOL + C::NT - 1) / C::NT;
+      El x[CPT][T];
+      for (int k = 0; k < CPT; ++k) {
+        const int c = tid + k * C::NT;
+        if (c < NCOL) {
+          int q, j;
+          C::template colmap<T>(c, q, j);
+          for (int r = 0; r < T; ++r) x[k][r] = lds_ld(lds, C::lidx(q, j + r * (L / T)));
+        }
+      }
+      if constexpr (STAGED) block_barrier();
+      for (int k = 0; k < CPT; ++k) {
+        const int c = tid + k * C::NT;
+        if (c < NCOL) {
+          int q, j;
+          C::template colmap<T>(c, q, j);
+          f32x2 v[T];
+          for (int r = 0; r < T; ++r) v[r] = Tr::unpack(x[k][r]);
+          dft_small<T>(v, inv);
+          for (int r = 0; r < T; ++r) {
+            const int mo = j + r * (L / T);
+            const f32x2 w = finish(v[r], q, mo);
+            if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(w));
+            else store_out(q, mo, Tr::pack(w));
+          }
+        }
+      }
+      if constexpr (STAGED) block_barrier();
     }
+
+    // ---------------- staged store: instance-fastest order ----------------
+    if constexpr (STAGED) {
+      for (int e = tid; e < Q * L; e += C::NT) {
+        const int q = e % Q, m = e / Q;
+        store_out(q, m, lds_ld(lds, C::lidx(q, m)));
+      }
+    }
+    block_barrier();  // LDS is reused by the next tile
   }
 }
 

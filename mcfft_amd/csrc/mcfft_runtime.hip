@@ -21,6 +21,8 @@ struct mcfftPlan_st {
   int device = 0;
   f32x2* tw[2] = {nullptr, nullptr};      // [0] forward, [1] inverse
   uint16_t* dft[2] = {nullptr, nullptr};
+  u32x4* aops[2] = {nullptr, nullptr};    // per-lane DFT-matrix MFMA operands
+  std::vector<int> grid_cap;              // resident workgroups per pass (occupancy x CUs)
   void* ws = nullptr;
   size_t ws_bytes = 0;
 };
@@ -57,6 +59,13 @@ __global__ void gen_dft_table(int sign, uint16_t* t) {
   t[80 + e] = __builtin_bit_cast(uint16_t, (__bf16)(float)im);
 }
 
+// per-lane MFMA operand words of the DFT matrix (64 lanes x 16 words)
+__global__ void gen_aops(const uint16_t* dft, int cm, uint32_t* out) {
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  aops_words(dft, cm, lane, [&](int k, uint32_t w) { out[lane * 16 + k] = w; });
+}
+
 std::mutex g_attr_mu;
 std::vector<const void*> g_attr_done;  // kernels whose LDS limit was raised
 
@@ -75,6 +84,7 @@ void free_plan(mcfftPlan p) {
   for (int d = 0; d < 2; ++d) {
     if (p->tw[d]) (void)hipFree(p->tw[d]);
     if (p->dft[d]) (void)hipFree(p->dft[d]);
+    if (p->aops[d]) (void)hipFree(p->aops[d]);
   }
   if (p->ws) (void)hipFree(p->ws);
   delete p;
@@ -99,12 +109,22 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     delete p;
     return MCFFT_HIP_ERROR;
   }
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || n_cu < 1) {
+    delete p;
+    return MCFFT_HIP_ERROR;
+  }
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);
+    int per_cu = 0;
+    if (s == MCFFT_SUCCESS &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp.k->fn, pp.k->nw * 64, pp.k->lds_bytes) != hipSuccess)
+      s = MCFFT_HIP_ERROR;
     if (s != MCFFT_SUCCESS) {
       delete p;
       return s;
     }
+    p->grid_cap.push_back((per_cu < 1 ? 1 : per_cu) * n_cu);
   }
   // tables: FP64 on the GPU, then rounded to FP32 / FP16 / BF16
   const long long ntw = (long long)p->core.tw.size();
@@ -115,6 +135,7 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     if (hipMalloc(&p->tw[d], sizeof(f32x2) * (ntw > 0 ? ntw : 1)) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
     if (rs == MCFFT_SUCCESS && hipMalloc(&p->dft[d], sizeof(uint16_t) * kDftTableEntries) != hipSuccess)
       rs = MCFFT_ALLOC_FAILED;
+    if (rs == MCFFT_SUCCESS && hipMalloc(&p->aops[d], sizeof(u32x4) * 64 * 4) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
   }
   if (rs == MCFFT_SUCCESS && ntw > 0 &&
       hipMemcpy(dspec, p->core.tw.data(), sizeof(TwSpec) * ntw, hipMemcpyHostToDevice) != hipSuccess)
@@ -123,6 +144,7 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     const int sign = d == 0 ? -1 : 1;
     if (ntw > 0) gen_twiddles<<<(unsigned)((ntw + 255) / 256), 256>>>(dspec, ntw, sign, p->tw[d]);
     gen_dft_table<<<1, 64>>>(sign, p->dft[d]);
+    gen_aops<<<1, 64>>>(p->dft[d], p->core.cm, (uint32_t*)p->aops[d]);
     if (hipGetLastError() != hipSuccess) rs = MCFFT_HIP_ERROR;
   }
   if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;
@@ -194,31 +216,39 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
       const int dst_fmt = (pp.dst == BUF_OUT) ? c.out_fmt : ws_fmt;
       a.in = bufptr(pp.src, pp.src_fmt);
       a.out = bufptr(pp.dst, dst_fmt);
-      a.in_sq = pp.in_sq;
-      a.in_sm = pp.in_sm;
-      a.out_sq = pp.out_sq;
-      a.out_sm = pp.out_sm;
+      a.stride_in = pp.stride_in;
+      a.stride_out = pp.stride_out;
       a.in_sg = pp.in_sg;
       a.in_st = pp.in_st;
       a.out_sg = pp.out_sg;
       a.out_st = pp.out_st;
-      a.n_inst = pp.n_inst;
+      a.n_inst = (int)pp.n_inst;
       a.tiles_per_group = pp.tiles_per_group;
       a.in_fmt = pp.src_fmt;
+      auto view_elems = [&](int b) -> long long {
+        return b == BUF_WS ? c.ws_elems : (c.batch - t0) * c.n;
+      };
+      a.in_elems = view_elems(pp.src);
+      a.out_elems = view_elems(pp.dst);
+      a.stage_tw_entries = (int)((long long)c.tw.size() - pp.stage_tw_off);
       a.inverse = d;
       a.out_scale = pp.out_scale;
       a.stage_tw = plan->tw[d] + pp.stage_tw_off;
-      a.dft = plan->dft[d];
+      a.aops = plan->aops[d];
       if (pp.fourstep) {
         a.fs_hi = plan->tw[d] + pp.fs_hi_off;
         a.fs_lo = plan->tw[d] + pp.fs_lo_off;
         a.fs_shift = pp.fs_shift;
+        a.fs_hi_entries = (int)(pp.fs_lo_off - pp.fs_hi_off);
+        a.fs_lo_entries = 1 << pp.fs_shift;
         a.fs_mask = pp.fs_mask;
-        a.fs_nmask = pp.fs_nmask;
+        a.fs_nmask = (unsigned)pp.fs_nmask;
       }
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
-      const long long grid = groups * (long long)pp.tiles_per_group;
-      if (grid <= 0 || grid > 0x7fffffffLL) return MCFFT_UNSUPPORTED;
+      const long long tiles = groups * (long long)pp.tiles_per_group;
+      if (tiles <= 0 || tiles > 0x7fffffffLL) return MCFFT_UNSUPPORTED;
+      a.n_tiles = (int)tiles;
+      const long long grid = tiles < plan->grid_cap[pi] ? tiles : plan->grid_cap[pi];
       void* args[] = {&a};
       if (hipLaunchKernel(pp.k->fn, dim3((unsigned)grid), dim3((unsigned)(pp.k->nw * 64)), args,
                           (size_t)pp.k->lds_bytes, s) != hipSuccess)

@@ -30,15 +30,15 @@ static bool split_len(long long L, int& S, int& T) {
   return true;
 }
 
-static const KernelInfo* find_kernel(int S, int T, int cm, int qfast, int staged, int fourstep) {
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
+
+static const KernelInfo* find_kernel(int S, int T, int cm, int kind) {
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
-  for (int l = 0; l < 4; ++l)
-    for (int i = 0; i < counts[l]; ++i) {
-      const KernelInfo& k = lists[l][i];
-      if (k.S == S && k.T == T && k.cm == cm && k.qfast == qfast && k.staged == staged && k.fourstep == fourstep)
-        return &k;
-    }
+  for (int i = 0; i < counts[kind]; ++i) {
+    const KernelInfo& k = lists[kind][i];
+    if (k.S == S && k.T == T && k.cm == cm) return &k;
+  }
   return nullptr;
 }
 
@@ -57,7 +57,7 @@ static float pow2(int e) {  // 2^e, exact
 }
 
 // Append the stage post-twiddle tables of one pass of length L (stages 16^S then T)
-// and return the per-stage shifts.  Table of stage P, entry rr*(L/16)+j, is the
+// and return the per-stage shifts.  Table of stage P, entry j*16+rr, is the
 // input twiddle of stage P+1 for the element stage P writes at
 // m' = (j/Ns)*Ns*16 + j%Ns + rr*Ns, times 2^-shift_P.
 static std::vector<int> add_stage_tables(PlanCore& p, PassPlan& pp, int S, int T, int scaling) {
@@ -73,8 +73,8 @@ static std::vector<int> add_stage_tables(PlanCore& p, PassPlan& pp, int S, int T
     const int R = radix[P];  // always 16 (the tail is last)
     const long long nsn = ns * R;
     const int Rn = radix[P + 1];
-    for (long long rr = 0; rr < R; ++rr)
-      for (long long j = 0; j < L / R; ++j) {
+    for (long long j = 0; j < L / R; ++j)
+      for (long long rr = 0; rr < R; ++rr) {
         const long long mo = (j / ns) * ns * R + (j % ns) + rr * ns;
         const long long r2 = mo / (L / Rn), j2 = mo % (L / Rn), k2 = j2 % nsn;
         p.tw.push_back({(r2 * k2) % (nsn * Rn), nsn * Rn, pow2(-shifts[P])});
@@ -111,15 +111,13 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, 0, 0, 0);
+    const KernelInfo* k = find_kernel(S, T, cm, KIND_ROW);
     if (!k) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
     pp.src = src;
     pp.dst = dst;
     pp.src_fmt = src_fmt;
-    pp.in_sq = pp.out_sq = L;
-    pp.in_sm = pp.out_sm = 1;
     pp.in_sg = pp.out_sg = 0;
     pp.in_st = pp.out_st = (long long)k->Q * L;
     pp.n_inst = n_rows;
@@ -137,15 +135,14 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
                      int dst, int src_fmt) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, 1, 0, 0);
-    if (!k || ncols % k->Q != 0) return ST_UNSUPPORTED;
+    const KernelInfo* k = find_kernel(S, T, cm, KIND_COL);
+    if (!k || ncols % k->Q != 0 || stride * L > 0x7fffffffLL) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
     pp.src = src;
     pp.dst = dst;
     pp.src_fmt = src_fmt;
-    pp.in_sq = pp.out_sq = 1;
-    pp.in_sm = pp.out_sm = stride;
+    pp.stride_in = pp.stride_out = (int)stride;
     pp.in_sg = pp.out_sg = gstride;
     pp.in_st = pp.out_st = k->Q;
     pp.n_inst = ncols;
@@ -177,8 +174,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
       int S1, T1, S2, T2;
       if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
-      const KernelInfo* kc = find_kernel(S1, T1, cm, 1, 0, 1);
-      const KernelInfo* kr = find_kernel(S2, T2, cm, 0, 1, 0);
+      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF);
+      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT);
       if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
       long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (256LL << 20);
       long long per = N * elem_bytes(ws_fmt);
@@ -194,8 +191,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       a.src = BUF_IN;
       a.dst = BUF_WS;
       a.src_fmt = in_fmt;
-      a.in_sq = a.out_sq = 1;
-      a.in_sm = a.out_sm = N2;
+      a.stride_in = a.stride_out = (int)N2;
       a.in_sg = a.out_sg = N;
       a.in_st = a.out_st = kc->Q;
       a.n_inst = N2;
@@ -221,10 +217,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       b.src = BUF_WS;
       b.dst = BUF_OUT;
       b.src_fmt = ws_fmt;
-      b.in_sq = N2;
-      b.in_sm = 1;
-      b.out_sq = 1;
-      b.out_sm = N1;
+      b.stride_out = (int)N1;
       b.in_sg = b.out_sg = N;
       b.in_st = (long long)kr->Q * N2;
       b.out_st = kr->Q;
@@ -264,8 +257,8 @@ std::string PlanCore::describe() const {
     << " chunk=" << chunk << "x" << n_chunks << " ws=" << ws_elems << "\n";
   for (const auto& pp : passes) {
     o << "  pass " << pp.k->name << " src=" << pp.src << " dst=" << pp.dst << " groups=" << pp.groups
-      << " tiles/group=" << pp.tiles_per_group << " n_inst=" << pp.n_inst << " in(sq,sm)=(" << pp.in_sq << ","
-      << pp.in_sm << ") out(sq,sm)=(" << pp.out_sq << "," << pp.out_sm << ") lds=" << pp.k->lds_bytes
+      << " tiles/group=" << pp.tiles_per_group << " n_inst=" << pp.n_inst << " stride(in,out)=(" << pp.stride_in
+      << "," << pp.stride_out << ") lds=" << pp.k->lds_bytes
       << (pp.fourstep ? " fourstep" : "") << " shifts=";
     for (int s : pp.shifts) o << s << ",";
     o << "\n";

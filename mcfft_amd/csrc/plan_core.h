@@ -29,7 +29,7 @@ struct PassPlan {
   const KernelInfo* k = nullptr;
   int src = BUF_IN, dst = BUF_OUT;
   int src_fmt = 0;
-  long long in_sq = 0, in_sm = 0, out_sq = 0, out_sm = 0;
+  int stride_in = 0, stride_out = 0;  // strided-axis strides (see PassArgs)
   long long in_sg = 0, in_st = 0, out_sg = 0, out_st = 0;
   long long n_inst = 0;         // valid instances per group
   int tiles_per_group = 1;

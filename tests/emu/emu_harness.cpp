@@ -71,6 +71,9 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   for (size_t i = 0; i < p.tw.size(); ++i) twiddle_host(p.tw[i], sign, tw[i]);
   std::vector<uint16_t> dft(kDftTableEntries);
   dft_table_host(sign, dft.data());
+  std::vector<u32x4> aops(64 * 4);
+  for (int lane = 0; lane < 64; ++lane)
+    aops_words(dft.data(), p.cm, lane, [&](int k, uint32_t w) { aops[lane * 4 + k / 4][k % 4] = w; });
   const int ws_fmt = (p.cm == 2) ? 2 : in_fmt;
   std::vector<unsigned char> ws((size_t)(p.ws_elems * elem_bytes(ws_fmt)) + 16);
   const size_t in_bytes = (size_t)(p.n * batch * elem_bytes(in_fmt));
@@ -80,7 +83,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   emu::register_range(out, out_bytes);
   emu::register_range(ws.data(), (size_t)(p.ws_elems * elem_bytes(ws_fmt)));
   emu::register_range(tw.data(), p.tw.size() * sizeof(f32x2));
-  emu::register_range(dft.data(), dft.size() * 2);
+  emu::register_range(aops.data(), aops.size() * sizeof(u32x4));
   for (long long c = 0; c < p.n_chunks; ++c) {
     const long long t0 = c * p.chunk;
     const long long nt = (t0 + p.chunk <= batch) ? p.chunk : batch - t0;
@@ -95,24 +98,35 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
       a.in = bufptr(pp.src, pp.src_fmt);
       a.out = bufptr(pp.dst, dst_fmt);
-      a.in_sq = pp.in_sq; a.in_sm = pp.in_sm; a.out_sq = pp.out_sq; a.out_sm = pp.out_sm;
+      a.stride_in = pp.stride_in; a.stride_out = pp.stride_out;
       a.in_sg = pp.in_sg; a.in_st = pp.in_st; a.out_sg = pp.out_sg; a.out_st = pp.out_st;
-      a.n_inst = pp.n_inst;
+      a.n_inst = (int)pp.n_inst;
       a.tiles_per_group = pp.tiles_per_group;
       a.in_fmt = pp.src_fmt;
+      auto view_elems = [&](int b) -> long long {
+        return b == BUF_WS ? p.ws_elems : (p.batch - t0) * p.n;
+      };
+      a.in_elems = view_elems(pp.src);
+      a.out_elems = view_elems(pp.dst);
+      a.stage_tw_entries = (int)((long long)p.tw.size() - pp.stage_tw_off);
       a.inverse = inverse;
       a.out_scale = pp.out_scale;
       a.stage_tw = tw.data() + pp.stage_tw_off;
-      a.dft = dft.data();
+      a.aops = aops.data();
       if (pp.fourstep) {
         a.fs_hi = tw.data() + pp.fs_hi_off;
         a.fs_lo = tw.data() + pp.fs_lo_off;
         a.fs_shift = pp.fs_shift;
+        a.fs_hi_entries = (int)(pp.fs_lo_off - pp.fs_hi_off);
+        a.fs_lo_entries = 1 << pp.fs_shift;
         a.fs_mask = pp.fs_mask;
-        a.fs_nmask = pp.fs_nmask;
+        a.fs_nmask = (unsigned)pp.fs_nmask;
       }
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
-      const long long grid = groups * pp.tiles_per_group;
+      const long long tiles = groups * pp.tiles_per_group;
+      a.n_tiles = (int)tiles;
+      // fewer workgroups than tiles exercises the persistent tile loop
+      const long long grid = tiles > 3 ? (tiles + 1) / 2 : tiles;
       void (*fn)(PassArgs) = (void (*)(PassArgs))pp.k->fn;
       emu::launch((int)grid, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); });
     }

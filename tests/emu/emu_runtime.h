@@ -39,7 +39,7 @@ inline void check_global(const void* p, size_t bytes, long long i) {
 }
 
 struct Block {
-  int nt = 0, bid = 0;
+  int nt = 0, bid = 0, nblocks = 0;
   std::vector<unsigned char> lds;
   std::barrier<>* bar = nullptr;
   std::vector<std::barrier<>*> wbar;
@@ -58,6 +58,7 @@ inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>&
     Block blk;
     blk.nt = nt;
     blk.bid = b;
+    blk.nblocks = nblocks;
     blk.lds.assign(lds, 0xCD);
     std::barrier<> bar(nt);
     blk.bar = &bar;
@@ -80,6 +81,7 @@ namespace mcfft {
 inline int lane_id() { return emu::tl_tid % 64; }
 inline int thread_id() { return emu::tl_tid; }
 inline int block_id() { return emu::tl_blk->bid; }
+inline int grid_size() { return emu::tl_blk->nblocks; }
 inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
 
 template <class T> inline T lds_ld(const T* p, int i) {
@@ -96,6 +98,30 @@ template <class T> inline void lds_st(T* p, int i, T v) {
 }
 template <class T> inline T g_ld(const T* p, long long i) { emu::check_global(p + i, sizeof(T), i); return p[i]; }
 template <class T> inline void g_st(T* p, long long i, T v) { emu::check_global(p + i, sizeof(T), i); p[i] = v; }
+
+// buffer-resource emulation: every access must stay inside [0, bytes)
+struct GBuf {
+  const char* p;
+  unsigned bytes;
+};
+inline GBuf make_gbuf(const void* p, unsigned bytes) { return GBuf{(const char*)p, bytes}; }
+template <class T> inline T buf_ld(GBuf b, int off) {
+  if (off < 0 || (unsigned)off + sizeof(T) > b.bytes) emu::fail("buffer load", off);
+  emu::check_global(b.p + off, sizeof(T), off);
+  T v;
+  memcpy(&v, b.p + off, sizeof(T));
+  return v;
+}
+template <class T> inline void buf_st(GBuf b, int off, T v) {
+  if (off < 0 || (unsigned)off + sizeof(T) > b.bytes) emu::fail("buffer store", off);
+  emu::check_global(b.p + off, sizeof(T), off);
+  memcpy((char*)b.p + off, &v, sizeof(T));
+}
+inline uint32_t buf_ld32(GBuf b, int off) { return buf_ld<uint32_t>(b, off); }
+inline u32x2 buf_ld64(GBuf b, int off) { return buf_ld<u32x2>(b, off); }
+inline u32x4 buf_ld128(GBuf b, int off) { return buf_ld<u32x4>(b, off); }
+inline void buf_st32(GBuf b, int off, uint32_t v) { buf_st<uint32_t>(b, off, v); }
+inline void buf_st64(GBuf b, int off, u32x2 v) { buf_st<u32x2>(b, off, v); }
 
 template <class Dec>
 inline f32x16 emu_mfma(u32x4 a, u32x4 b, f32x16 c, Dec dec) {
