@@ -90,6 +90,28 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
  * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
 
+/* Ragged batch: n_items transforms of any supported shapes (rank 1..3; 1D
+ * lengths 16..2^18, sides 16..4096), each at its own element offset in the
+ * exec input and output buffers, run by ONE persistent kernel launch over a
+ * device-side work queue (plus a tiny queue-reset kernel), so the whole batch is
+ * a single graph-capturable exec.  Back-to-back 1D items of one length up to
+ * 4096 are processed as one multi-row unit.  Items may have different output
+ * exponents: see mcfft_get_item_exponents.  No reference counterpart. */
+typedef struct mcfftRaggedItem_t {
+  int rank;
+  long long dims[3];
+  long long in_offset;   /* element offset of the transform in the exec input */
+  long long out_offset;  /* element offset in the exec output */
+} mcfftRaggedItem;
+
+mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedItem* items, mcfftFormat in_fmt,
+                              mcfftFormat out_fmt, mcfftScaling scaling);
+/* Output exponent of every item (n_items entries; 1 entry for a regular plan). */
+mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents);
+/* Ragged plans: nonzero if a queue wait of the last exec hit its bound (its
+ * result is then wrong).  Synchronises with the device. */
+mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags);
+
 /* Profiling entry points (no reference counterpart): the number of kernel
  * passes of a plan, and an exec that launches only passes [first, first+count)
  * (for every batch chunk), so a caller can time each kernel on its own stream.

@@ -30,6 +30,11 @@ STATUS = {0: "success", 1: "invalid value", 2: "unsupported", 3: "allocation fai
 _lib = None
 
 
+class RaggedItem(ctypes.Structure):
+    _fields_ = [("rank", ctypes.c_int), ("dims", ctypes.c_longlong * 3), ("in_offset", ctypes.c_longlong),
+                ("out_offset", ctypes.c_longlong)]
+
+
 class McfftError(RuntimeError):
     def __init__(self, status: int, what: str):
         super().__init__(f"{what}: {STATUS.get(status, status)} (status {status})")
@@ -58,8 +63,12 @@ def lib() -> ctypes.CDLL:
         L.mcfft_status_string.argtypes = [i]
         L.mcfft_status_string.restype = ctypes.c_char_p
         L.mcfft_version.restype = i
+        L.mcfft_plan_ragged.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(RaggedItem), i, i, i]
+        L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
+        L.mcfft_get_status_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
-                  "mcfft_exec_passes", "mcfft_get_pass_count",
+                  "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
+                  "mcfft_get_status_flags",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -138,6 +147,34 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+class RaggedPlan(Plan):
+    """One persistent launch over many transforms of different shapes.
+    `items`: list of (dims, in_offset, out_offset) with element offsets."""
+
+    def __init__(self, items, in_fmt: str = "fp16", out_fmt: str | None = None, scaling: str = "balanced"):
+        out_fmt = out_fmt or in_fmt
+        self.in_fmt, self.out_fmt, self.scaling = in_fmt, out_fmt, scaling
+        self.items = [(list(d), int(io), int(oo)) for d, io, oo in items]
+        arr = (RaggedItem * len(items))()
+        for k, (d, io, oo) in enumerate(self.items):
+            arr[k].rank = len(d)
+            for j in range(3):
+                arr[k].dims[j] = d[j] if j < len(d) else 1
+            arr[k].in_offset, arr[k].out_offset = io, oo
+        self._h = ctypes.c_void_p()
+        _check(lib().mcfft_plan_ragged(ctypes.byref(self._h), len(items), arr, FORMATS[in_fmt], FORMATS[out_fmt],
+                                       SCALINGS[scaling]), "mcfft_plan_ragged")
+        ex = (ctypes.c_int * len(items))()
+        _check(lib().mcfft_get_item_exponents(self._h, ex), "mcfft_get_item_exponents")
+        self.exponents = list(ex)
+        self.exponent = None
+
+    def status_flags(self) -> int:
+        f = ctypes.c_uint()
+        _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
+        return f.value
 
 
 def components(fmt: str) -> int:

@@ -22,6 +22,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #include <hip/hip_runtime.h>
 
 #define MCFFT_DEV __device__ __forceinline__
+#define MCFFT_NOINLINE __device__ __attribute__((noinline))
 #define MCFFT_KERNEL(nt) extern "C" __global__ __launch_bounds__(nt)
 #define MCFFT_TKERNEL(nt) __global__ __launch_bounds__(nt)
 #define MCFFT_DYN_LDS(type, name) \
@@ -33,6 +34,7 @@ MCFFT_DEV int lane_id() { return __lane_id(); }
 MCFFT_DEV int thread_id() { return threadIdx.x; }
 MCFFT_DEV int block_id() { return blockIdx.x; }
 MCFFT_DEV int grid_size() { return gridDim.x; }
+MCFFT_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 MCFFT_DEV void block_barrier() { __syncthreads(); }
 
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
@@ -56,6 +58,20 @@ MCFFT_DEV u32x2 buf_ld64(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_l
 MCFFT_DEV u32x4 buf_ld128(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_load_b128(b.r, off, 0, 0); }
 MCFFT_DEV void buf_st32(GBuf b, int off, uint32_t v) { __builtin_amdgcn_raw_buffer_store_b32(v, b.r, off, 0, 0); }
 MCFFT_DEV void buf_st64(GBuf b, int off, u32x2 v) { __builtin_amdgcn_raw_buffer_store_b64(v, b.r, off, 0, 0); }
+
+// device-scope (all XCDs) atomics and fences for the ragged work queue
+MCFFT_DEV unsigned atomic_add_dev(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MCFFT_DEV unsigned atomic_load_dev(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MCFFT_DEV void atomic_or_dev(unsigned* p, unsigned v) {
+  (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// release/acquire at device scope: writes back / invalidates the XCD's L2 as needed
+MCFFT_DEV void fence_dev() { __threadfence(); }
+MCFFT_DEV void nap() { __builtin_amdgcn_s_sleep(4); }
 
 // D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
 // Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and

@@ -300,8 +300,10 @@ struct PassCfg {
   }
 };
 
+// One tile of a pass: stages on the tile's Q instances, ending with a barrier
+// (LDS is reused by the workgroup's next tile).
 template <int S, int T, int Q, int NW, int CM, int KIND>
-MCFFT_DEV void fft_pass_body(const PassArgs& a) {
+MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
@@ -311,7 +313,6 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
 
   const int tid = thread_id(), lane = lane_id(), wave = tid >> 6, h = lane >> 5;
   const bool inv = a.inverse != 0;
-  const AOps A = load_aops(a.aops, lane);
   const int sin_ = a.stride_in, sout = a.stride_out;
   const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
   constexpr int esz_out = (int)sizeof(El);
@@ -326,7 +327,7 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
     return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
   };
 
-  for (int tile = block_id(); tile < a.n_tiles; tile += grid_size()) {
+  {
     const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
     const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
     const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
@@ -484,6 +485,13 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
     }
     block_barrier();  // LDS is reused by the next tile
   }
+}
+
+// A pass kernel: the workgroups loop over the pass's tiles.
+template <int S, int T, int Q, int NW, int CM, int KIND>
+MCFFT_DEV void fft_pass_body(const PassArgs& a) {
+  const AOps A = load_aops(a.aops, lane_id());
+  for (int tile = block_id(); tile < a.n_tiles; tile += grid_size()) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
 }
 
 }  // namespace mcfft

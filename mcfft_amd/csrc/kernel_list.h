@@ -80,6 +80,9 @@ extern const KernelInfo kColKernels[];
 extern const int kColKernelCount;
 extern const KernelInfo kColFKernels[];
 extern const int kColFKernelCount;
+// ragged batches: pass variants per compute mode and the persistent kernels
+extern const KernelInfo kRaggedVariantInfo[3][30];
+extern const KernelInfo kRaggedKernels[3];
 }  // namespace mcfft
 
 #define MCFFT_KNAME(S, T, Q, NW, CM, K) mcfft_pass_##S##_##T##_##Q##_##NW##_##CM##_##K

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -13,6 +14,8 @@
 #include "../../include/mcfft.h"
 #include "fft_pass.h"
 #include "plan_core.h"
+#include "ragged.h"
+#include "ragged_plan.h"
 
 using namespace mcfft;
 
@@ -25,6 +28,13 @@ struct mcfftPlan_st {
   std::vector<int> grid_cap;              // resident workgroups per pass (occupancy x CUs)
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // ragged plans
+  bool ragged = false;
+  RaggedPlanCore rcore;
+  RaggedPass* rpasses[2] = {nullptr, nullptr};
+  RaggedTask* rtasks = nullptr;
+  unsigned* rcounters = nullptr;
+  int rgrid = 0;
 };
 
 namespace {
@@ -66,6 +76,12 @@ __global__ void gen_aops(const uint16_t* dft, int cm, uint32_t* out) {
   aops_words(dft, cm, lane, [&](int k, uint32_t w) { out[lane * 16 + k] = w; });
 }
 
+// zero the ragged queue head, status and transform counters (a kernel, so the
+// exec stays capturable into a hipGraph without memset nodes)
+__global__ void reset_counters(unsigned* c, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = 0u;
+}
+
 std::mutex g_attr_mu;
 std::vector<const void*> g_attr_done;  // kernels whose LDS limit was raised
 
@@ -87,7 +103,38 @@ void free_plan(mcfftPlan p) {
     if (p->aops[d]) (void)hipFree(p->aops[d]);
   }
   if (p->ws) (void)hipFree(p->ws);
+  for (int d = 0; d < 2; ++d)
+    if (p->rpasses[d]) (void)hipFree(p->rpasses[d]);
+  if (p->rtasks) (void)hipFree(p->rtasks);
+  if (p->rcounters) (void)hipFree(p->rcounters);
   delete p;
+}
+
+// FP64 twiddle tables, DFT tables and MFMA operand tables of a plan (both directions)
+mcfftStatus make_tables(mcfftPlan p, const std::vector<TwSpec>& tw, int cm) {
+  const long long ntw = (long long)tw.size();
+  TwSpec* dspec = nullptr;
+  mcfftStatus rs = MCFFT_SUCCESS;
+  if (ntw > 0 && hipMalloc(&dspec, sizeof(TwSpec) * ntw) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+  for (int d = 0; d < 2 && rs == MCFFT_SUCCESS; ++d) {
+    if (hipMalloc(&p->tw[d], sizeof(f32x2) * (ntw > 0 ? ntw : 1)) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+    if (rs == MCFFT_SUCCESS && hipMalloc(&p->dft[d], sizeof(uint16_t) * kDftTableEntries) != hipSuccess)
+      rs = MCFFT_ALLOC_FAILED;
+    if (rs == MCFFT_SUCCESS && hipMalloc(&p->aops[d], sizeof(u32x4) * 64 * 4) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+  }
+  if (rs == MCFFT_SUCCESS && ntw > 0 &&
+      hipMemcpy(dspec, tw.data(), sizeof(TwSpec) * ntw, hipMemcpyHostToDevice) != hipSuccess)
+    rs = MCFFT_HIP_ERROR;
+  for (int d = 0; d < 2 && rs == MCFFT_SUCCESS; ++d) {
+    const int sign = d == 0 ? -1 : 1;
+    if (ntw > 0) gen_twiddles<<<(unsigned)((ntw + 255) / 256), 256>>>(dspec, ntw, sign, p->tw[d]);
+    gen_dft_table<<<1, 64>>>(sign, p->dft[d]);
+    gen_aops<<<1, 64>>>(p->dft[d], cm, (uint32_t*)p->aops[d]);
+    if (hipGetLastError() != hipSuccess) rs = MCFFT_HIP_ERROR;
+  }
+  if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;
+  if (dspec) (void)hipFree(dspec);
+  return rs;
 }
 
 }  // namespace
@@ -100,7 +147,10 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
   *plan = nullptr;
   mcfftPlan p = new (std::nothrow) mcfftPlan_st();
   if (!p) return MCFFT_ALLOC_FAILED;
-  int st = plan_build(p->core, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, 0);
+  // MCFFT_WS_BYTES overrides the four-step workspace bound (bytes)
+  long long ws_limit = 0;
+  if (const char* env = getenv("MCFFT_WS_BYTES")) ws_limit = atoll(env);
+  int st = plan_build(p->core, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, ws_limit);
   if (st != ST_OK) {
     delete p;
     return (mcfftStatus)st;
@@ -127,28 +177,7 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     p->grid_cap.push_back((per_cu < 1 ? 1 : per_cu) * n_cu);
   }
   // tables: FP64 on the GPU, then rounded to FP32 / FP16 / BF16
-  const long long ntw = (long long)p->core.tw.size();
-  TwSpec* dspec = nullptr;
-  mcfftStatus rs = MCFFT_SUCCESS;
-  if (ntw > 0 && hipMalloc(&dspec, sizeof(TwSpec) * ntw) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
-  for (int d = 0; d < 2 && rs == MCFFT_SUCCESS; ++d) {
-    if (hipMalloc(&p->tw[d], sizeof(f32x2) * (ntw > 0 ? ntw : 1)) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
-    if (rs == MCFFT_SUCCESS && hipMalloc(&p->dft[d], sizeof(uint16_t) * kDftTableEntries) != hipSuccess)
-      rs = MCFFT_ALLOC_FAILED;
-    if (rs == MCFFT_SUCCESS && hipMalloc(&p->aops[d], sizeof(u32x4) * 64 * 4) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
-  }
-  if (rs == MCFFT_SUCCESS && ntw > 0 &&
-      hipMemcpy(dspec, p->core.tw.data(), sizeof(TwSpec) * ntw, hipMemcpyHostToDevice) != hipSuccess)
-    rs = MCFFT_HIP_ERROR;
-  for (int d = 0; d < 2 && rs == MCFFT_SUCCESS; ++d) {
-    const int sign = d == 0 ? -1 : 1;
-    if (ntw > 0) gen_twiddles<<<(unsigned)((ntw + 255) / 256), 256>>>(dspec, ntw, sign, p->tw[d]);
-    gen_dft_table<<<1, 64>>>(sign, p->dft[d]);
-    gen_aops<<<1, 64>>>(p->dft[d], p->core.cm, (uint32_t*)p->aops[d]);
-    if (hipGetLastError() != hipSuccess) rs = MCFFT_HIP_ERROR;
-  }
-  if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;
-  if (dspec) (void)hipFree(dspec);
+  mcfftStatus rs = make_tables(p, p->core.tw, p->core.cm);
   if (rs == MCFFT_SUCCESS && p->core.ws_elems > 0) {
     const int ws_fmt = (p->core.cm == CM_SPLIT) ? FMT_SPLIT : p->core.in_fmt;
     p->ws_bytes = (size_t)p->core.ws_elems * (size_t)elem_bytes(ws_fmt);
@@ -176,13 +205,175 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
   return mcfft_plan_many(plan, 3, d, batch, in_fmt, out_fmt, MCFFT_SCALE_BALANCED);
 }
 
+mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedItem* items, mcfftFormat in_fmt,
+                              mcfftFormat out_fmt, mcfftScaling scaling) {
+  if (!plan || !items || n_items < 1) return MCFFT_INVALID_VALUE;
+  *plan = nullptr;
+  std::vector<RaggedItemSpec> spec(n_items);
+  for (int i = 0; i < n_items; ++i) {
+    spec[i].rank = items[i].rank;
+    for (int d = 0; d < 3; ++d) spec[i].dims[d] = items[i].dims[d];
+    spec[i].in_off = items[i].in_offset;
+    spec[i].out_off = items[i].out_offset;
+  }
+  mcfftPlan p = new (std::nothrow) mcfftPlan_st();
+  if (!p) return MCFFT_ALLOC_FAILED;
+  p->ragged = true;
+  int st = ragged_build(p->rcore, n_items, spec.data(), (int)in_fmt, (int)out_fmt, (int)scaling);
+  if (st != ST_OK) {
+    delete p;
+    return (mcfftStatus)st;
+  }
+  RaggedPlanCore& r = p->rcore;
+  p->core.in_fmt = r.in_fmt;
+  p->core.out_fmt = r.out_fmt;
+  p->core.cm = r.cm;
+  const KernelInfo* k = &kRaggedKernels[r.cm];
+  int n_cu = 0, per_cu = 0;
+  mcfftStatus rs = MCFFT_SUCCESS;
+  if (hipGetDevice(&p->device) != hipSuccess ||
+      hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+    rs = MCFFT_HIP_ERROR;
+  if (rs == MCFFT_SUCCESS) rs = set_lds_attr(k);
+  if (rs == MCFFT_SUCCESS &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, k->nw * 64, k->lds_bytes) != hipSuccess)
+    rs = MCFFT_HIP_ERROR;
+  p->rgrid = (per_cu < 1 ? 1 : per_cu) * (n_cu < 1 ? 1 : n_cu);
+  if (rs == MCFFT_SUCCESS) rs = make_tables(p, r.tw, r.cm);
+  const int ws_fmt = (r.cm == CM_SPLIT) ? FMT_SPLIT : r.in_fmt;
+  if (rs == MCFFT_SUCCESS && r.ws_elems > 0) {
+    p->ws_bytes = (size_t)r.ws_elems * (size_t)elem_bytes(ws_fmt);
+    if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+  }
+  // device pass descriptors per direction, tasks, counters
+  const size_t np = r.passes.size(), nt = r.task_pass.size();
+  for (int d = 0; d < 2 && rs == MCFFT_SUCCESS; ++d) {
+    std::vector<RaggedPass> dp(np);
+    for (size_t i = 0; i < np; ++i) {
+      const RaggedPassHost& h = r.passes[i];
+      RaggedPass& q = dp[i];
+      memset(&q, 0, sizeof(q));
+      PassArgs& a = q.args;
+      const int src_esz = elem_bytes(h.pp.src_fmt);
+      const int dst_esz = elem_bytes(h.pp.dst == BUF_WS ? ws_fmt : r.out_fmt);
+      a.in = (const void*)(size_t)(h.in_off * src_esz);
+      a.out = (void*)(size_t)(h.out_off * dst_esz);
+      a.in_sg = h.pp.in_sg;
+      a.in_st = h.pp.in_st;
+      a.out_sg = h.pp.out_sg;
+      a.out_st = h.pp.out_st;
+      a.stride_in = h.pp.stride_in;
+      a.stride_out = h.pp.stride_out;
+      a.n_inst = (int)h.pp.n_inst;
+      a.tiles_per_group = h.pp.tiles_per_group;
+      a.n_tiles = (int)(h.pp.groups * h.pp.tiles_per_group);
+      a.in_fmt = h.pp.src_fmt;
+      a.inverse = d;
+      a.out_scale = h.pp.out_scale;
+      a.stage_tw = p->tw[d] + h.pp.stage_tw_off;
+      a.stage_tw_entries = (int)((long long)r.tw.size() - h.pp.stage_tw_off);
+      a.aops = p->aops[d];
+      a.in_elems = h.in_elems;
+      a.out_elems = h.out_elems;
+      if (h.pp.fourstep) {
+        a.fs_hi = p->tw[d] + h.pp.fs_hi_off;
+        a.fs_lo = p->tw[d] + h.pp.fs_lo_off;
+        a.fs_shift = h.pp.fs_shift;
+        a.fs_mask = h.pp.fs_mask;
+        a.fs_nmask = (unsigned)h.pp.fs_nmask;
+        a.fs_hi_entries = (int)(h.pp.fs_lo_off - h.pp.fs_hi_off);
+        a.fs_lo_entries = 1 << h.pp.fs_shift;
+      }
+      q.in_buf = h.pp.src;
+      q.out_buf = h.pp.dst;
+      q.variant = h.variant;
+      q.counter = h.counter;
+      q.wait_for = h.wait_for;
+      q.signal = h.signal;
+    }
+    if (hipMalloc(&p->rpasses[d], sizeof(RaggedPass) * np) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+    else if (hipMemcpy(p->rpasses[d], dp.data(), sizeof(RaggedPass) * np, hipMemcpyHostToDevice) != hipSuccess)
+      rs = MCFFT_HIP_ERROR;
+  }
+  if (rs == MCFFT_SUCCESS) {
+    std::vector<RaggedTask> tasks(nt);
+    for (size_t i = 0; i < nt; ++i) tasks[i] = RaggedTask{r.task_pass[i], r.task_tile[i]};
+    if (hipMalloc(&p->rtasks, sizeof(RaggedTask) * (nt ? nt : 1)) != hipSuccess ||
+        hipMalloc(&p->rcounters, sizeof(unsigned) * (2 + r.n_counters + 1)) != hipSuccess)
+      rs = MCFFT_ALLOC_FAILED;
+    else if (hipMemcpy(p->rtasks, tasks.data(), sizeof(RaggedTask) * nt, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemset(p->rcounters, 0, sizeof(unsigned) * (2 + r.n_counters + 1)) != hipSuccess)
+      rs = MCFFT_HIP_ERROR;
+  }
+  if (rs != MCFFT_SUCCESS) {
+    free_plan(p);
+    return rs;
+  }
+  *plan = p;
+  return MCFFT_SUCCESS;
+}
+
+static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int direction, hipStream_t s) {
+  const RaggedPlanCore& r = plan->rcore;
+  if (((size_t)in % (size_t)elem_bytes(r.in_fmt)) || ((size_t)out % (size_t)elem_bytes(r.out_fmt)))
+    return MCFFT_MISALIGNED;
+  const int d = direction == MCFFT_FORWARD ? 0 : 1;
+  const int nc = 2 + r.n_counters;
+  reset_counters<<<1, 256, 0, s>>>(plan->rcounters, nc);
+  if (hipGetLastError() != hipSuccess) return MCFFT_HIP_ERROR;
+  RaggedArgs ra;
+  ra.in_base = (const char*)in;
+  ra.out_base = (char*)out;
+  ra.ws_base = (char*)plan->ws;
+  ra.passes = plan->rpasses[d];
+  ra.tasks = plan->rtasks;
+  ra.n_tasks = (int)r.task_pass.size();
+  ra.counters = plan->rcounters;
+  ra.spin_limit = 1u << 24;
+  ra.aops = plan->aops[d];
+  const KernelInfo* k = &kRaggedKernels[r.cm];
+  const int grid = plan->rgrid < ra.n_tasks ? plan->rgrid : (ra.n_tasks > 0 ? ra.n_tasks : 1);
+  void* args[] = {&ra};
+  if (hipLaunchKernel(k->fn, dim3((unsigned)grid), dim3((unsigned)(k->nw * 64)), args, (size_t)k->lds_bytes, s) !=
+      hipSuccess)
+    return MCFFT_HIP_ERROR;
+  return MCFFT_SUCCESS;
+}
+
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
+  if (plan->ragged) {
+    if (!in || !out) return MCFFT_INVALID_VALUE;
+    if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
+    return exec_ragged(plan, in, out, direction, (hipStream_t)stream);
+  }
   return mcfft_exec_passes(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
 }
 
-mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count) {
+mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {
   if (!plan) return MCFFT_INVALID_PLAN;
+  if (!exponents) return MCFFT_INVALID_VALUE;
+  if (!plan->ragged) {
+    exponents[0] = plan->core.out_exponent;
+    return MCFFT_SUCCESS;
+  }
+  for (int i = 0; i < plan->rcore.n_items; ++i) exponents[i] = plan->rcore.exponent[i];
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags) {
+  if (!plan) return MCFFT_INVALID_PLAN;
+  if (!flags) return MCFFT_INVALID_VALUE;
+  *flags = 0;
+  if (!plan->ragged) return MCFFT_SUCCESS;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(flags, plan->rcounters + 1, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+    return MCFFT_HIP_ERROR;
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count) {
+  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
   if (!count) return MCFFT_INVALID_VALUE;
   *count = (int)plan->core.passes.size();
   return MCFFT_SUCCESS;
@@ -190,7 +381,7 @@ mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count) {
 
 mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
                               int count) {
-  if (!plan) return MCFFT_INVALID_PLAN;
+  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
   if (first < 0 || count < 0 || first + count > (int)plan->core.passes.size()) return MCFFT_INVALID_VALUE;
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
@@ -275,7 +466,7 @@ mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
 mcfftStatus mcfft_plan_describe(mcfftPlan plan, char* buf, size_t len) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!buf || len == 0) return MCFFT_INVALID_VALUE;
-  const std::string s = plan->core.describe();
+  const std::string s = plan->ragged ? plan->rcore.describe() : plan->core.describe();
   snprintf(buf, len, "%s", s.c_str());
   return MCFFT_SUCCESS;
 }

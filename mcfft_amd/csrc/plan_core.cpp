@@ -15,14 +15,14 @@ int compute_mode(int in_fmt, int out_fmt) {
   return -1;
 }
 
-static int ilog2(long long v) {
+int ilog2(long long v) {
   int r = 0;
   while ((1LL << r) < v) ++r;
   return ((1LL << r) == v) ? r : -1;
 }
 
 // L = 16^S * T with S >= 1, T in {1,2,4,8}
-static bool split_len(long long L, int& S, int& T) {
+bool split_len(long long L, int& S, int& T) {
   int lg = ilog2(L);
   if (lg < 4 || lg > 12) return false;
   S = lg / 4;
@@ -49,7 +49,7 @@ static int stage_shift(int radix, int scaling) {
   return 0;
 }
 
-static float pow2(int e) {  // 2^e, exact
+float pow2(int e) {  // 2^e, exact
   float s = 1.f;
   for (int i = 0; i < e; ++i) s *= 2.f;
   for (int i = 0; i > e; --i) s *= 0.5f;
@@ -60,14 +60,14 @@ static float pow2(int e) {  // 2^e, exact
 // and return the per-stage shifts.  Table of stage P, entry j*16+rr, is the
 // input twiddle of stage P+1 for the element stage P writes at
 // m' = (j/Ns)*Ns*16 + j%Ns + rr*Ns, times 2^-shift_P.
-static std::vector<int> add_stage_tables(PlanCore& p, PassPlan& pp, int S, int T, int scaling) {
+std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, int T, int scaling) {
   const long long L = (1LL << (4 * S)) * T;
   std::vector<int> radix;
   for (int i = 0; i < S; ++i) radix.push_back(16);
   if (T > 1) radix.push_back(T);
   std::vector<int> shifts;
   for (int r : radix) shifts.push_back(stage_shift(r, scaling));
-  pp.stage_tw_off = (long long)p.tw.size();
+  pp.stage_tw_off = (long long)tw.size();
   long long ns = 1;
   for (size_t P = 0; P + 1 < radix.size(); ++P) {
     const int R = radix[P];  // always 16 (the tail is last)
@@ -77,7 +77,7 @@ static std::vector<int> add_stage_tables(PlanCore& p, PassPlan& pp, int S, int T
       for (long long rr = 0; rr < R; ++rr) {
         const long long mo = (j / ns) * ns * R + (j % ns) + rr * ns;
         const long long r2 = mo / (L / Rn), j2 = mo % (L / Rn), k2 = j2 % nsn;
-        p.tw.push_back({(r2 * k2) % (nsn * Rn), nsn * Rn, pow2(-shifts[P])});
+        tw.push_back({(r2 * k2) % (nsn * Rn), nsn * Rn, pow2(-shifts[P])});
       }
     ns = nsn;
   }
@@ -123,7 +123,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     pp.n_inst = n_rows;
     pp.tiles_per_group = (int)((n_rows + k->Q - 1) / k->Q);
     pp.groups = 1;
-    pp.shifts = add_stage_tables(p, pp, S, T, scaling);
+    pp.shifts = add_stage_tables(p.tw, pp, S, T, scaling);
     pp.out_scale = pow2(-pp.shifts.back());
     for (int s : pp.shifts) total_shift += s;
     p.passes.push_back(pp);
@@ -149,7 +149,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     pp.tiles_per_group = (int)(ncols / k->Q);
     pp.groups = groups;
     pp.groups_per_transform = groups / batch;
-    pp.shifts = add_stage_tables(p, pp, S, T, scaling);
+    pp.shifts = add_stage_tables(p.tw, pp, S, T, scaling);
     pp.out_scale = pow2(-pp.shifts.back());
     for (int s : pp.shifts) total_shift += s;
     p.passes.push_back(pp);
@@ -198,7 +198,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       a.tiles_per_group = (int)(N2 / kc->Q);
       a.groups = chunk;
       a.groups_per_transform = 1;
-      a.shifts = add_stage_tables(p, a, S1, T1, scaling);
+      a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
       a.fourstep = true;
       a.fs_shift = (lg + 1) / 2;
       a.fs_mask = (unsigned)((1LL << a.fs_shift) - 1);
@@ -225,7 +225,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       b.tiles_per_group = (int)(N1 / kr->Q);
       b.groups = chunk;
       b.groups_per_transform = 1;
-      b.shifts = add_stage_tables(p, b, S2, T2, scaling);
+      b.shifts = add_stage_tables(p.tw, b, S2, T2, scaling);
       b.out_scale = pow2(-b.shifts.back());
       for (int s : b.shifts) total_shift += s;
       p.passes.push_back(b);

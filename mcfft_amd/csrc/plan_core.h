@@ -74,6 +74,11 @@ enum {
 };
 
 int elem_bytes(int fmt);
+// helpers shared with the ragged planner
+int ilog2(long long v);                          // log2 of a power of two, -1 otherwise
+bool split_len(long long L, int& S, int& T);     // L = 16^S * T, S >= 1, T in {1,2,4,8}
+float pow2(int e);                               // 2^e exactly
+std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, int T, int scaling);
 // compute mode from the input/output formats; -1 if the pair is not supported
 int compute_mode(int in_fmt, int out_fmt);
 // ws_limit_bytes: upper bound of the four-step workspace (0 = default)

@@ -1,0 +1,163 @@
+// ragged.h -- one persistent launch for a ragged batch: thousands of transforms
+// of different lengths and ranks, each split into per-pass tile tasks on a
+// device-side work queue.
+//
+// Workgroups take tasks in queue order with one atomic per task.  A task of a
+// transform's second (third) pass first waits until every tile of the earlier
+// passes of the same transform has been counted; the queue lists all first-pass
+// tasks before any second-pass task, so every awaited tile was taken earlier by
+// a resident workgroup that never waits on a later task -- the wait always ends.
+// It is bounded anyway (spin_limit naps), after which a status bit is set and
+// the task runs on whatever data is there (the result is then wrong, never a hang).
+//
+// No reference counterpart: the reference runs one size per launch sequence
+// (source/mcfft_1d_device.cpp:596-678).
+#pragma once
+#include "fft_pass.h"
+
+namespace mcfft {
+
+// A pass of one transform.  args.in / args.out hold BYTE OFFSETS from the
+// buffer selected by in_buf / out_buf (0 = exec input, 1 = exec output,
+// 2 = workspace); the table pointers are absolute.
+struct RaggedPass {
+  PassArgs args;
+  int in_buf, out_buf;
+  int variant;     // index into the ragged kernel list
+  int counter;     // per-transform tile counter
+  int wait_for;    // tiles of earlier passes that must be counted first (0 = none)
+  int signal;      // count this pass's tiles
+};
+
+struct RaggedTask {
+  int pass;
+  int tile;
+};
+
+struct RaggedArgs {
+  const char* in_base;
+  char* out_base;
+  char* ws_base;
+  const RaggedPass* passes;
+  const RaggedTask* tasks;
+  int n_tasks;
+  unsigned* counters;   // [0] queue head, [1] status, [2 + i] transform i
+  unsigned spin_limit;
+  const u32x4* aops;
+};
+
+// ragged kernel variants: X(id, S, T, Q, NW, CM, KIND); all share NW = 8 and
+// keep LDS <= 64 KiB (two workgroups per CU)
+#define MCFFT_RAGGED_LIST_F(X, CM, QS)     \
+  X(0, 1, 1, 256, 8, CM, 0)                \
+  X(1, 1, 2, 128, 8, CM, 0)                \
+  X(2, 1, 4, 64, 8, CM, 0)                 \
+  X(3, 1, 8, 32, 8, CM, 0)                 \
+  X(4, 2, 1, 16, 8, CM, 0)                 \
+  X(5, 2, 2, 8, 8, CM, 0)                  \
+  X(6, 2, 4, 4, 8, CM, 0)                  \
+  X(7, 2, 8, 2, 8, CM, 0)                  \
+  X(8, 3, 1, 1, 8, CM, 0)                  \
+  X(9, 1, 1, 32, 8, CM, 1)                 \
+  X(10, 1, 2, 32, 8, CM, 1)                \
+  X(11, 1, 4, 32, 8, CM, 1)                \
+  X(12, 1, 8, 32, 8, CM, 1)                \
+  X(13, 2, 1, 32 / QS, 8, CM, 1)           \
+  X(14, 2, 2, 16 / QS, 8, CM, 1)           \
+  X(15, 1, 1, 32, 8, CM, 2)                \
+  X(16, 1, 2, 32, 8, CM, 2)                \
+  X(17, 1, 4, 32, 8, CM, 2)                \
+  X(18, 1, 8, 32, 8, CM, 2)                \
+  X(19, 2, 1, 32 / QS, 8, CM, 2)           \
+  X(20, 2, 2, 32 / QS, 8, CM, 2)           \
+  X(21, 2, 4, 16 / QS, 8, CM, 2)           \
+  X(22, 2, 8, 8 / QS, 8, CM, 2)            \
+  X(23, 3, 1, 4 / QS, 8, CM, 2)            \
+  X(24, 1, 1, 32, 8, CM, 3)                \
+  X(25, 1, 2, 32, 8, CM, 3)                \
+  X(26, 1, 4, 32, 8, CM, 3)                \
+  X(27, 1, 8, 32, 8, CM, 3)                \
+  X(28, 2, 1, 32 / QS, 8, CM, 3)           \
+  X(29, 2, 2, 32 / QS, 8, CM, 3)
+
+constexpr int kRaggedVariants = 30;
+constexpr int kRaggedNW = 8;
+
+template <int CM>
+constexpr int ragged_max_lds() {
+  int m = 0;
+#define MCFFT_RAGGED_MAXLDS(ID, S, T, Q, NW, CMx, K) \
+  if (PassCfg<S, T, Q, NW, CMx, K>::LDS_BYTES > m) m = PassCfg<S, T, Q, NW, CMx, K>::LDS_BYTES;
+  MCFFT_RAGGED_LIST_F(MCFFT_RAGGED_MAXLDS, CM, (CM == CM_SPLIT ? 2 : 1))
+#undef MCFFT_RAGGED_MAXLDS
+  return m;
+}
+// dynamic LDS of the ragged kernel: the largest variant plus the task slot
+template <int CM>
+constexpr int ragged_lds_bytes() {
+  return ragged_max_lds<CM>() + 16;
+}
+
+// each variant is an out-of-line call, so its address arithmetic is not hoisted
+// into the task loop next to every other variant's (which would spill)
+template <int S, int T, int Q, int NW, int CM, int KIND>
+MCFFT_NOINLINE void ragged_tile(const PassArgs& a, int tile, const AOps& A) {
+  fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+}
+
+template <int CM>
+MCFFT_DEV void ragged_dispatch(int variant, const PassArgs& a, int tile, const AOps& A) {
+  switch (variant) {
+#define MCFFT_RAGGED_CASE(ID, S, T, Q, NW, CMx, K) \
+  case ID:                                         \
+    ragged_tile<S, T, Q, NW, CMx, K>(a, tile, A);  \
+    break;
+    MCFFT_RAGGED_LIST_F(MCFFT_RAGGED_CASE, CM, (CM == CM_SPLIT ? 2 : 1))
+#undef MCFFT_RAGGED_CASE
+    default:
+      break;
+  }
+}
+
+template <int CM>
+MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
+  const int tid = thread_id();
+  const AOps A = load_aops(r.aops, lane_id());
+  MCFFT_DYN_LDS(unsigned char, ldsb);
+  int* slot = (int*)(ldsb + task_slot_offset);
+  for (;;) {
+    if (tid == 0) lds_st(slot, 0, (int)atomic_add_dev(&r.counters[0], 1u));
+    block_barrier();
+    const int ti = uniform(lds_ld(slot, 0));
+    if (ti >= r.n_tasks) break;  // every workgroup leaves through here
+    const RaggedTask t = r.tasks[ti];
+    const RaggedPass& rp = r.passes[t.pass];
+    if (rp.wait_for > 0) {
+      if (tid == 0) {
+        unsigned it = 0;
+        while (atomic_load_dev(&r.counters[2 + rp.counter]) < (unsigned)rp.wait_for) {
+          if (++it > r.spin_limit) {
+            atomic_or_dev(&r.counters[1], 1u);
+            break;
+          }
+          nap();
+        }
+      }
+      block_barrier();
+      fence_dev();
+    }
+    PassArgs a = rp.args;
+    const char* ib = rp.in_buf == 0 ? r.in_base : (rp.in_buf == 1 ? (const char*)r.out_base : (const char*)r.ws_base);
+    char* ob = rp.out_buf == 1 ? r.out_base : r.ws_base;
+    a.in = ib + (size_t)rp.args.in;
+    a.out = (void*)(ob + (size_t)rp.args.out);
+    ragged_dispatch<CM>(rp.variant, a, t.tile, A);  // ends with a barrier
+    if (rp.signal) {
+      fence_dev();
+      block_barrier();
+      if (tid == 0) atomic_add_dev(&r.counters[2 + rp.counter], 1u);
+    }
+  }
+}
+
+}  // namespace mcfft

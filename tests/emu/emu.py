@@ -88,3 +88,41 @@ def describe(dims, batch, in_fmt, out_fmt, scaling=1):
     if rc != 0:
         raise RuntimeError(f"plan status {rc}")
     return buf.value.decode()
+
+
+def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=False, nblocks=3):
+    """Ragged batch: xs = list of complex arrays, shapes = list of dims lists.
+    layout: list of (in_off, out_off) element offsets (default back to back).
+    Returns (list of outputs * 2^exponent, status)."""
+    n_items = len(xs)
+    sizes = [int(np.prod(s)) for s in shapes]
+    if layout is None:
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        layout = [(int(o), int(o)) for o in offs]
+    in_total = max(o[0] + s for o, s in zip(layout, sizes))
+    out_total = max(o[1] + s for o, s in zip(layout, sizes))
+    src_c = np.zeros(in_total, dtype=np.complex128)
+    for x, (io, _), s in zip(xs, layout, sizes):
+        src_c[io:io + s] = x
+    src = to_storage(src_c, in_fmt)
+    out = np.zeros(out_total * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    spec = np.zeros((n_items, 6), dtype=np.int64)
+    for i, (sh, (io, oo)) in enumerate(zip(shapes, layout)):
+        spec[i, 0] = len(sh)
+        spec[i, 1:1 + len(sh)] = sh
+        for d in range(len(sh), 3):
+            spec[i, 1 + d] = 1
+        spec[i, 4], spec[i, 5] = io, oo
+    exps = np.zeros(n_items, dtype=np.int32)
+    st = ctypes.c_uint(0)
+    L = lib()
+    L.emu_ragged_exec.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint)]
+    L.emu_ragged_exec.restype = ctypes.c_int
+    rc = L.emu_ragged_exec(n_items, spec.ctypes.data, in_fmt, out_fmt, scaling, int(inverse), src.ctypes.data,
+                           out.ctypes.data, nblocks, exps.ctypes.data, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"emu_ragged_exec status {rc}")
+    y = from_storage(out, out_fmt)
+    outs = [y[oo:oo + s] * (2.0 ** e) for (io, oo), s, e in zip(layout, sizes, exps)]
+    return outs, st.value

@@ -10,6 +10,8 @@
 
 #include "fft_pass.h"
 #include "plan_core.h"
+#include "ragged.h"
+#include "ragged_plan.h"
 
 using namespace mcfft;
 
@@ -131,5 +133,102 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       emu::launch((int)grid, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); });
     }
   }
+  return 0;
+}
+
+extern "C" void mcfft_ragged_0(RaggedArgs r);
+extern "C" void mcfft_ragged_1(RaggedArgs r);
+extern "C" void mcfft_ragged_2(RaggedArgs r);
+
+// Ragged batch through the persistent ragged kernel, `nblocks` workgroups run
+// concurrently.  items: n x {rank, d0, d1, d2, in_off, out_off} (long long).
+extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, int out_fmt, int scaling, int inverse,
+                               const void* in, void* out, int nblocks, int* exponents, unsigned* status) {
+  std::vector<RaggedItemSpec> items(n_items);
+  for (int i = 0; i < n_items; ++i) {
+    const long long* q = spec + 6 * i;
+    items[i].rank = (int)q[0];
+    items[i].dims[0] = q[1];
+    items[i].dims[1] = q[2];
+    items[i].dims[2] = q[3];
+    items[i].in_off = q[4];
+    items[i].out_off = q[5];
+  }
+  RaggedPlanCore r;
+  int st = ragged_build(r, n_items, items.data(), in_fmt, out_fmt, scaling);
+  if (st) return st;
+  for (int i = 0; i < n_items; ++i) exponents[i] = r.exponent[i];
+  const int sign = inverse ? 1 : -1;
+  std::vector<f32x2> tw(r.tw.size() + 1);
+  for (size_t i = 0; i < r.tw.size(); ++i) twiddle_host(r.tw[i], sign, tw[i]);
+  std::vector<uint16_t> dft(kDftTableEntries);
+  dft_table_host(sign, dft.data());
+  std::vector<u32x4> aops(64 * 4);
+  for (int lane = 0; lane < 64; ++lane)
+    aops_words(dft.data(), r.cm, lane, [&](int k, uint32_t w) { aops[lane * 4 + k / 4][k % 4] = w; });
+  const int ws_fmt = (r.cm == 2) ? 2 : in_fmt;
+  std::vector<unsigned char> ws((size_t)(r.ws_elems * elem_bytes(ws_fmt)) + 16);
+  std::vector<RaggedPass> dp(r.passes.size());
+  for (size_t i = 0; i < r.passes.size(); ++i) {
+    const RaggedPassHost& h = r.passes[i];
+    RaggedPass& d = dp[i];
+    memset(&d, 0, sizeof(d));
+    PassArgs& a = d.args;
+    const int src_esz = elem_bytes(h.pp.src_fmt);
+    const int dst_esz = elem_bytes(h.pp.dst == BUF_WS ? ws_fmt : out_fmt);
+    a.in = (const void*)(size_t)(h.in_off * src_esz);
+    a.out = (void*)(size_t)(h.out_off * dst_esz);
+    a.in_sg = h.pp.in_sg; a.in_st = h.pp.in_st; a.out_sg = h.pp.out_sg; a.out_st = h.pp.out_st;
+    a.stride_in = h.pp.stride_in; a.stride_out = h.pp.stride_out;
+    a.n_inst = (int)h.pp.n_inst;
+    a.tiles_per_group = h.pp.tiles_per_group;
+    a.n_tiles = (int)(h.pp.groups * h.pp.tiles_per_group);
+    a.in_fmt = h.pp.src_fmt;
+    a.inverse = inverse;
+    a.out_scale = h.pp.out_scale;
+    a.stage_tw = tw.data() + h.pp.stage_tw_off;
+    a.stage_tw_entries = (int)((long long)r.tw.size() - h.pp.stage_tw_off);
+    a.aops = aops.data();
+    a.in_elems = h.in_elems;
+    a.out_elems = h.out_elems;
+    if (h.pp.fourstep) {
+      a.fs_hi = tw.data() + h.pp.fs_hi_off;
+      a.fs_lo = tw.data() + h.pp.fs_lo_off;
+      a.fs_shift = h.pp.fs_shift;
+      a.fs_mask = h.pp.fs_mask;
+      a.fs_nmask = (unsigned)h.pp.fs_nmask;
+      a.fs_hi_entries = (int)(h.pp.fs_lo_off - h.pp.fs_hi_off);
+      a.fs_lo_entries = 1 << h.pp.fs_shift;
+    }
+    d.in_buf = h.pp.src;
+    d.out_buf = h.pp.dst;
+    d.variant = h.variant;
+    d.counter = h.counter;
+    d.wait_for = h.wait_for;
+    d.signal = h.signal;
+  }
+  std::vector<RaggedTask> tasks(r.task_pass.size());
+  for (size_t i = 0; i < tasks.size(); ++i) tasks[i] = RaggedTask{r.task_pass[i], r.task_tile[i]};
+  std::vector<unsigned> counters(2 + r.n_counters + 1, 0u);
+  emu::clear_ranges();
+  emu::register_range(in, (size_t)(r.in_extent * elem_bytes(in_fmt)));
+  emu::register_range(out, (size_t)(r.out_extent * elem_bytes(out_fmt)));
+  emu::register_range(ws.data(), (size_t)(r.ws_elems * elem_bytes(ws_fmt)));
+  emu::register_range(tw.data(), r.tw.size() * sizeof(f32x2));
+  emu::register_range(aops.data(), aops.size() * sizeof(u32x4));
+  RaggedArgs ra;
+  ra.in_base = (const char*)in;
+  ra.out_base = (char*)out;
+  ra.ws_base = (char*)ws.data();
+  ra.passes = dp.data();
+  ra.tasks = tasks.data();
+  ra.n_tasks = (int)tasks.size();
+  ra.counters = counters.data();
+  ra.spin_limit = 1u << 22;
+  ra.aops = aops.data();
+  void (*fn)(RaggedArgs) = r.cm == 0 ? mcfft_ragged_0 : r.cm == 1 ? mcfft_ragged_1 : mcfft_ragged_2;
+  const int lds = kRaggedKernels[r.cm].lds_bytes;
+  emu::launch_concurrent(nblocks, kRaggedNW * 64, (size_t)lds, [&] { fn(ra); });
+  *status = counters[1];
   return 0;
 }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #define MCFFT_DEV inline
+#define MCFFT_NOINLINE __attribute__((noinline))
 #define MCFFT_KERNEL(nt) extern "C"
 #define MCFFT_TKERNEL(nt)
 #define MCFFT_DYN_LDS(type, name) type* name = (type*)::emu::lds_base()
@@ -52,6 +53,34 @@ inline thread_local int tl_tid = 0;
 inline void* lds_base() { return tl_blk->lds.data(); }
 inline size_t lds_bytes() { return tl_blk->lds.size(); }
 
+// launch all workgroups at once (each as nt host threads): needed by kernels whose
+// workgroups wait on each other (the ragged work queue)
+inline void launch_concurrent(int nblocks, int nt, size_t lds, const std::function<void()>& body) {
+  std::vector<Block*> blks;
+  std::vector<std::barrier<>*> bars;
+  std::vector<std::thread> th;
+  for (int b = 0; b < nblocks; ++b) {
+    Block* blk = new Block();
+    blk->nt = nt;
+    blk->bid = b;
+    blk->nblocks = nblocks;
+    blk->lds.assign(lds, 0xCD);
+    blk->bar = new std::barrier<>(nt);
+    for (int w = 0; w < nt / 64; ++w) blk->wbar.push_back(new std::barrier<>(64));
+    blk->xa.assign((size_t)(nt / 64) * 64 * 8, 0.f);
+    blk->xb.assign((size_t)(nt / 64) * 64 * 8, 0.f);
+    blks.push_back(blk);
+  }
+  for (int b = 0; b < nblocks; ++b)
+    for (int t = 0; t < nt; ++t) th.emplace_back([&, b, t] { tl_blk = blks[b]; tl_tid = t; body(); });
+  for (auto& x : th) x.join();
+  for (Block* blk : blks) {
+    delete blk->bar;
+    for (auto* w : blk->wbar) delete w;
+    delete blk;
+  }
+}
+
 // launch `body` as a grid of nblocks workgroups of nt threads with `lds` bytes of LDS
 inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>& body) {
   for (int b = 0; b < nblocks; ++b) {
@@ -82,6 +111,7 @@ inline int lane_id() { return emu::tl_tid % 64; }
 inline int thread_id() { return emu::tl_tid; }
 inline int block_id() { return emu::tl_blk->bid; }
 inline int grid_size() { return emu::tl_blk->nblocks; }
+inline int uniform(int v) { return v; }
 inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
 
 template <class T> inline T lds_ld(const T* p, int i) {
@@ -122,6 +152,12 @@ inline u32x2 buf_ld64(GBuf b, int off) { return buf_ld<u32x2>(b, off); }
 inline u32x4 buf_ld128(GBuf b, int off) { return buf_ld<u32x4>(b, off); }
 inline void buf_st32(GBuf b, int off, uint32_t v) { buf_st<uint32_t>(b, off, v); }
 inline void buf_st64(GBuf b, int off, u32x2 v) { buf_st<u32x2>(b, off, v); }
+
+inline unsigned atomic_add_dev(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+inline void atomic_or_dev(unsigned* p, unsigned v) { (void)__atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline void fence_dev() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+inline void nap() { std::this_thread::yield(); }
 
 template <class Dec>
 inline f32x16 emu_mfma(u32x4 a, u32x4 b, f32x16 c, Dec dec) {

@@ -78,3 +78,31 @@ def test_plan_describe_and_errors():
         emu.describe([256], 1, 2, 0)         # split in, fp16 out
     with pytest.raises(RuntimeError, match="status 1"):
         emu.describe([256], 0, 0, 0)         # batch 0
+
+
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_ragged_queue(fi, fo):
+    """Mixed lengths/ranks through the persistent ragged kernel; workgroups run
+    concurrently, so second-pass tasks really wait on first-pass counters."""
+    rng = np.random.default_rng(3)
+    shapes = [[256], [256], [256], [8192], [64, 32], [4096], [512], [512], [16384], [16, 16, 32], [32], [2048],
+              [128], [65536], [16], [16]]
+    xs = [rng.standard_normal(int(np.prod(s))) + 1j * rng.standard_normal(int(np.prod(s))) for s in shapes]
+    ys, status = emu.ragged_exec(xs, shapes, fi, fo, nblocks=4)
+    assert status == 0
+    for x, s, y in zip(xs, shapes, ys):
+        xq = emu.from_storage(emu.to_storage(x, fi), fi)
+        err = oracle.rel_l2(y, oracle.fft(xq, tuple(s)))
+        assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (s, err)
+
+
+def test_ragged_offsets_and_inverse():
+    rng = np.random.default_rng(4)
+    shapes = [[1024], [8192], [32, 64]]
+    layout = [(20000, 0), (0, 30000), (9000, 1024)]
+    xs = [rng.standard_normal(int(np.prod(s))) + 1j * rng.standard_normal(int(np.prod(s))) for s in shapes]
+    ys, status = emu.ragged_exec(xs, shapes, emu.FMT_F16, emu.FMT_SPLIT, layout=layout, inverse=True, nblocks=3)
+    assert status == 0
+    for x, s, y in zip(xs, shapes, ys):
+        xq = emu.from_storage(emu.to_storage(x, emu.FMT_F16), emu.FMT_F16)
+        assert oracle.rel_l2(y, oracle.fft(xq, tuple(s), sign=+1)) <= 1e-6

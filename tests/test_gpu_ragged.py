@@ -1,0 +1,105 @@
+"""GPU: ragged batches (one persistent launch over a device-side work queue) and
+hipGraph capture of exec calls, against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import mcfft_amd as mc
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = {"fp16": 1.5e-3, "split": 1e-6}
+
+
+def _mixed(seed, n_small=200):
+    rng = np.random.default_rng(seed)
+    shapes = [[int(2 ** k)] for k in rng.integers(4, 19, n_small)]
+    shapes += [[256, 512], [64, 64], [16, 16, 32], [1024, 256], [32, 2048]]
+    rng.shuffle(shapes)
+    return shapes
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "split"])
+@pytest.mark.parametrize("sorted_layout", [False, True])
+def test_ragged_matches_oracle(torch_cuda, fmt, sorted_layout):
+    torch = torch_cuda
+    shapes = _mixed(1)
+    if sorted_layout:
+        shapes.sort(key=lambda s: (len(s), int(np.prod(s))))
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else ("fp16", "fp16")
+    p = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], in_fmt, out_fmt)
+    d_in = mc.to_device(x, in_fmt)
+    d_out = mc.empty(total, out_fmt)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, out_fmt)
+    xq = mc.decode(mc.encode(x, in_fmt), in_fmt)
+    worst = 0.0
+    for s, o, e in zip(shapes, offs, p.exponents):
+        n = int(np.prod(s))
+        err = oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s)))
+        worst = max(worst, err)
+    print(f"ragged {fmt} sorted={sorted_layout}: {len(shapes)} items, worst rel_l2 {worst:.3e}")
+    assert worst <= TOL[fmt]
+
+
+def test_ragged_inverse_and_gaps(torch_cuda):
+    """Items at arbitrary (gapped, out-of-order) offsets, inverse direction."""
+    torch = torch_cuda
+    shapes = [[4096], [131072], [512], [128, 128]]
+    sizes = [int(np.prod(s)) for s in shapes]
+    in_offs = [300000, 0, 140000, 150000]
+    out_offs = [0, 10000, 150000, 200000]
+    rng = np.random.default_rng(5)
+    total_in, total_out = 320000, 220000
+    x = rng.standard_normal(total_in) + 1j * rng.standard_normal(total_in)
+    p = mc.RaggedPlan([(s, i, o) for s, i, o in zip(shapes, in_offs, out_offs)], "fp16", "split")
+    d_in = mc.to_device(x, "fp16")
+    d_out = torch.zeros((total_out, 4), dtype=torch.float16, device="cuda")
+    p.exec(d_in, d_out, mc.INVERSE)
+    torch.cuda.synchronize()
+    y = mc.from_device(d_out, "split")
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    for s, n, i, o, e in zip(shapes, sizes, in_offs, out_offs, p.exponents):
+        err = oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[i:i + n], tuple(s), sign=+1))
+        assert err <= 1e-6, (s, err)
+
+
+def test_graph_capture_replays(torch_cuda):
+    """exec calls captured into a hipGraph (through torch.cuda.graph) give the
+    eager result on every replay, with eager work between replays."""
+    torch = torch_cuda
+    n, b = 131072, 8
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(n * b) + 1j * rng.standard_normal(n * b)
+    p = mc.Plan([n], b, "fp16", "split")
+    pr = mc.RaggedPlan([([4096], 0, 0), ([65536], 4096, 4096), ([256, 256], 69632, 69632)], "fp16", "fp16")
+    d_in = mc.to_device(x, "fp16")
+    d_out = mc.empty(n * b, "split")
+    d_rout = mc.empty(n * b, "fp16")
+    p.exec(d_in, d_out, mc.FORWARD)
+    pr.exec(d_in, d_rout, mc.FORWARD)
+    torch.cuda.synchronize()
+    ref, rref = d_out.clone(), d_rout.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            p.exec(d_in, d_out, mc.FORWARD)
+            pr.exec(d_in, d_rout, mc.FORWARD)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        d_out.zero_()
+        d_rout.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(d_out, ref), rep
+        assert torch.equal(d_rout, rref), rep
+    assert pr.status_flags() == 0
