@@ -1,0 +1,193 @@
+"""Throughput and accuracy of every workload the library is measured on.
+
+    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c5,c6] [--reps 10]
+
+One JSON line per (config, precision): GFLOP/s (5 N log2 N, N = elements per
+transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
+(one read of the input + one write of the output in their storage formats),
+and rel-L2 / max-abs error against an FP64 FFT of the same quantised input
+(computed on a sample of transforms where the full batch would take long on the
+host).  C5 compares the single persistent ragged launch with per-shape batched
+launches over the same data.
+
+The FP64 reference used for the error columns is the repository's CPU oracle
+(oracle/), imported only here, as the checker.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import mcfft_amd as mc  # noqa: E402
+
+HBM = 8000.0  # GB/s
+MALL = 256 << 20
+
+
+def _timer(torch, fn, reps, warmup=2):
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(reps):
+        fn(i)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _bytes(fi, fo):
+    return (8 if fi == "split" else 4) + (8 if fo == "split" else 4)
+
+
+def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transforms, direction=mc.FORWARD):
+    import oracle
+    n = int(np.prod(dims))
+    p = mc.Plan(dims, batch, fi, fo, scaling)
+    d_in = mc.to_device(x, fi)
+    # rotate inputs so the timed region reads HBM, not the Infinity Cache
+    rot = max(1, min(4, (2 * MALL) // max(1, d_in.numel() * d_in.element_size()) + 1))
+    ins = [d_in] + [d_in.clone() for _ in range(rot - 1)]
+    d_out = mc.empty(n * batch, fo)
+    ms = _timer(torch, lambda i: p.exec(ins[i % rot], d_out, direction), reps)
+    flops = 5.0 * n * math.log2(n) * batch
+    res = {"config": name, "dims": list(dims), "batch": batch, "in": fi, "out": fo, "scaling": scaling,
+           "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round(n * batch * _bytes(fi, fo) / (ms * 1e-3) / 1e9 / HBM, 4)}
+    p.exec(ins[0], d_out, direction)
+    torch.cuda.synchronize()
+    k = min(batch, check_transforms)
+    y = mc.from_device(d_out[: k * n], fo, p.exponent)
+    xq = mc.decode(mc.encode(x[: k * n], fi), fi)
+    ref = oracle.fft(xq, tuple(dims), sign=-1 if direction == mc.FORWARD else 1)
+    res.update({"rel_l2": oracle.rel_l2(y, ref), "max_abs": oracle.max_abs(y, ref), "checked": k})
+    p.destroy()
+    return res
+
+
+def config_inputs(cfg, rng, dims, batch):
+    n = int(np.prod(dims)) * batch
+    if cfg in ("c1", "c3", "c6"):
+        x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    else:
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    if cfg == "c3":  # + amplitude-100 2D exponential at a random (kx, ky) per transform
+        nx, ny = dims
+        xx = x.reshape(batch, nx, ny)
+        m = np.arange(nx)[:, None]
+        k = np.arange(ny)[None, :]
+        for b in range(batch):
+            kx, ky = rng.integers(0, nx), rng.integers(0, ny)
+            xx[b] += 100.0 * np.exp(2j * np.pi * (kx * m / nx + ky * k / ny))
+        x = xx.reshape(-1)
+    return x
+
+
+def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
+    """20000 1D transforms, N = 2^k with k uniform in 8..18, plus 64 2D transforms
+    with sides from {256, 512, 1024, 2048}; laid out grouped by shape."""
+    ks = rng.integers(8, 19, 20000)
+    shapes = [[int(2 ** k)] for k in ks]
+    sides = [256, 512, 1024, 2048]
+    shapes += [[int(rng.choice(sides)), int(rng.choice(sides))] for _ in range(64)]
+    shapes.sort(key=lambda s: (len(s), s))
+    sizes = np.array([int(np.prod(s)) for s in shapes], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sizes.sum())
+    flops = float(sum(5.0 * s * math.log2(s) for s in sizes))
+    # N(0,1) re/im generated on the device in FP16 (about 1e9 elements)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    d_in = torch.randn((total, 2), dtype=torch.float16, device="cuda", generator=gen)
+    d_out = mc.empty(total, fo)
+    rp = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], fi, fo)
+    ms_r = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
+    flags = rp.status_flags()
+    # per-shape batched launches over the same buffers
+    groups = {}
+    for s, o in zip(shapes, offs):
+        groups.setdefault(tuple(s), []).append(int(o))
+    plans = []
+    for s, os_ in groups.items():
+        n = int(np.prod(s))
+        plans.append((mc.Plan(list(s), len(os_), fi, fo), os_[0], n * len(os_)))
+    esz_in, esz_out = d_in.shape[1], d_out.shape[1]
+
+    def per_shape(i):
+        for p, o, cnt in plans:
+            p.exec_ptr(d_in.data_ptr() + o * esz_in * d_in.element_size(),
+                       d_out.data_ptr() + o * esz_out * d_out.element_size(), mc.FORWARD,
+                       torch.cuda.current_stream().cuda_stream)
+
+    ms_p = _timer(torch, per_shape, reps)
+    # accuracy on a sample of items (ragged output)
+    import oracle
+    rp.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    sample = rng.choice(len(shapes), 40, replace=False)
+    worst = 0.0
+    for i in sample:
+        s, o, n = shapes[i], int(offs[i]), int(sizes[i])
+        y = mc.from_device(d_out[o:o + n], fo, rp.exponents[i])
+        xq = mc.from_device(d_in[o:o + n], fi)
+        worst = max(worst, oracle.rel_l2(y, oracle.fft(xq, tuple(s))))
+    res = {"config": "c5", "items": len(shapes), "elements": total, "in": fi, "out": fo,
+           "ragged_ms": round(ms_r, 4), "ragged_gflops": round(flops / (ms_r * 1e-3) / 1e9, 1),
+           "ragged_hbm_frac": round(total * _bytes(fi, fo) / (ms_r * 1e-3) / 1e9 / HBM, 4),
+           "per_shape_launches": len(plans), "per_shape_ms": round(ms_p, 4),
+           "speedup_vs_per_shape": round(ms_p / ms_r, 3), "status_flags": flags,
+           "worst_rel_l2_sample": worst}
+    for p, _, _ in plans:
+        p.destroy()
+    rp.destroy()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c1,c2,c3,c4,c5,c6")
+    ap.add_argument("--reps", type=int, default=10)
+    args = ap.parse_args()
+    import torch
+    cfgs = args.config.split(",")
+    rng = np.random.default_rng(1)
+    table = {
+        "c1": ([256], 4096, "balanced", 4096),
+        "c2": ([131072], 256, "balanced", 8),
+        "c3": ([4096, 4096], 4, "full", 1),
+        "c4": ([512, 512, 512], 1, "balanced", 1),
+        "c6": ([65536], 4096, "balanced", 16),
+    }
+    for cfg in cfgs:
+        if cfg == "c5":
+            for fi, fo in (("fp16", "fp16"), ("fp16", "split")):
+                print(json.dumps(bench_c5(torch, args.reps, np.random.default_rng(5), fi, fo)), flush=True)
+            continue
+        dims, batch, scaling, check = table[cfg]
+        x = config_inputs(cfg, rng, dims, batch)
+        precs = [("fp16", "fp16"), ("fp16", "split")]
+        if cfg == "c3":
+            precs.append(("bf16", "bf16"))
+        for fi, fo in precs:
+            r = run_batched(torch, cfg, dims, batch, x, fi, fo, scaling, args.reps, check)
+            print(json.dumps(r), flush=True)
+        if cfg == "c2":  # the inverse leg of the forward+inverse config
+            r = run_batched(torch, "c2-inverse", dims, batch, x, "split", "split", "balanced", args.reps, check,
+                            direction=mc.INVERSE)
+            print(json.dumps(r), flush=True)
+        del x
+
+
+if __name__ == "__main__":
+    main()

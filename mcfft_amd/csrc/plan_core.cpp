@@ -108,7 +108,9 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   const int ws_fmt = (cm == 2) ? 2 : in_fmt;
   int total_shift = 0;
 
-  auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt) -> int {
+  // contiguous rows of length L: `n_rows` rows per group; with per_transform
+  // the groups are the chunk's transforms (rows = n_rows per transform)
+  auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt, bool per_transform) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
     const KernelInfo* k = find_kernel(S, T, cm, KIND_ROW);
@@ -118,11 +120,12 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     pp.src = src;
     pp.dst = dst;
     pp.src_fmt = src_fmt;
-    pp.in_sg = pp.out_sg = 0;
+    pp.in_sg = pp.out_sg = per_transform ? p.n : 0;
     pp.in_st = pp.out_st = (long long)k->Q * L;
     pp.n_inst = n_rows;
     pp.tiles_per_group = (int)((n_rows + k->Q - 1) / k->Q);
-    pp.groups = 1;
+    pp.groups = per_transform ? batch : 1;
+    pp.groups_per_transform = per_transform ? 1 : 0;
     pp.shifts = add_stage_tables(p.tw, pp, S, T, scaling);
     pp.out_scale = pow2(-pp.shifts.back());
     for (int s : pp.shifts) total_shift += s;
@@ -162,7 +165,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     if (N <= 4096) {
       p.chunk = batch;
       p.n_chunks = 1;
-      st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt);
+      st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);
     } else {
       // four-step: N = N1 * N2; columns of length N1 (stride N2) with twiddle
       // W_N^(k1*n2), then rows of length N2 stored transposed X[k1 + N1*k2]
@@ -231,16 +234,22 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       p.passes.push_back(b);
     }
   } else if (rank == 2) {
+    // transforms run in chunks whose data fits the Infinity Cache, so the
+    // column pass reads what the row pass just wrote from the cache
     const long long NX = p.dims[0], NY = p.dims[1];
-    p.chunk = batch;
-    p.n_chunks = 1;
-    st = add_row(NY, batch * NX, BUF_IN, BUF_OUT, in_fmt);
+    const long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (128LL << 20);
+    long long chunk = limit / (p.n * elem_bytes(p.out_fmt));
+    if (chunk < 1) chunk = 1;
+    if (chunk > batch) chunk = batch;
+    p.chunk = chunk;
+    p.n_chunks = (batch + chunk - 1) / chunk;
+    st = add_row(NY, NX, BUF_IN, BUF_OUT, in_fmt, true);
     if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
   } else {
     const long long NX = p.dims[0], NY = p.dims[1], NZ = p.dims[2];
     p.chunk = batch;
     p.n_chunks = 1;
-    st = add_row(NZ, batch * NX * NY, BUF_IN, BUF_OUT, in_fmt);
+    st = add_row(NZ, NX * NY, BUF_IN, BUF_OUT, in_fmt, true);
     if (st == ST_OK) st = add_col(NY, NZ, NZ, batch * NX, NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
     if (st == ST_OK) st = add_col(NX, NY * NZ, NY * NZ, batch, NX * NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
   }

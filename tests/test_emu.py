@@ -106,3 +106,11 @@ def test_ragged_offsets_and_inverse():
     for x, s, y in zip(xs, shapes, ys):
         xq = emu.from_storage(emu.to_storage(x, emu.FMT_F16), emu.FMT_F16)
         assert oracle.rel_l2(y, oracle.fft(xq, tuple(s), sign=+1)) <= 1e-6
+
+
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_multi_dim_chunked(fi, fo):
+    """2D plans run transform chunks (sized to stay cache resident); force one
+    transform per chunk and a partial last chunk."""
+    check([64, 32], 3, fi, fo, ws_limit=64 * 32 * 8)
+    check([32, 16, 32], 2, fi, fo, ws_limit=1)
