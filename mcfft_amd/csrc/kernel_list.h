@@ -19,12 +19,17 @@
   X(2, 2, 4, 4, CM, 0)            \
   X(2, 4, 2, 4, CM, 0)            \
   X(2, 8, 1, 4, CM, 0)            \
-  X(3, 1, 1, 8, CM, 0)
+  X(3, 1, 1, 8, CM, 0)            \
+  X(3, 2, 1, 16, CM, 0)           \
+  X(3, 4, 1, 16, CM, 0)
 
+// single-pass 2^15 (LDS 139 KiB at 4 B per element; not for split)
 #define MCFFT_ROW_LIST(X)   \
   MCFFT_ROW_LIST_CM(X, 0)   \
   MCFFT_ROW_LIST_CM(X, 1)   \
-  MCFFT_ROW_LIST_CM(X, 2)
+  MCFFT_ROW_LIST_CM(X, 2)   \
+  X(3, 8, 1, 16, 0, 0)      \
+  X(3, 8, 1, 16, 1, 0)
 
 // ---- ROWT: Q rows stored transposed (segments of Q elements) ----
 #define MCFFT_ROWT_LIST_CM(X, CM, QB)  \

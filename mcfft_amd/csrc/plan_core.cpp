@@ -24,7 +24,7 @@ int ilog2(long long v) {
 // L = 16^S * T with S >= 1, T in {1,2,4,8}
 bool split_len(long long L, int& S, int& T) {
   int lg = ilog2(L);
-  if (lg < 4 || lg > 12) return false;
+  if (lg < 4 || lg > 15) return false;
   S = lg / 4;
   T = 1 << (lg % 4);
   return true;
@@ -162,7 +162,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   int st = ST_OK;
   if (rank == 1) {
     const long long N = p.dims[0];
-    if (N <= 4096) {
+    int S0, T0;
+    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW)) {  // single pass up to 2^15
       p.chunk = batch;
       p.n_chunks = 1;
       st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);
@@ -173,7 +174,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       int lg2 = (lg + 1) / 2;
       if (lg2 > 11) lg2 = 11;
       const int lg1 = lg - lg2;
-      if (lg1 > 12) return ST_UNSUPPORTED;
+      if (lg1 > 12 || lg2 > 12) return ST_UNSUPPORTED;
       const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
       int S1, T1, S2, T2;
       if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;

@@ -25,13 +25,18 @@ def check(dims, batch, fi, fo, inverse=False, scaling=1, ws_limit=0, seed=0):
 
 
 @pytest.mark.parametrize("n,batch", [(16, 128), (32, 64), (64, 32), (128, 16), (256, 8), (512, 4), (1024, 2),
-                                     (2048, 1), (4096, 1), (256, 5), (16, 3)])
+                                     (2048, 1), (4096, 1), (256, 5), (16, 3), (8192, 1), (16384, 1)])
 @pytest.mark.parametrize("fi,fo", FMTS)
 def test_single_pass(n, batch, fi, fo):
     check([n], batch, fi, fo)
 
 
-@pytest.mark.parametrize("n,batch,ws", [(8192, 1, 0), (8192, 3, 8192 * 8), (65536, 1, 0)])
+def test_single_pass_32768():
+    check([32768], 1, emu.FMT_F16, emu.FMT_F16)
+    check([32768], 1, emu.FMT_BF16, emu.FMT_BF16)
+
+
+@pytest.mark.parametrize("n,batch,ws", [(65536, 1, 0), (65536, 3, 65536 * 8), (1 << 17, 1, 0)])
 @pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
 def test_four_step(n, batch, ws, fi, fo):
     check([n], batch, fi, fo, ws_limit=ws)
@@ -68,12 +73,14 @@ def test_split_input_roundtrip():
 def test_plan_describe_and_errors():
     d = emu.describe([131072], 256, emu.FMT_F16, emu.FMT_F16)
     assert "fourstep" in d and d.count("pass ") == 2
+    assert emu.describe([32768], 4, emu.FMT_F16, emu.FMT_F16).count("pass ") == 1   # single pass
+    assert emu.describe([32768], 4, emu.FMT_F16, emu.FMT_SPLIT).count("pass ") == 2  # split: four-step
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([1000], 1, 0, 0)        # not a power of two
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([8], 1, 0, 0)           # below 16
     with pytest.raises(RuntimeError, match="status 2"):
-        emu.describe([16, 8192], 1, 0, 0)    # side above 4096
+        emu.describe([8192, 16], 1, 0, 0)    # strided side above 4096
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([256], 1, 2, 0)         # split in, fp16 out
     with pytest.raises(RuntimeError, match="status 1"):
