@@ -73,6 +73,10 @@ MCFFT_DEV void atomic_or_dev(unsigned* p, unsigned v) {
 MCFFT_DEV void fence_dev() { __threadfence(); }
 MCFFT_DEV void nap() { __builtin_amdgcn_s_sleep(4); }
 
+// pack the low halves / high halves of two packed pairs into one pair (v_perm_b32)
+MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x05040100u); }
+MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x07060302u); }
+
 // D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
 // Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and
 // D[8*(v/4) + 4*(l/32) + v%4][l%32] in accumulator v = 0..15.
@@ -95,7 +99,4 @@ MCFFT_DEV float f16_val(uint32_t bits) { return (float)__builtin_bit_cast(_Float
 MCFFT_DEV uint32_t bf16_bits(float x) { return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x); }
 MCFFT_DEV float bf16_val(uint32_t bits) { return (float)__builtin_bit_cast(__bf16, (uint16_t)bits); }
 MCFFT_DEV uint32_t pack2(uint32_t lo, uint32_t hi) { return (lo & 0xffffu) | (hi << 16); }
-// pack the low halves / high halves of two packed pairs into one pair
-MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
-MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }
 }  // namespace mcfft

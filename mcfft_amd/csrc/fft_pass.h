@@ -61,13 +61,10 @@ struct PassArgs {
   float out_scale;            // last-stage scale when there is no four-step table
   const f32x2* stage_tw;      // post-twiddle tables [stage][j][16] (see plan_core)
   const u32x4* aops;          // DFT-matrix MFMA operands [64 lanes][4]: A1, A2, A1lo, A2lo
-  const f32x2* fs_hi;         // four-step twiddle W_N^e = fs_hi[e>>fs_shift] * fs_lo[e&fs_mask]
-  const f32x2* fs_lo;
-  int fs_shift;
-  unsigned fs_mask;
-  unsigned fs_nmask;          // N-1 (N a power of two, N <= 2^31)
+  const f32x2* fs_tw;         // four-step twiddles W_N^(m*column), N1 x fs_cols entries
+  int fs_cols;                // N2 (columns of the four-step decomposition)
   long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
-  int stage_tw_entries, fs_hi_entries, fs_lo_entries;
+  int stage_tw_entries, fs_tw_entries;
 };
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:
@@ -317,11 +314,8 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
   constexpr int esz_out = (int)sizeof(El);
   const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);
-  GBuf gfh, gfl;
-  if constexpr (FOURSTEP) {
-    gfh = make_gbuf(a.fs_hi, (unsigned)a.fs_hi_entries * 8u);
-    gfl = make_gbuf(a.fs_lo, (unsigned)a.fs_lo_entries * 8u);
-  }
+  GBuf gfs;
+  if constexpr (FOURSTEP) gfs = make_gbuf(a.fs_tw, (unsigned)a.fs_tw_entries * 8u);
   auto view = [](const void* base, long long first, long long total, int esz) -> GBuf {
     const long long rem = (total - first) * esz;
     return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
@@ -331,24 +325,29 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
     const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
     const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
     const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
-    const GBuf gin = view(a.in, ib, a.in_elems, esz_in);
-    const GBuf gout = view(a.out, ob, a.out_elems, esz_out);
     const int inst0 = til * Q;
-    auto valid = [&](int q) -> bool { return inst0 + q < a.n_inst; };
+    // Column kinds always run full tiles (the planner guarantees it).  A ROW
+    // tile's instances are contiguous, so its buffer views end at the last valid
+    // instance: loads beyond read 0 and stores beyond are dropped.
+    long long in_end = a.in_elems, out_end = a.out_elems;
+    if constexpr (KIND == KIND_ROW) {
+      const long long vq = (long long)(a.n_inst - inst0 < Q ? a.n_inst - inst0 : Q);
+      in_end = ib + vq * L < in_end ? ib + vq * L : in_end;
+      out_end = ob + vq * L < out_end ? ob + vq * L : out_end;
+    }
+    const GBuf gin = view(a.in, ib, in_end, esz_in);
+    const GBuf gout = view(a.out, ob, out_end, esz_out);
 
     // output post-processing of the pass's last stage: four-step twiddle or scale
     auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
       if constexpr (FOURSTEP) {
-        const unsigned e = ((unsigned)m * (unsigned)(inst0 + q)) & a.fs_nmask;
-        const f32x2 th = __builtin_bit_cast(f32x2, buf_ld64(gfh, (int)((e >> a.fs_shift) * 8u)));
-        const f32x2 tl = __builtin_bit_cast(f32x2, buf_ld64(gfl, (int)((e & a.fs_mask) * 8u)));
-        return cmul(v, cmul(th, tl));
+        return cmul(v, __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8)));
       } else {
         return v * a.out_scale;
       }
     };
     auto store_out = [&](int q, int m, El e) {
-      if (valid(q)) Tr::store(gout, C::out_off(q, m, sout), e);
+      Tr::store(gout, C::out_off(q, m, sout), e);
     };
 
     // ---------------- radix-16 MFMA stages ----------------
@@ -365,10 +364,9 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           int q, j;
           C::template colmap<16>(ti * 32 + (lane & 31), q, j);
           if constexpr (P == 0) {
-            const bool ok = valid(q);
             const int o0 = C::in_off(q, j + 8 * h * (L / 16), sin_);
             const int ostep = C::in_off(0, L / 16, sin_);
-            for (int t = 0; t < 8; ++t) x[k][t] = ok ? Tr::load(gin, o0 + t * ostep, a.in_fmt) : El{};
+            for (int t = 0; t < 8; ++t) x[k][t] = Tr::load(gin, o0 + t * ostep, a.in_fmt);
           } else {
             for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, C::lidx(q, j + (8 * h + t) * (L / 16)));
           }
@@ -412,13 +410,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
             }
             if constexpr (LAST && !STAGED) {
               // direct store: offsets and four-step exponents advance linearly in u
-              if (valid(q)) {
+              {
                 const int o0 = C::out_off(q, mo0, sout), ostep = C::out_off(0, NS, sout);
-                unsigned e0 = 0, estep = 0;
+                int w0 = 0, wstep = 0;
                 if constexpr (FOURSTEP) {
-                  const unsigned col = (unsigned)(inst0 + q);
-                  e0 = (unsigned)mo0 * col;
-                  estep = (unsigned)NS * col;
+                  w0 = (mo0 * a.fs_cols + inst0 + q) * 8;
+                  wstep = NS * a.fs_cols * 8;
                 }
                 for (int u = 0; u < 4; ++u) {
                   const int s = 4 * g + u;
@@ -426,10 +423,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
                   v.x = y[s];
                   v.y = y[8 + s];
                   if constexpr (FOURSTEP) {
-                    const unsigned e = (e0 + (unsigned)u * estep) & a.fs_nmask;
-                    const f32x2 th = __builtin_bit_cast(f32x2, buf_ld64(gfh, (int)((e >> a.fs_shift) * 8u)));
-                    const f32x2 tl = __builtin_bit_cast(f32x2, buf_ld64(gfl, (int)((e & a.fs_mask) * 8u)));
-                    v = cmul(v, cmul(th, tl));
+                    v = cmul(v, __builtin_bit_cast(f32x2, buf_ld64(gfs, w0 + u * wstep)));
                   } else {
                     v = v * a.out_scale;
                   }

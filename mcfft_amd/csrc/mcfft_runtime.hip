@@ -276,13 +276,9 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
       a.in_elems = h.in_elems;
       a.out_elems = h.out_elems;
       if (h.pp.fourstep) {
-        a.fs_hi = p->tw[d] + h.pp.fs_hi_off;
-        a.fs_lo = p->tw[d] + h.pp.fs_lo_off;
-        a.fs_shift = h.pp.fs_shift;
-        a.fs_mask = h.pp.fs_mask;
-        a.fs_nmask = (unsigned)h.pp.fs_nmask;
-        a.fs_hi_entries = (int)(h.pp.fs_lo_off - h.pp.fs_hi_off);
-        a.fs_lo_entries = 1 << h.pp.fs_shift;
+        a.fs_tw = p->tw[d] + h.pp.fs_tw_off;
+        a.fs_tw_entries = (int)((long long)r.tw.size() - h.pp.fs_tw_off);
+        a.fs_cols = h.pp.fs_cols;
       }
       q.in_buf = h.pp.src;
       q.out_buf = h.pp.dst;
@@ -427,13 +423,9 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
       a.stage_tw = plan->tw[d] + pp.stage_tw_off;
       a.aops = plan->aops[d];
       if (pp.fourstep) {
-        a.fs_hi = plan->tw[d] + pp.fs_hi_off;
-        a.fs_lo = plan->tw[d] + pp.fs_lo_off;
-        a.fs_shift = pp.fs_shift;
-        a.fs_hi_entries = (int)(pp.fs_lo_off - pp.fs_hi_off);
-        a.fs_lo_entries = 1 << pp.fs_shift;
-        a.fs_mask = pp.fs_mask;
-        a.fs_nmask = (unsigned)pp.fs_nmask;
+        a.fs_tw = plan->tw[d] + pp.fs_tw_off;
+        a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);
+        a.fs_cols = pp.fs_cols;
       }
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * (long long)pp.tiles_per_group;

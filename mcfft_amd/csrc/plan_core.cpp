@@ -204,15 +204,12 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       a.groups_per_transform = 1;
       a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
       a.fourstep = true;
-      a.fs_shift = (lg + 1) / 2;
-      a.fs_mask = (unsigned)((1LL << a.fs_shift) - 1);
-      a.fs_nmask = (unsigned long long)(N - 1);
-      a.fs_hi_off = (long long)p.tw.size();
-      for (long long i = 0; i < (N >> a.fs_shift); ++i) p.tw.push_back({i << a.fs_shift, N, 1.f});
-      a.fs_lo_off = (long long)p.tw.size();
+      // direct table W_N^(k1*n2), k1 < N1, n2 < N2 (scale of the last stage folded in)
       const float sl = pow2(-a.shifts.back());
-      for (long long i = 0; i < (1LL << a.fs_shift); ++i) p.tw.push_back({i, N, sl});
-      a.out_scale = 1.f;
+      a.fs_cols = (int)N2;
+      a.fs_tw_off = (long long)p.tw.size();
+      for (long long k1 = 0; k1 < N1; ++k1)
+        for (long long n2 = 0; n2 < N2; ++n2) p.tw.push_back({(k1 * n2) % N, N, sl});
       for (int s : a.shifts) total_shift += s;
       p.passes.push_back(a);
       // pass B (rows, transposed store): ws -> out

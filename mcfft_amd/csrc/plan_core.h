@@ -38,10 +38,8 @@ struct PassPlan {
   float out_scale = 1.f;
   long long stage_tw_off = 0;   // first entry of this pass's stage tables
   bool fourstep = false;
-  long long fs_hi_off = 0, fs_lo_off = 0;
-  int fs_shift = 0;
-  unsigned fs_mask = 0;
-  unsigned long long fs_nmask = 0;
+  long long fs_tw_off = 0;      // four-step table W_N^(k1*n2): N1 x fs_cols entries
+  int fs_cols = 0;
   std::vector<int> shifts;      // per stage, for reporting
 };
 

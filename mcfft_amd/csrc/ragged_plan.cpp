@@ -108,14 +108,12 @@ int build_shape(RaggedPlanCore& r, int rank, const long long* dims, long long ro
     a.pp.tiles_per_group = (int)(N2 / kc.Q);
     a.pp.shifts = add_stage_tables(r.tw, a.pp, S1, T1, r.scaling);
     a.pp.fourstep = true;
-    a.pp.fs_shift = (lg + 1) / 2;
-    a.pp.fs_mask = (unsigned)((1LL << a.pp.fs_shift) - 1);
-    a.pp.fs_nmask = (unsigned long long)(N - 1);
-    a.pp.fs_hi_off = (long long)r.tw.size();
-    for (long long i = 0; i < (N >> a.pp.fs_shift); ++i) r.tw.push_back({i << a.pp.fs_shift, N, 1.f});
-    a.pp.fs_lo_off = (long long)r.tw.size();
+    // direct table W_N^(k1*n2), k1 < N1, n2 < N2 (scale of the last stage folded in)
     const float sl = pow2(-a.pp.shifts.back());
-    for (long long i = 0; i < (1LL << a.pp.fs_shift); ++i) r.tw.push_back({i, N, sl});
+    a.pp.fs_cols = (int)N2;
+    a.pp.fs_tw_off = (long long)r.tw.size();
+    for (long long k1 = 0; k1 < N1; ++k1)
+      for (long long n2 = 0; n2 < N2; ++n2) r.tw.push_back({(k1 * n2) % N, N, sl});
     for (int s : a.pp.shifts) total += s;
     out.passes.push_back(a);
     RaggedPassHost b;

@@ -116,13 +116,9 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       a.stage_tw = tw.data() + pp.stage_tw_off;
       a.aops = aops.data();
       if (pp.fourstep) {
-        a.fs_hi = tw.data() + pp.fs_hi_off;
-        a.fs_lo = tw.data() + pp.fs_lo_off;
-        a.fs_shift = pp.fs_shift;
-        a.fs_hi_entries = (int)(pp.fs_lo_off - pp.fs_hi_off);
-        a.fs_lo_entries = 1 << pp.fs_shift;
-        a.fs_mask = pp.fs_mask;
-        a.fs_nmask = (unsigned)pp.fs_nmask;
+        a.fs_tw = tw.data() + pp.fs_tw_off;
+        a.fs_tw_entries = (int)((long long)p.tw.size() - pp.fs_tw_off);
+        a.fs_cols = pp.fs_cols;
       }
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * pp.tiles_per_group;
@@ -192,13 +188,9 @@ extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, i
     a.in_elems = h.in_elems;
     a.out_elems = h.out_elems;
     if (h.pp.fourstep) {
-      a.fs_hi = tw.data() + h.pp.fs_hi_off;
-      a.fs_lo = tw.data() + h.pp.fs_lo_off;
-      a.fs_shift = h.pp.fs_shift;
-      a.fs_mask = h.pp.fs_mask;
-      a.fs_nmask = (unsigned)h.pp.fs_nmask;
-      a.fs_hi_entries = (int)(h.pp.fs_lo_off - h.pp.fs_hi_off);
-      a.fs_lo_entries = 1 << h.pp.fs_shift;
+      a.fs_tw = tw.data() + h.pp.fs_tw_off;
+      a.fs_tw_entries = (int)((long long)r.tw.size() - h.pp.fs_tw_off);
+      a.fs_cols = h.pp.fs_cols;
     }
     d.in_buf = h.pp.src;
     d.out_buf = h.pp.dst;

@@ -129,21 +129,26 @@ template <class T> inline void lds_st(T* p, int i, T v) {
 template <class T> inline T g_ld(const T* p, long long i) { emu::check_global(p + i, sizeof(T), i); return p[i]; }
 template <class T> inline void g_st(T* p, long long i, T v) { emu::check_global(p + i, sizeof(T), i); p[i] = v; }
 
-// buffer-resource emulation: every access must stay inside [0, bytes)
+// buffer-resource emulation with the hardware's range check: an access outside
+// [0, bytes) of the view reads 0 / is dropped; an access inside it must hit
+// registered memory (otherwise the view itself is wrong: abort)
 struct GBuf {
   const char* p;
   unsigned bytes;
 };
 inline GBuf make_gbuf(const void* p, unsigned bytes) { return GBuf{(const char*)p, bytes}; }
 template <class T> inline T buf_ld(GBuf b, int off) {
-  if (off < 0 || (unsigned)off + sizeof(T) > b.bytes) emu::fail("buffer load", off);
-  emu::check_global(b.p + off, sizeof(T), off);
   T v;
+  if ((unsigned)off + sizeof(T) > b.bytes || (unsigned)off > b.bytes) {
+    memset(&v, 0, sizeof(T));
+    return v;
+  }
+  emu::check_global(b.p + off, sizeof(T), off);
   memcpy(&v, b.p + off, sizeof(T));
   return v;
 }
 template <class T> inline void buf_st(GBuf b, int off, T v) {
-  if (off < 0 || (unsigned)off + sizeof(T) > b.bytes) emu::fail("buffer store", off);
+  if ((unsigned)off + sizeof(T) > b.bytes || (unsigned)off > b.bytes) return;
   emu::check_global(b.p + off, sizeof(T), off);
   memcpy((char*)b.p + off, &v, sizeof(T));
 }
@@ -158,6 +163,9 @@ inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, _
 inline void atomic_or_dev(unsigned* p, unsigned v) { (void)__atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline void fence_dev() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline void nap() { std::this_thread::yield(); }
+
+inline uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
+inline uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }
 
 template <class Dec>
 inline f32x16 emu_mfma(u32x4 a, u32x4 b, f32x16 c, Dec dec) {
