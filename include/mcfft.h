@@ -112,6 +112,16 @@ mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents);
  * result is then wrong).  Synchronises with the device. */
 mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags);
 
+/* Autotuner: times every compiled kernel variant (tile size, waves per
+ * workgroup) of each pass of `plan` on the given device buffers, `reps` launches
+ * each, keeps the fastest and records the choice in the tuning cache, keyed by
+ * GPU architecture and (rank, dims, batch, formats, scaling).  Blocking; the
+ * contents of `out` (and of `in` for an in-place plan) are overwritten.  Plans
+ * created later read the cache, so a shape is tuned once per machine.  Cache
+ * file: $MCFFT_TUNE_CACHE, default $HOME/.cache/mcfft/tune-<arch>.txt;
+ * MCFFT_TUNE_CACHE=off disables it.  Not for ragged plans. */
+mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps);
+
 /* Profiling entry points (no reference counterpart): the number of kernel
  * passes of a plan, and an exec that launches only passes [first, first+count)
  * (for every batch chunk), so a caller can time each kernel on its own stream.

@@ -66,9 +66,10 @@ def lib() -> ctypes.CDLL:
         L.mcfft_plan_ragged.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(RaggedItem), i, i, i]
         L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_status_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.mcfft_plan_tune.argtypes = [vp, vp, vp, i, vp, i]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
-                  "mcfft_get_status_flags",
+                  "mcfft_get_status_flags", "mcfft_plan_tune",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -123,6 +124,15 @@ class Plan:
         elif hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream)
+
+    def tune(self, inp, out, direction: int = FORWARD, reps: int = 5, stream=None):
+        """Time every compiled kernel variant of each pass on these buffers, keep
+        the fastest and record it in the tuning cache (blocking; overwrites out)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(inp.device).cuda_stream
+        _check(lib().mcfft_plan_tune(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, reps),
+               "mcfft_plan_tune")
 
     def pass_count(self) -> int:
         c = ctypes.c_int()

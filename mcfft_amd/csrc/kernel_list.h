@@ -23,13 +23,25 @@
   X(3, 2, 1, 16, CM, 0)           \
   X(3, 4, 1, 16, CM, 0)
 
+// alternative tile shapes the autotuner may pick (the first entry of a
+// (S, T, CM, KIND) combination above is the planner's default)
+#define MCFFT_ROW_TUNE_CM(X, CM)  \
+  X(2, 1, 4, 2, CM, 0)            \
+  X(2, 1, 16, 8, CM, 0)           \
+  X(2, 1, 16, 4, CM, 0)           \
+  X(2, 2, 8, 8, CM, 0)            \
+  X(3, 1, 1, 4, CM, 0)            \
+  X(3, 1, 2, 8, CM, 0)
+
 // single-pass 2^15 (LDS 139 KiB at 4 B per element; not for split)
 #define MCFFT_ROW_LIST(X)   \
   MCFFT_ROW_LIST_CM(X, 0)   \
   MCFFT_ROW_LIST_CM(X, 1)   \
   MCFFT_ROW_LIST_CM(X, 2)   \
   X(3, 8, 1, 16, 0, 0)      \
-  X(3, 8, 1, 16, 1, 0)
+  X(3, 8, 1, 16, 1, 0)      \
+  MCFFT_ROW_TUNE_CM(X, 0)   \
+  MCFFT_ROW_TUNE_CM(X, 1)
 
 // ---- ROWT: Q rows stored transposed (segments of Q elements) ----
 #define MCFFT_ROWT_LIST_CM(X, CM, QB)  \
@@ -42,10 +54,17 @@
   X(2, 4, 16, 8, CM, 1)                \
   X(2, 8, QB, 16, CM, 1)
 
+#define MCFFT_ROWT_TUNE_CM(X, CM)  \
+  X(2, 1, 16, 4, CM, 1)            \
+  X(2, 2, 16, 8, CM, 1)            \
+  X(2, 2, 16, 4, CM, 1)
+
 #define MCFFT_ROWT_LIST(X)      \
   MCFFT_ROWT_LIST_CM(X, 0, 16)  \
   MCFFT_ROWT_LIST_CM(X, 1, 16)  \
-  MCFFT_ROWT_LIST_CM(X, 2, 8)
+  MCFFT_ROWT_LIST_CM(X, 2, 8)   \
+  MCFFT_ROWT_TUNE_CM(X, 0)      \
+  MCFFT_ROWT_TUNE_CM(X, 2)
 
 // ---- COL / COLF: Q consecutive columns ----
 #define MCFFT_COL_LIST_CM(X, CM, K, Q512, Q1024, Q2048, Q4096) \
@@ -59,15 +78,24 @@
   X(2, 8, Q2048, 16, CM, K)                                    \
   X(3, 1, Q4096, 16, CM, K)
 
+#define MCFFT_COL_TUNE_CM(X, CM, K)  \
+  X(2, 1, 16, 4, CM, K)              \
+  X(2, 1, 64, 8, CM, K)              \
+  X(3, 1, 4, 8, CM, K)
+
 #define MCFFT_COL_LIST(X)                          \
   MCFFT_COL_LIST_CM(X, 0, 2, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 1, 2, 32, 16, 16, 8)        \
-  MCFFT_COL_LIST_CM(X, 2, 2, 16, 16, 8, 4)
+  MCFFT_COL_LIST_CM(X, 2, 2, 16, 16, 8, 4)         \
+  MCFFT_COL_TUNE_CM(X, 0, 2)                       \
+  MCFFT_COL_TUNE_CM(X, 1, 2)
 
 #define MCFFT_COLF_LIST(X)                         \
   MCFFT_COL_LIST_CM(X, 0, 3, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 1, 3, 32, 16, 16, 8)        \
-  MCFFT_COL_LIST_CM(X, 2, 3, 16, 16, 8, 4)
+  MCFFT_COL_LIST_CM(X, 2, 3, 16, 16, 8, 4)         \
+  MCFFT_COL_TUNE_CM(X, 0, 3)                       \
+  X(2, 1, 16, 4, 2, 3)
 
 namespace mcfft {
 struct KernelInfo {

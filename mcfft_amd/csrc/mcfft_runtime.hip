@@ -1,8 +1,12 @@
 // mcfft_runtime.hip -- the C ABI (include/mcfft.h): plan objects, FP64 table
 // generation on the GPU, and pass launches on the caller's stream.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <cmath>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,6 +26,7 @@ using namespace mcfft;
 struct mcfftPlan_st {
   PlanCore core;
   int device = 0;
+  std::string arch;
   f32x2* tw[2] = {nullptr, nullptr};      // [0] forward, [1] inverse
   uint16_t* dft[2] = {nullptr, nullptr};
   u32x4* aops[2] = {nullptr, nullptr};    // per-lane DFT-matrix MFMA operands
@@ -95,6 +100,86 @@ mcfftStatus set_lds_attr(const KernelInfo* k) {
   return MCFFT_SUCCESS;
 }
 
+// ---------------- tuning cache ----------------
+// file format: first line "mcfft-tune-cache 1", then "arch|signature=k0,k1,..."
+std::mutex g_cache_mu;
+std::map<std::string, std::string> g_cache;
+bool g_cache_loaded = false;
+
+std::string cache_path(const std::string& arch) {
+  const char* env = getenv("MCFFT_TUNE_CACHE");
+  if (env && std::string(env) == "off") return "";
+  if (env && *env) return env;
+  const char* home = getenv("HOME");
+  if (!home || !*home) return "";
+  return std::string(home) + "/.cache/mcfft/tune-" + arch + ".txt";
+}
+
+void load_cache_locked(const std::string& arch) {
+  if (g_cache_loaded) return;
+  g_cache_loaded = true;
+  const std::string path = cache_path(arch);
+  if (path.empty()) return;
+  std::ifstream f(path);
+  std::string line;
+  if (!f || !std::getline(f, line) || line != "mcfft-tune-cache 1") return;  // absent or other version
+  while (std::getline(f, line)) {
+    const size_t eq = line.find('=');
+    if (eq != std::string::npos) g_cache[line.substr(0, eq)] = line.substr(eq + 1);
+  }
+}
+
+bool save_cache_locked(const std::string& arch) {
+  const std::string path = cache_path(arch);
+  if (path.empty()) return true;
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    std::string dir = path.substr(0, slash), cur;
+    std::stringstream ss(dir);
+    std::string part;
+    while (std::getline(ss, part, '/')) {
+      cur += part + "/";
+      (void)mkdir(cur.c_str(), 0755);
+    }
+  }
+  std::ofstream f(path + ".tmp");
+  if (!f) return false;
+  f << "mcfft-tune-cache 1\n";
+  for (const auto& kv : g_cache) f << kv.first << "=" << kv.second << "\n";
+  f.close();
+  return rename((path + ".tmp").c_str(), path.c_str()) == 0;
+}
+
+std::string arch_of(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+  std::string a = prop.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+// apply a cached kernel choice to a freshly built plan (silently keeps the
+// defaults when an entry no longer matches the compiled kernels)
+void apply_cached(mcfftPlan p) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  load_cache_locked(p->arch);
+  const auto it = g_cache.find(p->arch + "|" + plan_signature(p->core));
+  if (it == g_cache.end()) return;
+  std::vector<std::string> names;
+  std::stringstream ss(it->second);
+  std::string n;
+  while (std::getline(ss, n, ',')) names.push_back(n);
+  if (names.size() != p->core.passes.size()) return;
+  for (size_t i = 0; i < names.size(); ++i) {
+    PassPlan& pp = p->core.passes[i];
+    for (const KernelInfo* k : pass_candidates(pp, p->core.cm))
+      if (names[i] == k->name) {
+        PassPlan trial = pp;
+        if (set_pass_kernel(trial, k)) pp = trial;
+      }
+  }
+}
+
 void free_plan(mcfftPlan p) {
   if (!p) return;
   for (int d = 0; d < 2; ++d) {
@@ -164,6 +249,8 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     delete p;
     return MCFFT_HIP_ERROR;
   }
+  p->arch = arch_of(p->device);
+  apply_cached(p);
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);
     int per_cu = 0;
@@ -344,6 +431,63 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
     return exec_ragged(plan, in, out, direction, (hipStream_t)stream);
   }
   return mcfft_exec_passes(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
+}
+
+mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
+  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
+  if (!in || !out || reps < 1) return MCFFT_INVALID_VALUE;
+  hipStream_t s = (hipStream_t)stream;
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, plan->device) != hipSuccess)
+    return MCFFT_HIP_ERROR;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MCFFT_HIP_ERROR;
+  mcfftStatus rs = MCFFT_SUCCESS;
+  std::string choice;
+  for (size_t pi = 0; pi < plan->core.passes.size() && rs == MCFFT_SUCCESS; ++pi) {
+    PassPlan best = plan->core.passes[pi];
+    int best_cap = plan->grid_cap[pi];
+    float best_ms = 1e30f;
+    for (const KernelInfo* k : pass_candidates(plan->core.passes[pi], plan->core.cm)) {
+      PassPlan trial = plan->core.passes[pi];
+      if (!set_pass_kernel(trial, k)) continue;
+      int per_cu = 0;
+      if (set_lds_attr(k) != MCFFT_SUCCESS ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, k->nw * 64, k->lds_bytes) != hipSuccess ||
+          per_cu < 1)
+        continue;
+      const PassPlan saved = plan->core.passes[pi];
+      const int saved_cap = plan->grid_cap[pi];
+      plan->core.passes[pi] = trial;
+      plan->grid_cap[pi] = per_cu * n_cu;
+      rs = mcfft_exec_passes(plan, in, out, direction, stream, (int)pi, 1);  // warm-up
+      float ms = 0.f;
+      if (rs == MCFFT_SUCCESS && hipEventRecord(e0, s) == hipSuccess) {
+        for (int r = 0; r < reps && rs == MCFFT_SUCCESS; ++r)
+          rs = mcfft_exec_passes(plan, in, out, direction, stream, (int)pi, 1);
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+          rs = MCFFT_HIP_ERROR;
+      }
+      if (rs == MCFFT_SUCCESS && ms < best_ms) {
+        best_ms = ms;
+        best = trial;
+        best_cap = per_cu * n_cu;
+      }
+      plan->core.passes[pi] = saved;
+      plan->grid_cap[pi] = saved_cap;
+    }
+    plan->core.passes[pi] = best;
+    plan->grid_cap[pi] = best_cap;
+    choice += (pi ? "," : "") + std::string(best.k->name);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rs != MCFFT_SUCCESS) return rs;
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  load_cache_locked(plan->arch);
+  g_cache[plan->arch + "|" + plan_signature(plan->core)] = choice;
+  return save_cache_locked(plan->arch) ? MCFFT_SUCCESS : MCFFT_CACHE_ERROR;
 }
 
 mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {

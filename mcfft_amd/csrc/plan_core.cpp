@@ -117,6 +117,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     if (!k) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
+    pp.kind = KIND_ROW;
+    pp.L = L;
     pp.src = src;
     pp.dst = dst;
     pp.src_fmt = src_fmt;
@@ -142,6 +144,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     if (!k || ncols % k->Q != 0 || stride * L > 0x7fffffffLL) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
+    pp.kind = KIND_COL;
+    pp.L = L;
     pp.src = src;
     pp.dst = dst;
     pp.src_fmt = src_fmt;
@@ -192,6 +196,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       // pass A (columns + four-step twiddle): in -> ws
       PassPlan a;
       a.k = kc;
+      a.kind = KIND_COLF;
+      a.L = N1;
       a.src = BUF_IN;
       a.dst = BUF_WS;
       a.src_fmt = in_fmt;
@@ -215,6 +221,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       // pass B (rows, transposed store): ws -> out
       PassPlan b;
       b.k = kr;
+      b.kind = KIND_ROWT;
+      b.L = N2;
       b.src = BUF_WS;
       b.dst = BUF_OUT;
       b.src_fmt = ws_fmt;
@@ -254,6 +262,49 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   if (st != ST_OK) return st;
   p.out_exponent = total_shift;
   return ST_OK;
+}
+
+std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm) {
+  std::vector<const KernelInfo*> out;
+  const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
+  const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  if (!pp.k) return out;
+  for (int i = 0; i < counts[pp.kind]; ++i) {
+    const KernelInfo& k = lists[pp.kind][i];
+    if (k.S == pp.k->S && k.T == pp.k->T && k.cm == cm) out.push_back(&k);
+  }
+  return out;
+}
+
+bool set_pass_kernel(PassPlan& pp, const KernelInfo* k) {
+  const long long Q = k->Q;
+  switch (pp.kind) {
+    case KIND_ROW:
+      pp.in_st = pp.out_st = Q * pp.L;
+      pp.tiles_per_group = (int)((pp.n_inst + Q - 1) / Q);
+      break;
+    case KIND_ROWT:
+      if (pp.n_inst % Q) return false;
+      pp.in_st = Q * pp.L;
+      pp.out_st = Q;
+      pp.tiles_per_group = (int)(pp.n_inst / Q);
+      break;
+    default:
+      if (pp.n_inst % Q) return false;
+      pp.in_st = pp.out_st = Q;
+      pp.tiles_per_group = (int)(pp.n_inst / Q);
+      break;
+  }
+  pp.k = k;
+  return true;
+}
+
+std::string plan_signature(const PlanCore& p) {
+  std::ostringstream o;
+  o << p.rank << "|" << p.dims[0];
+  for (int d = 1; d < p.rank; ++d) o << "x" << p.dims[d];
+  o << "|" << p.batch << "|" << p.in_fmt << "|" << p.out_fmt << "|" << p.scaling;
+  return o.str();
 }
 
 std::string PlanCore::describe() const {

@@ -27,6 +27,8 @@ struct TwSpec {
 
 struct PassPlan {
   const KernelInfo* k = nullptr;
+  int kind = 0;                 // 0 ROW, 1 ROWT, 2 COL, 3 COLF
+  long long L = 0;              // points per instance
   int src = BUF_IN, dst = BUF_OUT;
   int src_fmt = 0;
   int stride_in = 0, stride_out = 0;  // strided-axis strides (see PassArgs)
@@ -72,6 +74,14 @@ enum {
 };
 
 int elem_bytes(int fmt);
+// every compiled kernel that can run pass `pp` (same kind, length and compute
+// mode; the first is the planner's default); used by the tuner
+std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm);
+// bind kernel `k` to pass `pp`, recomputing the tile-size dependent fields;
+// false if its tile does not divide the pass
+bool set_pass_kernel(PassPlan& pp, const KernelInfo* k);
+// "rank|dims|batch|in|out|scaling" -- tuning cache key of a plan
+std::string plan_signature(const PlanCore& p);
 // helpers shared with the ragged planner
 int ilog2(long long v);                          // log2 of a power of two, -1 otherwise
 bool split_len(long long L, int& S, int& T);     // L = 16^S * T, S >= 1, T in {1,2,4,8}

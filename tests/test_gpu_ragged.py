@@ -80,8 +80,8 @@ def test_graph_capture_replays(torch_cuda):
     p = mc.Plan([n], b, "fp16", "split")
     pr = mc.RaggedPlan([([4096], 0, 0), ([65536], 4096, 4096), ([256, 256], 69632, 69632)], "fp16", "fp16")
     d_in = mc.to_device(x, "fp16")
-    d_out = mc.empty(n * b, "split")
-    d_rout = mc.empty(n * b, "fp16")
+    d_out = mc.empty(n * b, "split").zero_()
+    d_rout = mc.empty(n * b, "fp16").zero_()   # the ragged items cover only a prefix
     p.exec(d_in, d_out, mc.FORWARD)
     pr.exec(d_in, d_rout, mc.FORWARD)
     torch.cuda.synchronize()
