@@ -114,6 +114,11 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
     rp = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], fi, fo)
     ms_r = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
     flags = rp.status_flags()
+    # the same work as ONE launch with in-kernel pass dependencies
+    os.environ["MCFFT_RAGGED_SINGLE"] = "1"
+    ms_single = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
+    flags |= rp.status_flags()
+    del os.environ["MCFFT_RAGGED_SINGLE"]
     # per-shape batched launches over the same buffers
     groups = {}
     for s, o in zip(shapes, offs):
@@ -144,6 +149,7 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
         worst = max(worst, oracle.rel_l2(y, oracle.fft(xq, tuple(s))))
     res = {"config": "c5", "items": len(shapes), "elements": total, "in": fi, "out": fo,
            "ragged_ms": round(ms_r, 4), "ragged_gflops": round(flops / (ms_r * 1e-3) / 1e9, 1),
+           "ragged_single_launch_ms": round(ms_single, 4),
            "ragged_hbm_frac": round(total * _bytes(fi, fo) / (ms_r * 1e-3) / 1e9 / HBM, 4),
            "per_shape_launches": len(plans), "per_shape_ms": round(ms_p, 4),
            "speedup_vs_per_shape": round(ms_p / ms_r, 3), "status_flags": flags,

@@ -35,6 +35,11 @@ MCFFT_DEV int thread_id() { return threadIdx.x; }
 MCFFT_DEV int block_id() { return blockIdx.x; }
 MCFFT_DEV int grid_size() { return gridDim.x; }
 MCFFT_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+MCFFT_DEV const void* uniform_ptr(const void* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (const void*)(((unsigned long long)hi << 32) | lo);
+}
 MCFFT_DEV void block_barrier() { __syncthreads(); }
 
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }

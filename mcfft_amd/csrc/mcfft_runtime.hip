@@ -382,10 +382,10 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
     std::vector<RaggedTask> tasks(nt);
     for (size_t i = 0; i < nt; ++i) tasks[i] = RaggedTask{r.task_pass[i], r.task_tile[i]};
     if (hipMalloc(&p->rtasks, sizeof(RaggedTask) * (nt ? nt : 1)) != hipSuccess ||
-        hipMalloc(&p->rcounters, sizeof(unsigned) * (2 + r.n_counters + 1)) != hipSuccess)
+        hipMalloc(&p->rcounters, sizeof(unsigned) * (kRaggedCounterBase + r.n_counters + 1)) != hipSuccess)
       rs = MCFFT_ALLOC_FAILED;
     else if (hipMemcpy(p->rtasks, tasks.data(), sizeof(RaggedTask) * nt, hipMemcpyHostToDevice) != hipSuccess ||
-             hipMemset(p->rcounters, 0, sizeof(unsigned) * (2 + r.n_counters + 1)) != hipSuccess)
+             hipMemset(p->rcounters, 0, sizeof(unsigned) * (kRaggedCounterBase + r.n_counters + 1)) != hipSuccess)
       rs = MCFFT_HIP_ERROR;
   }
   if (rs != MCFFT_SUCCESS) {
@@ -401,7 +401,7 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
   if (((size_t)in % (size_t)elem_bytes(r.in_fmt)) || ((size_t)out % (size_t)elem_bytes(r.out_fmt)))
     return MCFFT_MISALIGNED;
   const int d = direction == MCFFT_FORWARD ? 0 : 1;
-  const int nc = 2 + r.n_counters;
+  const int nc = kRaggedCounterBase + r.n_counters;
   reset_counters<<<1, 256, 0, s>>>(plan->rcounters, nc);
   if (hipGetLastError() != hipSuccess) return MCFFT_HIP_ERROR;
   RaggedArgs ra;
@@ -410,20 +410,32 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
   ra.ws_base = (char*)plan->ws;
   ra.passes = plan->rpasses[d];
   ra.tasks = plan->rtasks;
-  ra.n_tasks = (int)r.task_pass.size();
   ra.counters = plan->rcounters;
   ra.spin_limit = 1u << 24;
   ra.aops = plan->aops[d];
   const KernelInfo* k = &kRaggedKernels[r.cm];
-  const int grid = plan->rgrid < ra.n_tasks ? plan->rgrid : (ra.n_tasks > 0 ? ra.n_tasks : 1);
-  void* args[] = {&ra};
-  if (hipLaunchKernel(k->fn, dim3((unsigned)grid), dim3((unsigned)(k->nw * 64)), args, (size_t)k->lds_bytes, s) !=
-      hipSuccess)
-    return MCFFT_HIP_ERROR;
+  // MCFFT_RAGGED_SINGLE=1: one launch with in-kernel pass dependencies;
+  // default: one launch per pass phase (kernel boundaries order the phases)
+  const char* env = getenv("MCFFT_RAGGED_SINGLE");
+  const bool single = env && env[0] == '1';
+  const int n_phase = (int)r.phase_begin.size() - 1;
+  for (int ph = 0; ph < (single ? 1 : n_phase); ++ph) {
+    ra.task_begin = single ? 0 : r.phase_begin[ph];
+    ra.task_end = single ? r.phase_begin.back() : r.phase_begin[ph + 1];
+    ra.queue = single ? 3 : ph;
+    ra.use_deps = single ? 1 : 0;
+    const int n = ra.task_end - ra.task_begin;
+    if (n <= 0) continue;
+    const int grid = plan->rgrid < n ? plan->rgrid : n;
+    void* args[] = {&ra};
+    if (hipLaunchKernel(k->fn, dim3((unsigned)grid), dim3((unsigned)(k->nw * 64)), args, (size_t)k->lds_bytes, s) !=
+        hipSuccess)
+      return MCFFT_HIP_ERROR;
+  }
   return MCFFT_SUCCESS;
 }
 
-mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
+mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (plan->ragged) {
     if (!in || !out) return MCFFT_INVALID_VALUE;
@@ -507,7 +519,7 @@ mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags) {
   *flags = 0;
   if (!plan->ragged) return MCFFT_SUCCESS;
   if (hipDeviceSynchronize() != hipSuccess ||
-      hipMemcpy(flags, plan->rcounters + 1, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(flags, plan->rcounters + 4, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
     return MCFFT_HIP_ERROR;
   return MCFFT_SUCCESS;
 }

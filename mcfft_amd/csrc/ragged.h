@@ -34,17 +34,24 @@ struct RaggedTask {
   int tile;
 };
 
+// One launch runs tasks [task_begin, task_end).  Phased execution launches the
+// kernel once per pass phase (kernel boundaries order the phases, no waits);
+// with use_deps a single launch runs every phase and second-phase tasks wait on
+// their transform's counter.
 struct RaggedArgs {
   const char* in_base;
   char* out_base;
   char* ws_base;
   const RaggedPass* passes;
   const RaggedTask* tasks;
-  int n_tasks;
-  unsigned* counters;   // [0] queue head, [1] status, [2 + i] transform i
+  int task_begin, task_end;
+  int queue;            // index of this launch's queue-head counter
+  int use_deps;
+  unsigned* counters;   // [0..3] queue heads, [4] status, [5 + i] transform-group i
   unsigned spin_limit;
   const u32x4* aops;
 };
+constexpr int kRaggedCounterBase = 5;
 
 // ragged kernel variants: X(id, S, T, Q, NW, CM, KIND); all share NW = 8 and
 // keep LDS <= 64 KiB (two workgroups per CU)
@@ -98,19 +105,24 @@ constexpr int ragged_lds_bytes() {
   return ragged_max_lds<CM>() + 16;
 }
 
-// each variant is an out-of-line call, so its address arithmetic is not hoisted
-// into the task loop next to every other variant's (which would spill)
+// Each variant is an out-of-line call, so its address arithmetic is not hoisted
+// into the task loop next to every other variant's (which would spill).  The
+// pass arguments are read from the descriptor in global memory through a
+// wave-uniform pointer (scalar loads), not passed through private memory.
 template <int S, int T, int Q, int NW, int CM, int KIND>
-MCFFT_NOINLINE void ragged_tile(const PassArgs& a, int tile, const AOps& A) {
-  fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+MCFFT_NOINLINE void ragged_tile(const PassArgs* gpa, const void* in, void* out, int tile, AOps A) {
+  PassArgs a = *(const PassArgs*)uniform_ptr(gpa);
+  a.in = uniform_ptr(in);
+  a.out = (void*)uniform_ptr(out);
+  fft_tile<S, T, Q, NW, CM, KIND>(a, uniform(tile), A);
 }
 
 template <int CM>
-MCFFT_DEV void ragged_dispatch(int variant, const PassArgs& a, int tile, const AOps& A) {
+MCFFT_DEV void ragged_dispatch(int variant, const PassArgs* gpa, const void* in, void* out, int tile, const AOps& A) {
   switch (variant) {
-#define MCFFT_RAGGED_CASE(ID, S, T, Q, NW, CMx, K) \
-  case ID:                                         \
-    ragged_tile<S, T, Q, NW, CMx, K>(a, tile, A);  \
+#define MCFFT_RAGGED_CASE(ID, S, T, Q, NW, CMx, K)         \
+  case ID:                                                 \
+    ragged_tile<S, T, Q, NW, CMx, K>(gpa, in, out, tile, A); \
     break;
     MCFFT_RAGGED_LIST_F(MCFFT_RAGGED_CASE, CM, (CM == CM_SPLIT ? 2 : 1))
 #undef MCFFT_RAGGED_CASE
@@ -125,19 +137,20 @@ MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
   const AOps A = load_aops(r.aops, lane_id());
   MCFFT_DYN_LDS(unsigned char, ldsb);
   int* slot = (int*)(ldsb + task_slot_offset);
+  unsigned* ctr = r.counters + kRaggedCounterBase;
   for (;;) {
-    if (tid == 0) lds_st(slot, 0, (int)atomic_add_dev(&r.counters[0], 1u));
+    if (tid == 0) lds_st(slot, 0, r.task_begin + (int)atomic_add_dev(&r.counters[r.queue], 1u));
     block_barrier();
     const int ti = uniform(lds_ld(slot, 0));
-    if (ti >= r.n_tasks) break;  // every workgroup leaves through here
+    if (ti >= r.task_end) break;  // every workgroup leaves through here
     const RaggedTask t = r.tasks[ti];
-    const RaggedPass& rp = r.passes[t.pass];
-    if (rp.wait_for > 0) {
+    const RaggedPass* rp = (const RaggedPass*)uniform_ptr(r.passes + t.pass);
+    if (r.use_deps && rp->wait_for > 0) {
       if (tid == 0) {
         unsigned it = 0;
-        while (atomic_load_dev(&r.counters[2 + rp.counter]) < (unsigned)rp.wait_for) {
+        while (atomic_load_dev(&ctr[rp->counter]) < (unsigned)rp->wait_for) {
           if (++it > r.spin_limit) {
-            atomic_or_dev(&r.counters[1], 1u);
+            atomic_or_dev(&r.counters[4], 1u);
             break;
           }
           nap();
@@ -146,16 +159,15 @@ MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
       block_barrier();
       fence_dev();
     }
-    PassArgs a = rp.args;
-    const char* ib = rp.in_buf == 0 ? r.in_base : (rp.in_buf == 1 ? (const char*)r.out_base : (const char*)r.ws_base);
-    char* ob = rp.out_buf == 1 ? r.out_base : r.ws_base;
-    a.in = ib + (size_t)rp.args.in;
-    a.out = (void*)(ob + (size_t)rp.args.out);
-    ragged_dispatch<CM>(rp.variant, a, t.tile, A);  // ends with a barrier
-    if (rp.signal) {
+    const int ib = rp->in_buf, ob = rp->out_buf;
+    const char* in = (ib == 0 ? r.in_base : ib == 1 ? (const char*)r.out_base : (const char*)r.ws_base) +
+                     (size_t)rp->args.in;
+    char* out = (ob == 1 ? r.out_base : r.ws_base) + (size_t)rp->args.out;
+    ragged_dispatch<CM>(rp->variant, &rp->args, in, out, t.tile, A);  // ends with a barrier
+    if (r.use_deps && rp->signal) {
       fence_dev();
       block_barrier();
-      if (tid == 0) atomic_add_dev(&r.counters[2 + rp.counter], 1u);
+      if (tid == 0) atomic_add_dev(&ctr[rp->counter], 1u);
     }
   }
 }

@@ -248,7 +248,8 @@ int ragged_build(RaggedPlanCore& r, int n_items, const RaggedItemSpec* items, in
       pass_index[ui].push_back((int)r.passes.size());
       r.passes.push_back(h);
     }
-  for (int phase = 0; phase < 3; ++phase)
+  for (int phase = 0; phase < 3; ++phase) {
+    r.phase_begin.push_back((int)r.task_pass.size());
     for (size_t ui : order) {
       if ((size_t)phase >= pass_index[ui].size()) continue;
       const int pi = pass_index[ui][phase];
@@ -259,6 +260,10 @@ int ragged_build(RaggedPlanCore& r, int n_items, const RaggedItemSpec* items, in
         r.task_tile.push_back(t);
       }
     }
+  }
+  r.phase_begin.push_back((int)r.task_pass.size());
+  while (r.phase_begin.size() > 2 && r.phase_begin[r.phase_begin.size() - 2] == r.phase_begin.back())
+    r.phase_begin.pop_back();  // drop empty trailing phases
   return ST_OK;
 }
 

@@ -31,6 +31,7 @@ struct RaggedPlanCore {
   int n_counters = 0;               // transform groups with more than one pass
   std::vector<RaggedPassHost> passes;
   std::vector<int> task_pass, task_tile;
+  std::vector<int> phase_begin;     // task index where each pass phase starts (+ end)
   std::vector<TwSpec> tw;
   std::vector<int> exponent;        // per item
   long long ws_elems = 0;

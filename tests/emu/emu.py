@@ -90,7 +90,7 @@ def describe(dims, batch, in_fmt, out_fmt, scaling=1):
     return buf.value.decode()
 
 
-def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=False, nblocks=3):
+def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=False, nblocks=3, single=False):
     """Ragged batch: xs = list of complex arrays, shapes = list of dims lists.
     layout: list of (in_off, out_off) element offsets (default back to back).
     Returns (list of outputs * 2^exponent, status)."""
@@ -117,10 +117,11 @@ def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=Fal
     st = ctypes.c_uint(0)
     L = lib()
     L.emu_ragged_exec.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint)]
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint),
+                                  ctypes.c_int]
     L.emu_ragged_exec.restype = ctypes.c_int
     rc = L.emu_ragged_exec(n_items, spec.ctypes.data, in_fmt, out_fmt, scaling, int(inverse), src.ctypes.data,
-                           out.ctypes.data, nblocks, exps.ctypes.data, ctypes.byref(st))
+                           out.ctypes.data, nblocks, exps.ctypes.data, ctypes.byref(st), int(single))
     if rc != 0:
         raise RuntimeError(f"emu_ragged_exec status {rc}")
     y = from_storage(out, out_fmt)

@@ -139,7 +139,7 @@ extern "C" void mcfft_ragged_2(RaggedArgs r);
 // Ragged batch through the persistent ragged kernel, `nblocks` workgroups run
 // concurrently.  items: n x {rank, d0, d1, d2, in_off, out_off} (long long).
 extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, int out_fmt, int scaling, int inverse,
-                               const void* in, void* out, int nblocks, int* exponents, unsigned* status) {
+                               const void* in, void* out, int nblocks, int* exponents, unsigned* status, int single) {
   std::vector<RaggedItemSpec> items(n_items);
   for (int i = 0; i < n_items; ++i) {
     const long long* q = spec + 6 * i;
@@ -201,7 +201,7 @@ extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, i
   }
   std::vector<RaggedTask> tasks(r.task_pass.size());
   for (size_t i = 0; i < tasks.size(); ++i) tasks[i] = RaggedTask{r.task_pass[i], r.task_tile[i]};
-  std::vector<unsigned> counters(2 + r.n_counters + 1, 0u);
+  std::vector<unsigned> counters(kRaggedCounterBase + r.n_counters + 1, 0u);
   emu::clear_ranges();
   emu::register_range(in, (size_t)(r.in_extent * elem_bytes(in_fmt)));
   emu::register_range(out, (size_t)(r.out_extent * elem_bytes(out_fmt)));
@@ -214,13 +214,19 @@ extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, i
   ra.ws_base = (char*)ws.data();
   ra.passes = dp.data();
   ra.tasks = tasks.data();
-  ra.n_tasks = (int)tasks.size();
   ra.counters = counters.data();
   ra.spin_limit = 1u << 22;
   ra.aops = aops.data();
   void (*fn)(RaggedArgs) = r.cm == 0 ? mcfft_ragged_0 : r.cm == 1 ? mcfft_ragged_1 : mcfft_ragged_2;
   const int lds = kRaggedKernels[r.cm].lds_bytes;
-  emu::launch_concurrent(nblocks, kRaggedNW * 64, (size_t)lds, [&] { fn(ra); });
-  *status = counters[1];
+  const int n_phase = (int)r.phase_begin.size() - 1;
+  for (int ph = 0; ph < (single ? 1 : n_phase); ++ph) {
+    ra.task_begin = single ? 0 : r.phase_begin[ph];
+    ra.task_end = single ? r.phase_begin.back() : r.phase_begin[ph + 1];
+    ra.queue = single ? 3 : ph;
+    ra.use_deps = single ? 1 : 0;
+    emu::launch_concurrent(nblocks, kRaggedNW * 64, (size_t)lds, [&] { fn(ra); });
+  }
+  *status = counters[4];
   return 0;
 }

@@ -112,6 +112,7 @@ inline int thread_id() { return emu::tl_tid; }
 inline int block_id() { return emu::tl_blk->bid; }
 inline int grid_size() { return emu::tl_blk->nblocks; }
 inline int uniform(int v) { return v; }
+inline const void* uniform_ptr(const void* p) { return p; }
 inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
 
 template <class T> inline T lds_ld(const T* p, int i) {

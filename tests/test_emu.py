@@ -87,15 +87,16 @@ def test_plan_describe_and_errors():
         emu.describe([256], 0, 0, 0)         # batch 0
 
 
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
-def test_ragged_queue(fi, fo):
+def test_ragged_queue(fi, fo, single):
     """Mixed lengths/ranks through the persistent ragged kernel; workgroups run
     concurrently, so second-pass tasks really wait on first-pass counters."""
     rng = np.random.default_rng(3)
     shapes = [[256], [256], [256], [8192], [64, 32], [4096], [512], [512], [16384], [16, 16, 32], [32], [2048],
               [128], [65536], [16], [16]]
     xs = [rng.standard_normal(int(np.prod(s))) + 1j * rng.standard_normal(int(np.prod(s))) for s in shapes]
-    ys, status = emu.ragged_exec(xs, shapes, fi, fo, nblocks=4)
+    ys, status = emu.ragged_exec(xs, shapes, fi, fo, nblocks=4, single=single)
     assert status == 0
     for x, s, y in zip(xs, shapes, ys):
         xq = emu.from_storage(emu.to_storage(x, fi), fi)
