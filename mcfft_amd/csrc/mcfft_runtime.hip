@@ -435,7 +435,7 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
   return MCFFT_SUCCESS;
 }
 
-mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
+mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (plan->ragged) {
     if (!in || !out) return MCFFT_INVALID_VALUE;
