@@ -485,7 +485,18 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
 template <int S, int T, int Q, int NW, int CM, int KIND>
 MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   const AOps A = load_aops(a.aops, lane_id());
-  for (int tile = block_id(); tile < a.n_tiles; tile += grid_size()) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+  // Column tiles narrower than a 128-byte line share lines with their
+  // neighbours.  Workgroups are dealt to the 8 XCDs round robin (block b on XCD
+  // b % 8), each XCD with its own L2, so within every aligned group of 32 blocks
+  // the 4 blocks b, b+8, b+16, b+24 (one XCD) take 4 adjacent tiles.  This only
+  // changes which block runs which tile.
+  constexpr bool XCD_GROUP = (KIND == KIND_COL || KIND == KIND_COLF) && Q * (int)sizeof(typename Traits<CM>::T) < 128;
+  int b = block_id();
+  const int g = grid_size();
+  if constexpr (XCD_GROUP) {
+    if (b < (g & ~31)) b = (b & ~31) + 4 * (b & 7) + ((b >> 3) & 3);
+  }
+  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
 }
 
 }  // namespace mcfft
