@@ -122,6 +122,13 @@ mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags);
  * MCFFT_TUNE_CACHE=off disables it.  Not for ragged plans. */
 mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps);
 
+/* Error norms on the GPU in FP64: rel-L2 = ||y - ref||_2 / ||ref||_2 and
+ * max-abs = max |y - ref| over re and im, where y is `n` elements stored in
+ * `fmt` (device pointer) times 2^exponent and `ref` is n interleaved FP64
+ * complex values on the device.  Blocking (synchronises `stream`). */
+mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                              double* rel_l2, double* max_abs, void* stream);
+
 /* Profiling entry points (no reference counterpart): the number of kernel
  * passes of a plan, and an exec that launches only passes [first, first+count)
  * (for every batch chunk), so a caller can time each kernel on its own stream.

@@ -67,9 +67,11 @@ def lib() -> ctypes.CDLL:
         L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_status_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.mcfft_plan_tune.argtypes = [vp, vp, vp, i, vp, i]
+        L.mcfft_error_norms.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), vp]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
-                  "mcfft_get_status_flags", "mcfft_plan_tune",
+                  "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -185,6 +187,19 @@ class RaggedPlan(Plan):
         f = ctypes.c_uint()
         _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
         return f.value
+
+
+def error_norms(y, fmt: str, exponent: int, ref, stream=None):
+    """(rel-L2, max-abs) of device tensor `y` (storage format `fmt`, times
+    2^exponent) against `ref`, a device float64 tensor of interleaved complex
+    values -- reduced on the GPU in FP64."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(y.device).cuda_stream
+    r, m = ctypes.c_double(), ctypes.c_double()
+    _check(lib().mcfft_error_norms(y.data_ptr(), FORMATS[fmt], exponent, ref.data_ptr(), ref.numel() // 2,
+                                   ctypes.byref(r), ctypes.byref(m), stream), "mcfft_error_norms")
+    return r.value, m.value
 
 
 def components(fmt: str) -> int:

@@ -87,6 +87,48 @@ __global__ void reset_counters(unsigned* c, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = 0u;
 }
 
+// FP64 error norms: per-block partial sums of |d|^2, |ref|^2 and max |d|
+__global__ void error_norms_kernel(const void* y, int fmt, double scale, const double* ref, long long n,
+                                   double* part) {
+  __shared__ double s_num[256], s_den[256], s_max[256];
+  double num = 0, den = 0, mx = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double yr, yi;
+    if (fmt == FMT_SPLIT) {
+      const uint2 e = ((const uint2*)y)[i];
+      yr = (double)f16_val(e.x & 0xffffu) + (double)f16_val(e.y & 0xffffu) / 2048.0;
+      yi = (double)f16_val(e.x >> 16) + (double)f16_val(e.y >> 16) / 2048.0;
+    } else {
+      const uint32_t e = ((const uint32_t*)y)[i];
+      yr = fmt == FMT_BF16 ? (double)bf16_val(e & 0xffffu) : (double)f16_val(e & 0xffffu);
+      yi = fmt == FMT_BF16 ? (double)bf16_val(e >> 16) : (double)f16_val(e >> 16);
+    }
+    const double dr = yr * scale - ref[2 * i], di = yi * scale - ref[2 * i + 1];
+    num += dr * dr + di * di;
+    den += ref[2 * i] * ref[2 * i] + ref[2 * i + 1] * ref[2 * i + 1];
+    mx = fmax(mx, fmax(fabs(dr), fabs(di)));
+    if (dr != dr || di != di) mx = dr;  // NaN propagates
+  }
+  s_num[threadIdx.x] = num;
+  s_den[threadIdx.x] = den;
+  s_max[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_num[threadIdx.x] += s_num[threadIdx.x + w];
+      s_den[threadIdx.x] += s_den[threadIdx.x + w];
+      const double o = s_max[threadIdx.x + w];
+      if (!(o <= s_max[threadIdx.x])) s_max[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s_num[0];
+    part[3 * blockIdx.x + 1] = s_den[0];
+    part[3 * blockIdx.x + 2] = s_max[0];
+  }
+}
+
 std::mutex g_attr_mu;
 std::vector<const void*> g_attr_done;  // kernels whose LDS limit was raised
 
@@ -500,6 +542,33 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
   load_cache_locked(plan->arch);
   g_cache[plan->arch + "|" + plan_signature(plan->core)] = choice;
   return save_cache_locked(plan->arch) ? MCFFT_SUCCESS : MCFFT_CACHE_ERROR;
+}
+
+mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                              double* rel_l2, double* max_abs, void* stream) {
+  if (!y || !ref || n < 1 || !rel_l2 || !max_abs || (int)fmt < 0 || (int)fmt > 2) return MCFFT_INVALID_VALUE;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = 1024;
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * 3 * blocks) != hipSuccess) return MCFFT_ALLOC_FAILED;
+  error_norms_kernel<<<blocks, 256, 0, s>>>(y, (int)fmt, ldexp(1.0, exponent), ref, n, part);
+  std::vector<double> h(3 * blocks);
+  mcfftStatus rs = MCFFT_SUCCESS;
+  if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * blocks,
+                                                        hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    rs = MCFFT_HIP_ERROR;
+  (void)hipFree(part);
+  if (rs != MCFFT_SUCCESS) return rs;
+  double num = 0, den = 0, mx = 0;
+  for (int b = 0; b < blocks; ++b) {
+    num += h[3 * b];
+    den += h[3 * b + 1];
+    if (!(h[3 * b + 2] <= mx)) mx = h[3 * b + 2];
+  }
+  *rel_l2 = sqrt(num / (den > 0 ? den : 1.0));
+  *max_abs = mx;
+  return MCFFT_SUCCESS;
 }
 
 mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {

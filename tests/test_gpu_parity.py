@@ -126,3 +126,19 @@ def test_split_roundtrip_c2_full_size(torch_cuda):
     err = oracle.rel_l2(y, xq)
     print(f"roundtrip rel_l2={err:.3e}")
     assert err <= 2e-6
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_gpu_error_norms_match_host(torch_cuda, fmt):
+    """The FP64 GPU error-norm reduction agrees with the host oracle's norms."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    n = 300001
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = x + 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    d = mc.to_device(x, fmt)
+    dref = torch.from_numpy(ref.view(np.float64).copy()).cuda()
+    r, m = mc.error_norms(d, fmt, 3, dref)
+    y = mc.decode(mc.encode(x, fmt), fmt, 3)
+    assert abs(r - oracle.rel_l2(y, ref)) <= 1e-12 * max(1.0, r)
+    assert abs(m - oracle.max_abs(y, ref)) <= 1e-12 * max(1.0, m)
