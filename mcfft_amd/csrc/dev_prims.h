@@ -40,6 +40,12 @@ MCFFT_DEV const void* uniform_ptr(const void* p) {
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
   return (const void*)(((unsigned long long)hi << 32) | lo);
 }
+// a copy of v the compiler cannot see through (keeps values derived from it
+// from being hoisted out of a loop)
+MCFFT_DEV int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 MCFFT_DEV void block_barrier() { __syncthreads(); }
 
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
@@ -63,6 +69,12 @@ MCFFT_DEV u32x2 buf_ld64(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_l
 MCFFT_DEV u32x4 buf_ld128(GBuf b, int off) { return __builtin_amdgcn_raw_buffer_load_b128(b.r, off, 0, 0); }
 MCFFT_DEV void buf_st32(GBuf b, int off, uint32_t v) { __builtin_amdgcn_raw_buffer_store_b32(v, b.r, off, 0, 0); }
 MCFFT_DEV void buf_st64(GBuf b, int off, u32x2 v) { __builtin_amdgcn_raw_buffer_store_b64(v, b.r, off, 0, 0); }
+// the same with a wave-uniform byte offset added in a scalar register
+MCFFT_DEV uint32_t buf_ld32s(GBuf b, int voff, int soff) { return __builtin_amdgcn_raw_buffer_load_b32(b.r, voff, soff, 0); }
+MCFFT_DEV u32x4 buf_ld128s(GBuf b, int voff, int soff) { return __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0); }
+MCFFT_DEV void buf_st32s(GBuf b, int voff, int soff, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, b.r, voff, soff, 0);
+}
 
 // device-scope (all XCDs) atomics and fences for the ragged work queue
 MCFFT_DEV unsigned atomic_add_dev(unsigned* p, unsigned v) {

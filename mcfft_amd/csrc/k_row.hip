@@ -1,10 +1,18 @@
 // k_row.hip -- instantiations of the pass kernel for the MCFFT_ROW_LIST configurations.
 #include "fft_pass.h"
+#include "fft_big.h"
 #include "kernel_list.h"
 
 MCFFT_ROW_LIST(MCFFT_DEFINE_KERNEL)
+// single-pass 65536 points (one workgroup per transform, two-round LDS exchange)
+MCFFT_KERNEL(512) void mcfft_pass_big_0(mcfft::PassArgs a) { mcfft::fft_big_body<0>(a); }
+MCFFT_KERNEL(512) void mcfft_pass_big_1(mcfft::PassArgs a) { mcfft::fft_big_body<1>(a); }
 
 namespace mcfft {
-const KernelInfo kRowKernels[] = {MCFFT_ROW_LIST(MCFFT_TABLE_ENTRY)};
+const KernelInfo kRowKernels[] = {
+    MCFFT_ROW_LIST(MCFFT_TABLE_ENTRY)
+    {"mcfft_pass_big_0", (const void*)&mcfft_pass_big_0, 4, 1, 1, 8, 0, 0, BigCfg::LDS_BYTES},
+    {"mcfft_pass_big_1", (const void*)&mcfft_pass_big_1, 4, 1, 1, 8, 1, 0, BigCfg::LDS_BYTES},
+};
 const int kRowKernelCount = (int)(sizeof(kRowKernels) / sizeof(kRowKernels[0]));
 }  // namespace mcfft

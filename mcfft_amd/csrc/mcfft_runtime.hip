@@ -498,6 +498,8 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MCFFT_HIP_ERROR;
   mcfftStatus rs = MCFFT_SUCCESS;
   std::string choice;
+  const char* venv = getenv("MCFFT_TUNE_VERBOSE");  // print every variant's time to stderr
+  const bool verbose = venv && venv[0] == '1';
   for (size_t pi = 0; pi < plan->core.passes.size() && rs == MCFFT_SUCCESS; ++pi) {
     PassPlan best = plan->core.passes[pi];
     int best_cap = plan->grid_cap[pi];
@@ -523,6 +525,8 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
           rs = MCFFT_HIP_ERROR;
       }
+      if (rs == MCFFT_SUCCESS && verbose)
+        fprintf(stderr, "mcfft tune: pass %zu %s %.4f ms/launch\n", pi, k->name, ms / (float)reps);
       if (rs == MCFFT_SUCCESS && ms < best_ms) {
         best_ms = ms;
         best = trial;

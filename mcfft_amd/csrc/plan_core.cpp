@@ -24,7 +24,7 @@ int ilog2(long long v) {
 // L = 16^S * T with S >= 1, T in {1,2,4,8}
 bool split_len(long long L, int& S, int& T) {
   int lg = ilog2(L);
-  if (lg < 4 || lg > 15) return false;
+  if (lg < 4 || lg > 16) return false;
   S = lg / 4;
   T = 1 << (lg % 4);
   return true;
@@ -167,7 +167,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   if (rank == 1) {
     const long long N = p.dims[0];
     int S0, T0;
-    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW)) {  // single pass up to 2^15
+    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW)) {  // single pass: 2^15, 2^16 in FP16/BF16
       p.chunk = batch;
       p.n_chunks = 1;
       st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);

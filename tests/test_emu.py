@@ -36,6 +36,19 @@ def test_single_pass_32768():
     check([32768], 1, emu.FMT_BF16, emu.FMT_BF16)
 
 
+@pytest.mark.parametrize("fi", [emu.FMT_F16, emu.FMT_BF16])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_single_pass_65536(fi, inverse):
+    """65536 points in one workgroup (registers + two-round LDS exchange)."""
+    assert "mcfft_pass_big_" in emu.describe([65536], 2, fi, fi)
+    check([65536], 2, fi, fi, inverse=inverse)
+
+
+def test_rows_65536_in_2d():
+    assert "mcfft_pass_big_1" in emu.describe([16, 65536], 1, emu.FMT_BF16, emu.FMT_BF16)
+    check([16, 65536], 1, emu.FMT_BF16, emu.FMT_BF16)
+
+
 @pytest.mark.parametrize("n,batch,ws", [(65536, 1, 0), (65536, 3, 65536 * 8), (1 << 17, 1, 0)])
 @pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
 def test_four_step(n, batch, ws, fi, fo):
