@@ -20,6 +20,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libmcfft.so")
+# MCFFT_LIB: load another build of the same library (e.g. an experimental variant)
+LIB_PATH = os.environ.get("MCFFT_LIB", LIB_PATH)
 
 FORMATS = {"fp16": 0, "bf16": 1, "split": 2}
 SCALINGS = {"none": 0, "balanced": 1, "full": 2}

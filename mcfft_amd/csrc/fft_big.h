@@ -37,8 +37,20 @@ struct BigCfg {
   static constexpr int rrc(int s) { return (s & 3) + 8 * (s >> 2); }  // rr - 4h of MFMA output slot s
 };
 
+// GBuf of transform `tile` (ROW layout, one transform per tile)
+MCFFT_DEV GBuf big_view(const void* p, long long sg, long long st, int tpg, int tile) {
+  const int grp = tile / tpg, til = tile - grp * tpg;
+  const long long base = (long long)grp * sg + (long long)til * st;
+  return make_gbuf((const char*)p + base * 4, (unsigned)BigCfg::L * 4u);
+}
+
+// One transform whose stage-0 inputs are already in x.  The last stage stores
+// MFMA tile k and, if next >= 0, immediately loads tile k of transform `next`
+// into the registers it just freed, so the next transform's loads overlap this
+// one's stores (one workgroup per CU leaves no other overlap).
 template <int CM>
-MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, const AOps& A) {
+MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A,
+                            typename Traits<CM>::T (&x)[BigCfg::TPW][8]) {
   static_assert(CM == CM_F16 || CM == CM_BF16, "4-byte elements only");
   typedef Traits<CM> Tr;
   typedef typename Tr::T El;
@@ -50,26 +62,14 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, const AOps& A) {
   const int tid = opaque(thread_id());
   const int lane = tid & 63, wave = tid >> 6, h = lane >> 5, c5 = lane & 31;
   const int low = 32 * wave + c5;  // column j of MFMA tile k is low + 256 k
-  const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
-  const long long ibase = (long long)grp * a.in_sg + (long long)til * a.in_st;
-  const long long obase = (long long)grp * a.out_sg + (long long)til * a.out_st;
-  const GBuf gin = make_gbuf((const char*)a.in + ibase * 4, (unsigned)B::L * 4u);
-  const GBuf gout = make_gbuf((char*)a.out + obase * 4, (unsigned)B::L * 4u);
+  const GBuf gout = big_view(a.out, a.out_sg, a.out_st, a.tiles_per_group, tile);
+  const GBuf gnext = big_view(a.in, a.in_sg, a.in_st, a.tiles_per_group, next < 0 ? tile : next);
   const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);
   const int rbase = low + (low >> 4) + 2 * h * 8704;
+  const int vin = (low + 8 * h * B::LQ) * 4;  // stage-0 input r = 8h + t of column j: element j + r * 4096
 
-  El x[B::TPW][8];
   El tmp[8][4];
   constexpr int T0[4] = {0, 1, 4, 5}, T1[4] = {2, 3, 6, 7};
-
-  // stage 0 input r = 8h + t of column j: element j + r * 4096
-  {
-    const int v = (low + 8 * h * B::LQ) * 4;
-#pragma unroll
-    for (int k = 0; k < B::TPW; ++k)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) x[k][t] = buf_ld32s(gin, v, k * 1024 + t * 16384);
-  }
 
   For<0, 4>::run([&](auto pc) __attribute__((always_inline)) {
     constexpr int P = decltype(pc)::value;
@@ -105,6 +105,12 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, const AOps& A) {
           } else {
             buf_st32s(gout, vout, (k * 256 + B::rrc(s) * B::LQ) * 4, Tr::pack(v * a.out_scale));
           }
+        }
+      }
+      if constexpr (P == 3) {
+        if (next >= 0) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) x[k][t] = buf_ld32s(gnext, vin, k * 1024 + t * 16384);
         }
       }
     }
@@ -151,8 +157,24 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, const AOps& A) {
 
 template <int CM>
 MCFFT_DEV void fft_big_body(const PassArgs& a) {
+  typedef BigCfg B;
   const AOps A = load_aops(a.aops, lane_id());
-  for (int tile = block_id(); tile < a.n_tiles; tile += grid_size()) fft_big_tile<CM>(a, tile, A);
+  typename Traits<CM>::T x[B::TPW][8];
+  const int g = grid_size();
+  int tile = block_id();
+  if (tile < a.n_tiles) {
+    const int tid = thread_id(), lane = tid & 63, low = 32 * (tid >> 6) + (lane & 31);
+    const int vin = (low + 8 * (lane >> 5) * B::LQ) * 4;
+    const GBuf gin = big_view(a.in, a.in_sg, a.in_st, a.tiles_per_group, tile);
+#pragma unroll
+    for (int k = 0; k < B::TPW; ++k)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) x[k][t] = buf_ld32s(gin, vin, k * 1024 + t * 16384);
+  }
+  for (; tile < a.n_tiles; tile += g) {
+    const int next = tile + g < a.n_tiles ? tile + g : -1;
+    fft_big_tile<CM>(a, tile, next, A, x);
+  }
 }
 
 }  // namespace mcfft

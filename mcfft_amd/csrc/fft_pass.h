@@ -265,6 +265,7 @@ template <int E> struct For<E, E> {
 template <int S, int T, int Q, int NW, int CM, int KIND>
 struct PassCfg {
   static constexpr int L = ipow16(S) * T;
+  static constexpr int Q_ = Q, NW_ = NW;
   static constexpr int NT = NW * 64;
   static constexpr int NSTAGE = S + (T > 1 ? 1 : 0);
   static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF);
@@ -274,6 +275,11 @@ struct PassCfg {
   static constexpr int LDS_ELEMS = QFAST ? Q * L : Q * ROW;
   typedef typename Traits<CM>::T El;
   static constexpr int LDS_BYTES = LDS_ELEMS * (int)sizeof(El);
+  static constexpr int NCOLT = Q * L / 512;          // 32-column MFMA tiles of the tile
+  static constexpr int TPW = (NCOLT + NW - 1) / NW;  // MFMA tiles per wave
+  // Passes of two or more stages load the next tile's stage-0 inputs while the
+  // current tile is processed (see fft_pass_body).
+  static constexpr bool PREFETCH = NSTAGE > 1;
   static_assert(S >= 1, "at least one radix-16 stage");
   static_assert((Q * L) % 512 == 0, "tile must hold a multiple of 32 sixteen-point columns");
   static_assert(T == 1 || T == 2 || T == 4 || T == 8, "tail radix");
@@ -297,10 +303,51 @@ struct PassCfg {
   }
 };
 
+MCFFT_DEV GBuf clamp_view(const void* base, long long first, long long total, int esz) {
+  const long long rem = (total - first) * esz;
+  return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
+}
+
+// Input view of a tile.  Column kinds always run full tiles (the planner
+// guarantees it).  A ROW tile's instances are contiguous, so its view ends at
+// the last valid instance: loads beyond read 0.
+template <class C, int KIND>
+MCFFT_DEV GBuf tile_in_view(const PassArgs& a, int tile, int esz_in) {
+  const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
+  const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
+  long long in_end = a.in_elems;
+  if constexpr (KIND == KIND_ROW) {
+    const int inst0 = til * C::Q_;
+    const long long vq = (long long)(a.n_inst - inst0 < C::Q_ ? a.n_inst - inst0 : C::Q_);
+    in_end = ib + vq * C::L < in_end ? ib + vq * C::L : in_end;
+  }
+  return clamp_view(a.in, ib, in_end, esz_in);
+}
+
+// Stage-0 inputs of one tile: MFMA tile k = wave + k*NW, lane inputs r = 8h + t.
+template <class C, class Tr>
+MCFFT_DEV void load_stage0(const PassArgs& a, GBuf gin, int wave, int lane, typename C::El (*x)[8]) {
+  constexpr int L = C::L;
+  const int h = lane >> 5;
+  for (int k = 0; k < C::TPW; ++k) {
+    const int ti = wave + k * C::NW_;
+    if (ti < C::NCOLT) {
+      int q, j;
+      C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+      const int o0 = C::in_off(q, j + 8 * h * (L / 16), a.stride_in);
+      const int ostep = C::in_off(0, L / 16, a.stride_in);
+      for (int t = 0; t < 8; ++t) x[k][t] = Tr::load(gin, o0 + t * ostep, a.in_fmt);
+    }
+  }
+}
+
 // One tile of a pass: stages on the tile's Q instances, ending with a barrier
-// (LDS is reused by the workgroup's next tile).
-template <int S, int T, int Q, int NW, int CM, int KIND>
-MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
+// (LDS is reused by the workgroup's next tile).  With PF the tile's stage-0
+// inputs arrive in `pre` (loaded earlier) and, once stage 0 has consumed them,
+// `pre` is refilled with the inputs of tile `next` (when next >= 0).
+template <int S, int T, int Q, int NW, int CM, int KIND, bool PF = false>
+MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
+                        typename PassCfg<S, T, Q, NW, CM, KIND>::El (*pre)[8] = nullptr, int next = -1) {
   typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
@@ -308,7 +355,9 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   constexpr bool STAGED = C::STAGED, FOURSTEP = C::FOURSTEP;
   MCFFT_DYN_LDS(El, lds);
 
-  const int tid = thread_id(), lane = lane_id(), wave = tid >> 6, h = lane >> 5;
+  // opaque per tile: index math is redone for each tile instead of being hoisted
+  // out of the tile loop and held in registers (or spilled) across it
+  const int tid = opaque(thread_id()), lane = tid & 63, wave = tid >> 6, h = lane >> 5;
   const bool inv = a.inverse != 0;
   const int sin_ = a.stride_in, sout = a.stride_out;
   const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
@@ -316,10 +365,6 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);
   GBuf gfs;
   if constexpr (FOURSTEP) gfs = make_gbuf(a.fs_tw, (unsigned)a.fs_tw_entries * 8u);
-  auto view = [](const void* base, long long first, long long total, int esz) -> GBuf {
-    const long long rem = (total - first) * esz;
-    return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
-  };
 
   {
     const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
@@ -335,8 +380,8 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       in_end = ib + vq * L < in_end ? ib + vq * L : in_end;
       out_end = ob + vq * L < out_end ? ob + vq * L : out_end;
     }
-    const GBuf gin = view(a.in, ib, in_end, esz_in);
-    const GBuf gout = view(a.out, ob, out_end, esz_out);
+    const GBuf gin = clamp_view(a.in, ib, in_end, esz_in);
+    const GBuf gout = clamp_view(a.out, ob, out_end, esz_out);
 
     // output post-processing of the pass's last stage: four-step twiddle or scale
     auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
@@ -355,19 +400,22 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       constexpr int P = decltype(pc)::value;
       constexpr int NS = ipow16(P);
       constexpr bool LAST = (P == C::NSTAGE - 1);
-      constexpr int NCOLT = Q * L / 512;             // 32-column MFMA tiles
-      constexpr int TPW = (NCOLT + NW - 1) / NW;     // tiles per wave
+      constexpr int NCOLT = C::NCOLT;
+      constexpr int TPW = C::TPW;
       El x[TPW][8];
-      for (int k = 0; k < TPW; ++k) {
-        const int ti = wave + k * NW;
-        if (ti < NCOLT) {
-          int q, j;
-          C::template colmap<16>(ti * 32 + (lane & 31), q, j);
-          if constexpr (P == 0) {
-            const int o0 = C::in_off(q, j + 8 * h * (L / 16), sin_);
-            const int ostep = C::in_off(0, L / 16, sin_);
-            for (int t = 0; t < 8; ++t) x[k][t] = Tr::load(gin, o0 + t * ostep, a.in_fmt);
-          } else {
+      if constexpr (P == 0) {
+        if constexpr (PF) {
+          for (int k = 0; k < TPW; ++k)
+            for (int t = 0; t < 8; ++t) x[k][t] = pre[k][t];
+        } else {
+          load_stage0<C, Tr>(a, gin, wave, lane, x);
+        }
+      } else {
+        for (int k = 0; k < TPW; ++k) {
+          const int ti = wave + k * NW;
+          if (ti < NCOLT) {
+            int q, j;
+            C::template colmap<16>(ti * 32 + (lane & 31), q, j);
             for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, C::lidx(q, j + (8 * h + t) * (L / 16)));
           }
         }
@@ -434,30 +482,26 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           }
         }
       }
+      if constexpr (PF && P == 0) {
+        // the next tile's inputs: in flight while this tile's remaining stages run
+        if (next >= 0) load_stage0<C, Tr>(a, tile_in_view<C, KIND>(a, next, esz_in), wave, lane, pre);
+      }
       if constexpr (!LAST || STAGED) block_barrier();
     });
 
     // ---------------- radix-T tail stage (VALU, FP32) ----------------
+    // Column (q, j) reads and writes the same T positions j + r*L/T, so with
+    // the staged store the results go back in place without a barrier.
     if constexpr (T > 1) {
       constexpr int NCOL = Q * L / T;
       constexpr int CPT = (NCOL + C::NT - 1) / C::NT;
-      El x[CPT][T];
-      for (int k = 0; k < CPT; ++k) {
-        const int c = tid + k * C::NT;
-        if (c < NCOL) {
-          int q, j;
-          C::template colmap<T>(c, q, j);
-          for (int r = 0; r < T; ++r) x[k][r] = lds_ld(lds, C::lidx(q, j + r * (L / T)));
-        }
-      }
-      if constexpr (STAGED) block_barrier();
       for (int k = 0; k < CPT; ++k) {
         const int c = tid + k * C::NT;
         if (c < NCOL) {
           int q, j;
           C::template colmap<T>(c, q, j);
           f32x2 v[T];
-          for (int r = 0; r < T; ++r) v[r] = Tr::unpack(x[k][r]);
+          for (int r = 0; r < T; ++r) v[r] = Tr::unpack(lds_ld(lds, C::lidx(q, j + r * (L / T))));
           dft_small<T>(v, inv);
           for (int r = 0; r < T; ++r) {
             const int mo = j + r * (L / T);
@@ -496,7 +540,16 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   if constexpr (XCD_GROUP) {
     if (b < (g & ~31)) b = (b & ~31) + 4 * (b & 7) + ((b >> 3) & 3);
   }
-  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+  typedef PassCfg<S, T, Q, NW, CM, KIND> C;
+  if constexpr (C::PREFETCH) {
+    typename C::El pre[C::TPW][8];
+    const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
+    if (b < a.n_tiles) load_stage0<C, Traits<CM>>(a, tile_in_view<C, KIND>(a, b, esz_in), thread_id() >> 6, lane_id(), pre);
+    for (int tile = b; tile < a.n_tiles; tile += g)
+      fft_tile<S, T, Q, NW, CM, KIND, true>(a, tile, A, pre, tile + g < a.n_tiles ? tile + g : -1);
+  } else {
+    for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+  }
 }
 
 }  // namespace mcfft

@@ -127,3 +127,27 @@ def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=Fal
     y = from_storage(out, out_fmt)
     outs = [y[oo:oo + s] * (2.0 ** e) for (io, oo), s, e in zip(layout, sizes, exps)]
     return outs, st.value
+
+
+def ragged_check(shapes, in_fmt, out_fmt, scaling=1):
+    """Structural checks of the ragged plan of `shapes` laid out back to back
+    (no kernel runs).  Returns (violations, stats)."""
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    spec = np.zeros((len(shapes), 6), dtype=np.int64)
+    for i, s in enumerate(shapes):
+        spec[i, 0] = len(s)
+        spec[i, 1:1 + len(s)] = s
+        for d in range(len(s), 3):
+            spec[i, 1 + d] = 1
+        spec[i, 4] = spec[i, 5] = offs[i]
+    st = np.zeros(6, dtype=np.int64)
+    L = lib()
+    L.emu_ragged_check.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p]
+    L.emu_ragged_check.restype = ctypes.c_longlong
+    bad = L.emu_ragged_check(len(shapes), spec.ctypes.data, in_fmt, out_fmt, scaling, st.ctypes.data)
+    if bad < 0:
+        raise RuntimeError(f"ragged_build status {-bad}")
+    keys = ("tasks", "passes", "phases", "counters", "ws_elems", "tw_entries")
+    return int(bad), {k: int(v) for k, v in zip(keys, st)}

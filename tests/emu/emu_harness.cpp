@@ -230,3 +230,59 @@ extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, i
   *status = counters[4];
   return 0;
 }
+
+// Structural checks of a ragged plan (no kernel runs): every dependency wait is
+// satisfiable by tiles queued before it, tile indices are in range, buffer
+// views stay inside the exec buffers / workspace, and phases hold one pass
+// level each.  Returns the number of violations (first few printed).
+extern "C" long long emu_ragged_check(int n_items, const long long* spec, int in_fmt, int out_fmt, int scaling,
+                                      long long* stats) {
+  std::vector<RaggedItemSpec> items(n_items);
+  for (int i = 0; i < n_items; ++i) {
+    const long long* q = spec + 6 * i;
+    items[i] = RaggedItemSpec{(int)q[0], {q[1], q[2], q[3]}, q[4], q[5]};
+  }
+  RaggedPlanCore r;
+  int st = ragged_build(r, n_items, items.data(), in_fmt, out_fmt, scaling);
+  if (st) return -st;
+  const int ws_fmt = (r.cm == 2) ? 2 : in_fmt;
+  long long bad = 0;
+  auto fail = [&](const char* what, long long a, long long b) {
+    if (bad++ < 8) fprintf(stderr, "ragged check: %s (%lld, %lld)\n", what, a, b);
+  };
+  const size_t nt = r.task_pass.size();
+  std::vector<long long> signalled(r.n_counters + 1, 0), needed(r.n_counters + 1, 0);
+  // level of each pass inside its unit = number of earlier passes with the same counter
+  for (size_t i = 0; i < nt; ++i) {
+    const int pi = r.task_pass[i];
+    if (pi < 0 || pi >= (int)r.passes.size()) { fail("pass index", (long long)i, pi); continue; }
+    const RaggedPassHost& h = r.passes[pi];
+    const long long tiles = h.pp.groups * h.pp.tiles_per_group;
+    if (r.task_tile[i] < 0 || r.task_tile[i] >= tiles) fail("tile index", (long long)i, r.task_tile[i]);
+    if (h.wait_for > 0 && signalled[h.counter] < h.wait_for) fail("wait not satisfiable", (long long)i, h.wait_for);
+    if (h.signal) ++signalled[h.counter];
+    needed[h.counter] = std::max(needed[h.counter], (long long)h.wait_for);
+  }
+  for (int c = 0; c < r.n_counters; ++c)
+    if (signalled[c] != needed[c]) fail("counter total", c, signalled[c] - needed[c]);
+  for (size_t pi = 0; pi < r.passes.size(); ++pi) {
+    const RaggedPassHost& h = r.passes[pi];
+    const long long lim_in = h.pp.src == BUF_IN ? r.in_extent : h.pp.src == BUF_OUT ? r.out_extent : r.ws_elems;
+    const long long lim_out = h.pp.dst == BUF_OUT ? r.out_extent : r.ws_elems;
+    if (h.in_off < 0 || h.in_off + h.in_elems > lim_in) fail("input view", (long long)pi, h.in_off + h.in_elems - lim_in);
+    if (h.out_off < 0 || h.out_off + h.out_elems > lim_out) fail("output view", (long long)pi, h.out_off + h.out_elems - lim_out);
+    if (h.variant < 0 || h.variant >= kRaggedVariants) fail("variant", (long long)pi, h.variant);
+    const long long esz = h.pp.src == BUF_WS ? elem_bytes(ws_fmt) : elem_bytes(h.pp.src_fmt);
+    if (h.in_elems * esz > 0xffffffffLL) fail("input view above 4 GiB", (long long)pi, h.in_elems);
+  }
+  for (size_t k = 0; k + 1 < r.phase_begin.size(); ++k)
+    if (r.phase_begin[k] > r.phase_begin[k + 1]) fail("phase order", (long long)k, r.phase_begin[k]);
+  if (r.phase_begin.back() != (int)nt) fail("phase end", r.phase_begin.back(), (long long)nt);
+  stats[0] = (long long)nt;
+  stats[1] = (long long)r.passes.size();
+  stats[2] = (long long)r.phase_begin.size() - 1;
+  stats[3] = r.n_counters;
+  stats[4] = r.ws_elems;
+  stats[5] = (long long)r.tw.size();
+  return bad;
+}

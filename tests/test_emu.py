@@ -39,9 +39,10 @@ def test_single_pass_32768():
 @pytest.mark.parametrize("fi", [emu.FMT_F16, emu.FMT_BF16])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_single_pass_65536(fi, inverse):
-    """65536 points in one workgroup (registers + two-round LDS exchange)."""
-    assert "mcfft_pass_big_" in emu.describe([65536], 2, fi, fi)
-    check([65536], 2, fi, fi, inverse=inverse)
+    """65536 points in one workgroup (registers + two-round LDS exchange); 5
+    transforms on 3 workgroups, so two workgroups prefetch a second transform."""
+    assert "mcfft_pass_big_" in emu.describe([65536], 5, fi, fi)
+    check([65536], 5, fi, fi, inverse=inverse)
 
 
 def test_rows_65536_in_2d():
@@ -115,6 +116,34 @@ def test_ragged_queue(fi, fo, single):
         xq = emu.from_storage(emu.to_storage(x, fi), fi)
         err = oracle.rel_l2(y, oracle.fft(xq, tuple(s)))
         assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (s, err)
+
+
+def test_ragged_long_columns():
+    """2D items whose strided side is 512 or 2048 (column variants of the ragged
+    kernel not reached by the mixed set above)."""
+    rng = np.random.default_rng(6)
+    shapes = [[2048, 64], [512, 128], [64, 2048]]
+    xs = [rng.standard_normal(int(np.prod(s))) + 1j * rng.standard_normal(int(np.prod(s))) for s in shapes]
+    ys, status = emu.ragged_exec(xs, shapes, emu.FMT_F16, emu.FMT_F16, nblocks=4, single=True)
+    assert status == 0
+    for x, s, y in zip(xs, shapes, ys):
+        xq = emu.from_storage(emu.to_storage(x, emu.FMT_F16), emu.FMT_F16)
+        assert oracle.rel_l2(y, oracle.fft(xq, tuple(s))) <= TOL[emu.FMT_F16], s
+
+
+def test_ragged_plan_structure_large():
+    """Structural checks of the plan of a 20064-item batch (20000 1D lengths
+    2^8..2^18 plus 64 2D shapes with sides 256..2048, grouped by shape): every
+    dependency wait is satisfiable by earlier queue entries, tiles and buffer
+    views are in range (no kernel runs)."""
+    rng = np.random.default_rng(5)
+    shapes = [[int(2 ** k)] for k in rng.integers(8, 19, 20000)]
+    shapes += [[int(rng.choice([256, 512, 1024, 2048])), int(rng.choice([256, 512, 1024, 2048]))] for _ in range(64)]
+    shapes.sort(key=lambda s: (len(s), s))
+    for fi, fo in ((emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)):
+        bad, stats = emu.ragged_check(shapes, fi, fo)
+        assert bad == 0, stats
+        assert stats["phases"] == 2 and stats["tasks"] > 200000
 
 
 def test_ragged_offsets_and_inverse():
