@@ -51,7 +51,8 @@ def _bytes(fi, fo):
     return (8 if fi == "split" else 4) + (8 if fo == "split" else 4)
 
 
-def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transforms, direction=mc.FORWARD):
+def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transforms, direction=mc.FORWARD,
+                vendor=True):
     import oracle
     n = int(np.prod(dims))
     p = mc.Plan(dims, batch, fi, fo, scaling)
@@ -73,7 +74,41 @@ def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transf
     ref = oracle.fft(xq, tuple(dims), sign=-1 if direction == mc.FORWARD else 1)
     res.update({"rel_l2": oracle.rel_l2(y, ref), "max_abs": oracle.max_abs(y, ref), "checked": k})
     p.destroy()
+    if vendor:
+        res.update(run_rocfft(torch, dims, batch, x, ins, fi, fo, reps, k, ref, direction, ms))
     return res
+
+
+def run_rocfft(torch, dims, batch, x, ins, fi, fo, reps, k, ref, direction, ours_ms):
+    """rocFFT on the same data: half precision (same 4-byte interleaved storage)
+    against plain FP16/BF16 runs, single precision against split runs."""
+    import oracle
+    from mcfft_amd import vendor_baseline as vb
+    n = int(np.prod(dims))
+    prec = "single" if fo == "split" else "half"
+    try:
+        rp = vb.RocfftPlan(dims, batch, prec, inverse=(direction == mc.INVERSE))
+    except Exception as e:  # noqa: BLE001 -- report, do not fail the run
+        return {"rocfft": f"unavailable: {e}"}
+    if prec == "half":
+        if fi != "fp16":
+            return {}
+        vins = ins
+    else:
+        xs = torch.from_numpy(np.ascontiguousarray(mc.decode(mc.encode(x, fi), fi).astype(np.complex64)).view(np.float32))
+        vins = [xs.to("cuda").view(-1, 2)]
+    vout = torch.empty((n * batch, 2), dtype=vins[0].dtype, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    rot = len(vins)
+    vms = _timer(torch, lambda i: rp.exec(vins[i % rot].data_ptr(), vout.data_ptr(), st), reps)
+    rp.exec(vins[0].data_ptr(), vout.data_ptr(), st)
+    torch.cuda.synchronize()
+    yv = vout[: k * n].double().cpu().numpy()
+    yv = yv[:, 0] + 1j * yv[:, 1]
+    rp.destroy()
+    flops = 5.0 * n * math.log2(n) * batch
+    return {"rocfft_precision": prec, "rocfft_ms": round(vms, 4), "rocfft_gflops": round(flops / (vms * 1e-3) / 1e9, 1),
+            "rocfft_rel_l2": oracle.rel_l2(yv, ref), "speedup_vs_rocfft": round(vms / ours_ms, 3)}
 
 
 def config_inputs(cfg, rng, dims, batch):

@@ -40,6 +40,15 @@ MCFFT_DEV const void* uniform_ptr(const void* p) {
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
   return (const void*)(((unsigned long long)hi << 32) | lo);
 }
+// (cos 2 pi x, sin 2 pi x) from the hardware transcendental units, whose argument
+// is in revolutions (v_cos_f32 / v_sin_f32; |error| <= 1.3e-7 over x = e / 2^k,
+// e < 2^23, measured on MI355X -- about one FP32 ulp of 1)
+MCFFT_DEV f32x2 cis_rev(float x) {
+  f32x2 r;
+  r.x = __builtin_amdgcn_cosf(x);
+  r.y = __builtin_amdgcn_sinf(x);
+  return r;
+}
 // a copy of v the compiler cannot see through (keeps values derived from it
 // from being hoisted out of a loop)
 MCFFT_DEV int opaque(int v) {
@@ -74,6 +83,9 @@ MCFFT_DEV uint32_t buf_ld32s(GBuf b, int voff, int soff) { return __builtin_amdg
 MCFFT_DEV u32x4 buf_ld128s(GBuf b, int voff, int soff) { return __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0); }
 MCFFT_DEV void buf_st32s(GBuf b, int voff, int soff, uint32_t v) {
   __builtin_amdgcn_raw_buffer_store_b32(v, b.r, voff, soff, 0);
+}
+MCFFT_DEV void buf_st64s(GBuf b, int voff, int soff, u32x2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(v, b.r, voff, soff, 0);
 }
 
 // device-scope (all XCDs) atomics and fences for the ragged work queue

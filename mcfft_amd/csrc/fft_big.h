@@ -64,7 +64,6 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
   const int low = 32 * wave + c5;  // column j of MFMA tile k is low + 256 k
   const GBuf gout = big_view(a.out, a.out_sg, a.out_st, a.tiles_per_group, tile);
   const GBuf gnext = big_view(a.in, a.in_sg, a.in_st, a.tiles_per_group, next < 0 ? tile : next);
-  const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);
   const int rbase = low + (low >> 4) + 2 * h * 8704;
   const int vin = (low + 8 * h * B::LQ) * 4;  // stage-0 input r = 8h + t of column j: element j + r * 4096
 
@@ -77,7 +76,6 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
                     : P == 1 ? (low >> 4) * 272 + (low & 15) + 68 * h
                              : low + (low >> 4) + 272 * 4 * h;
     constexpr int wmul = P == 0 ? 1 : P == 1 ? 17 : 272;
-    const int vtw = (16 * low + 4 * h) * 8;       // twiddle row j, entries 4h + 8g + u
     const int vout = (low + 4 * h * B::LQ) * 4;   // last stage: element j + rr * 4096
 #pragma unroll
     for (int k = 0; k < B::TPW; ++k) {
@@ -86,13 +84,18 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
       for (int g = 0; g < 2; ++g) {
         f32x2 tw[4];
         if constexpr (P < 3) {
-          const int so = P * B::L * 8 + k * 256 * 16 * 8 + g * 64;
-          const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128s(gtw, vtw, so));
-          const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128s(gtw, vtw, so + 16));
-          tw[0] = t01.xy;
-          tw[1] = t01.zw;
-          tw[2] = t23.xy;
-          tw[3] = t23.zw;
+          // element written at m' = jb + rr * NS is input m' >> 12 of the next
+          // stage's column m' & 4095: twiddle W_{256 NS}^((m' >> 12) * (m' % 16NS))
+          constexpr int NS = ipow16(P);
+          constexpr float INV_M = 1.0f / (float)(256 * NS);
+          const int j = low + 256 * k;
+          const int mo0 = (j / NS) * NS * 16 + (j % NS) + (4 * h + 8 * g) * NS;
+          const float sc = a.st_scale[P];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int mo = mo0 + u * NS;
+            tw[u] = twiddle_rev((mo >> 12) * (mo & (16 * NS - 1)), INV_M, a.inverse != 0, sc);
+          }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

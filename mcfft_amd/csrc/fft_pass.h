@@ -14,8 +14,13 @@
 // m' = (j/Ns)*Ns*R + j%Ns + r'*Ns, so input and output are in natural order and
 // every stage reads 32 consecutive j (or q) per MFMA column block.  The twiddle a
 // stage's input needs is applied by the previous stage to its FP32 accumulators
-// before they are rounded (one rounding per stage); per-stage power-of-two
-// scale factors are folded into those tables.
+// before they are rounded (one rounding per stage), with the stage's
+// power-of-two scale factor.  The element written at m' is input
+// r2 = m' / (L/Rn) of column j2 = m' % (L/Rn) of the next stage (radix Rn), so
+// its twiddle is W_M^(r2 * (m' % 16Ns)), M = 16 Ns Rn: an exact rational angle.
+// FP16/BF16 passes evaluate it with the hardware sin/cos (argument in
+// revolutions, about one FP32 ulp); split passes, whose results carry about 22
+// bits, read FP32 tables generated in FP64 (scale folded in).
 //
 // Complex GEMM on real MFMA: with F = Fr + i Fi and X = Xr + i Xi,
 //     [Yr; Yi] (32 x 32 cols) = [Fr; Fi] * Xr + [-Fi; Fr] * Xi
@@ -65,6 +70,7 @@ struct PassArgs {
   int fs_cols;                // N2 (columns of the four-step decomposition)
   long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
   int stage_tw_entries, fs_tw_entries;
+  float st_scale[6];          // 2^-shift of stage P (twiddles computed in the kernel carry it)
 };
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:
@@ -95,6 +101,7 @@ template <> struct Traits<CM_F16> {
   static MCFFT_DEV f32x2 unpack(T e) { return up_f16(e); }
   static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
   static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, i * 4, si * 4, v); }
 };
 template <> struct Traits<CM_BF16> {
   typedef uint32_t T;
@@ -102,6 +109,7 @@ template <> struct Traits<CM_BF16> {
   static MCFFT_DEV f32x2 unpack(T e) { return up_bf16(e); }
   static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
   static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, i * 4, si * 4, v); }
 };
 template <> struct Traits<CM_SPLIT> {
   typedef u32x2 T;  // .x = hi pair (re, im), .y = lo pair
@@ -120,6 +128,7 @@ template <> struct Traits<CM_SPLIT> {
     return e;
   }
   static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st64(b, i * 8, v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st64s(b, i * 8, si * 8, v); }
 };
 
 // DFT-matrix operands of one lane: A1 = [Fr; Fi], A2 = [-Fi; Fr] (hi and lo parts)
@@ -238,6 +247,16 @@ MCFFT_DEV void dft_small(f32x2 (&v)[T], bool inv) {
 }
 
 constexpr int ipow16(int s) { return s == 0 ? 1 : 16 * ipow16(s - 1); }
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// W_M^e * scale for e < 2^24 (forward exp(-2 pi i e / M), inverse exp(+...));
+// inv_m = 1/M is a power of two, so e * inv_m is exact
+MCFFT_DEV f32x2 twiddle_rev(int e, float inv_m, bool inv, float scale) {
+  f32x2 w = cis_rev((float)e * inv_m);
+  w.x *= scale;
+  w.y *= inv ? scale : -scale;
+  return w;
+}
 
 template <int V> struct IC { static constexpr int value = V; };
 template <int P, int E> struct For {
@@ -384,9 +403,18 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
     const GBuf gout = clamp_view(a.out, ob, out_end, esz_out);
 
     // output post-processing of the pass's last stage: four-step twiddle or scale
+    // four-step twiddle W_N^(m * column), N = L * fs_cols (m < L, column < fs_cols)
+    const float inv_n = 1.0f / ((float)L * (float)a.fs_cols);
+    auto fs_twiddle = [&](int q, int m) -> f32x2 {
+      if constexpr (CM == CM_SPLIT) {
+        return __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8));
+      } else {
+        return twiddle_rev(m * (inst0 + q), inv_n, inv, a.out_scale);
+      }
+    };
     auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
       if constexpr (FOURSTEP) {
-        return cmul(v, __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8)));
+        return cmul(v, fs_twiddle(q, m));
       } else {
         return v * a.out_scale;
       }
@@ -433,17 +461,29 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
           const int jb = (j / NS) * NS * 16 + (j % NS);
           for (int g = 0; g < 2; ++g) {
             f32x2 tw[4];
-            if constexpr (!LAST) {
-              // entries rr = 4h + 8g + 0..3 of row j: two 16-byte loads
-              const int toff = (P * L + j * 16 + 4 * h + 8 * g) * 8;
-              const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
-              const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
-              tw[0] = t01.xy;
-              tw[1] = t01.zw;
-              tw[2] = t23.xy;
-              tw[3] = t23.zw;
-            }
             const int mo0 = jb + (4 * h + 8 * g) * NS;
+            if constexpr (!LAST) {
+              if constexpr (CM == CM_SPLIT) {
+                // table entries rr = 4h + 8g + 0..3 of row j: two 16-byte loads
+                const int toff = (P * L + j * 16 + 4 * h + 8 * g) * 8;
+                const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
+                const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
+                tw[0] = t01.xy;
+                tw[1] = t01.zw;
+                tw[2] = t23.xy;
+                tw[3] = t23.zw;
+              } else {
+                constexpr int RN = (P + 1 < S) ? 16 : T;      // radix of the next stage
+                constexpr int LGQ2 = ilog2c(L / RN);          // its column count L/RN
+                constexpr int NSN = 16 * NS;
+                constexpr float INV_M = 1.0f / (float)(NSN * RN);
+                const float sc = a.st_scale[P];
+                for (int u = 0; u < 4; ++u) {
+                  const int mo = mo0 + u * NS;
+                  tw[u] = twiddle_rev((mo >> LGQ2) * (mo & (NSN - 1)), INV_M, inv, sc);
+                }
+              }
+            }
             for (int u = 0; u < 4; ++u) {
               const int s = 4 * g + u;
               const int mo = mo0 + u * NS;
@@ -460,18 +500,13 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
               // direct store: offsets and four-step exponents advance linearly in u
               {
                 const int o0 = C::out_off(q, mo0, sout), ostep = C::out_off(0, NS, sout);
-                int w0 = 0, wstep = 0;
-                if constexpr (FOURSTEP) {
-                  w0 = (mo0 * a.fs_cols + inst0 + q) * 8;
-                  wstep = NS * a.fs_cols * 8;
-                }
                 for (int u = 0; u < 4; ++u) {
                   const int s = 4 * g + u;
                   f32x2 v;
                   v.x = y[s];
                   v.y = y[8 + s];
                   if constexpr (FOURSTEP) {
-                    v = cmul(v, __builtin_bit_cast(f32x2, buf_ld64(gfs, w0 + u * wstep)));
+                    v = cmul(v, fs_twiddle(q, mo0 + u * NS));
                   } else {
                     v = v * a.out_scale;
                   }
@@ -516,10 +551,14 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
 
     // ---------------- staged store: instance-fastest order ----------------
     if constexpr (STAGED) {
-      for (int e = tid; e < Q * L; e += C::NT) {
-        const int q = e % Q, m = e / Q;
-        store_out(q, m, lds_ld(lds, C::lidx(q, m)));
-      }
+      // element e = tid + i*NT is (q, m) = (tid % Q, tid / Q + i*NT/Q): one
+      // per-lane offset plus a wave-uniform one (scalar register) per i
+      static_assert(C::NT % Q == 0 && (Q * L) % C::NT == 0, "staged store covers the tile exactly");
+      constexpr int MSTEP = C::NT / Q;
+      const int q = tid % Q, m0 = tid / Q;
+      const int vo = C::out_off(q, m0, sout);
+      for (int i = 0; i < Q * L / C::NT; ++i)
+        Tr::store_s(gout, vo, i * MSTEP * sout, lds_ld(lds, C::lidx(q, m0 + i * MSTEP)));
     }
     block_barrier();  // LDS is reused by the next tile
   }

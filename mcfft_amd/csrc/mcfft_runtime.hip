@@ -399,6 +399,7 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
       a.in_fmt = h.pp.src_fmt;
       a.inverse = d;
       a.out_scale = h.pp.out_scale;
+      for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < h.pp.shifts.size() ? pow2(-h.pp.shifts[si]) : 1.f;
       a.stage_tw = p->tw[d] + h.pp.stage_tw_off;
       a.stage_tw_entries = (int)((long long)r.tw.size() - h.pp.stage_tw_off);
       a.aops = p->aops[d];
@@ -649,6 +650,7 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
       a.stage_tw_entries = (int)((long long)c.tw.size() - pp.stage_tw_off);
       a.inverse = d;
       a.out_scale = pp.out_scale;
+      for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
       a.stage_tw = plan->tw[d] + pp.stage_tw_off;
       a.aops = plan->aops[d];
       if (pp.fourstep) {

@@ -210,12 +210,15 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       a.groups_per_transform = 1;
       a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
       a.fourstep = true;
-      // direct table W_N^(k1*n2), k1 < N1, n2 < N2 (scale of the last stage folded in)
+      // four-step twiddle W_N^(k1*n2) times the last stage's scale: computed in the
+      // kernel for FP16/BF16, a direct table (k1 < N1, n2 < N2) for split
       const float sl = pow2(-a.shifts.back());
+      a.out_scale = sl;
       a.fs_cols = (int)N2;
       a.fs_tw_off = (long long)p.tw.size();
-      for (long long k1 = 0; k1 < N1; ++k1)
-        for (long long n2 = 0; n2 < N2; ++n2) p.tw.push_back({(k1 * n2) % N, N, sl});
+      if (cm == 2)
+        for (long long k1 = 0; k1 < N1; ++k1)
+          for (long long n2 = 0; n2 < N2; ++n2) p.tw.push_back({(k1 * n2) % N, N, sl});
       for (int s : a.shifts) total_shift += s;
       p.passes.push_back(a);
       // pass B (rows, transposed store): ws -> out

@@ -108,12 +108,15 @@ int build_shape(RaggedPlanCore& r, int rank, const long long* dims, long long ro
     a.pp.tiles_per_group = (int)(N2 / kc.Q);
     a.pp.shifts = add_stage_tables(r.tw, a.pp, S1, T1, r.scaling);
     a.pp.fourstep = true;
-    // direct table W_N^(k1*n2), k1 < N1, n2 < N2 (scale of the last stage folded in)
+    // four-step twiddle W_N^(k1*n2) times the last stage's scale: computed in the
+    // kernel for FP16/BF16, a direct table (k1 < N1, n2 < N2) for split
     const float sl = pow2(-a.pp.shifts.back());
+    a.pp.out_scale = sl;
     a.pp.fs_cols = (int)N2;
     a.pp.fs_tw_off = (long long)r.tw.size();
-    for (long long k1 = 0; k1 < N1; ++k1)
-      for (long long n2 = 0; n2 < N2; ++n2) r.tw.push_back({(k1 * n2) % N, N, sl});
+    if (cm == 2)
+      for (long long k1 = 0; k1 < N1; ++k1)
+        for (long long n2 = 0; n2 < N2; ++n2) r.tw.push_back({(k1 * n2) % N, N, sl});
     for (int s : a.pp.shifts) total += s;
     out.passes.push_back(a);
     RaggedPassHost b;

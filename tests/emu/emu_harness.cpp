@@ -113,6 +113,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       a.stage_tw_entries = (int)((long long)p.tw.size() - pp.stage_tw_off);
       a.inverse = inverse;
       a.out_scale = pp.out_scale;
+      for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
       a.stage_tw = tw.data() + pp.stage_tw_off;
       a.aops = aops.data();
       if (pp.fourstep) {
@@ -182,6 +183,7 @@ extern "C" int emu_ragged_exec(int n_items, const long long* spec, int in_fmt, i
     a.in_fmt = h.pp.src_fmt;
     a.inverse = inverse;
     a.out_scale = h.pp.out_scale;
+    for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < h.pp.shifts.size() ? pow2(-h.pp.shifts[si]) : 1.f;
     a.stage_tw = tw.data() + h.pp.stage_tw_off;
     a.stage_tw_entries = (int)((long long)r.tw.size() - h.pp.stage_tw_off);
     a.aops = aops.data();

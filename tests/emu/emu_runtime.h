@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <mutex>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -114,6 +115,12 @@ inline int grid_size() { return emu::tl_blk->nblocks; }
 inline int uniform(int v) { return v; }
 inline const void* uniform_ptr(const void* p) { return p; }
 inline int opaque(int v) { return v; }
+inline f32x2 cis_rev(float x) {
+  f32x2 r;
+  r.x = (float)std::cos(2.0 * M_PI * (double)x);
+  r.y = (float)std::sin(2.0 * M_PI * (double)x);
+  return r;
+}
 inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
 
 template <class T> inline T lds_ld(const T* p, int i) {
@@ -162,6 +169,7 @@ inline void buf_st64(GBuf b, int off, u32x2 v) { buf_st<u32x2>(b, off, v); }
 inline uint32_t buf_ld32s(GBuf b, int voff, int soff) { return buf_ld<uint32_t>(b, voff + soff); }
 inline u32x4 buf_ld128s(GBuf b, int voff, int soff) { return buf_ld<u32x4>(b, voff + soff); }
 inline void buf_st32s(GBuf b, int voff, int soff, uint32_t v) { buf_st<uint32_t>(b, voff + soff, v); }
+inline void buf_st64s(GBuf b, int voff, int soff, u32x2 v) { buf_st<u32x2>(b, voff + soff, v); }
 
 inline unsigned atomic_add_dev(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
