@@ -49,6 +49,8 @@ MCFFT_DEV f32x2 cis_rev(float x) {
   r.y = __builtin_amdgcn_sinf(x);
   return r;
 }
+// value of lane (lane ^ mask) of the same wave (all 64 lanes take part)
+MCFFT_DEV float lane_xor(float v, int mask) { return __shfl_xor(v, mask, 64); }
 // a copy of v the compiler cannot see through (keeps values derived from it
 // from being hoisted out of a loop)
 MCFFT_DEV int opaque(int v) {

@@ -296,9 +296,12 @@ struct PassCfg {
   static constexpr int LDS_BYTES = LDS_ELEMS * (int)sizeof(El);
   static constexpr int NCOLT = Q * L / 512;          // 32-column MFMA tiles of the tile
   static constexpr int TPW = (NCOLT + NW - 1) / NW;  // MFMA tiles per wave
-  // Passes of two or more stages load the next tile's stage-0 inputs while the
-  // current tile is processed (see fft_pass_body).
-  static constexpr bool PREFETCH = NSTAGE > 1;
+  // L = 512 in the row layout: the 32 columns of the last radix-16 stage are one
+  // instance's, and the radix-2 partners (positions m, m + 256) sit in lanes l
+  // and l ^ 16 of the same MFMA tile, so the tail is done there by a lane
+  // exchange instead of another pass through LDS.
+  static constexpr bool FUSE2 = (S == 2 && T == 2 && !QFAST);
+  static_assert(!FUSE2 || NCOLT % NW == 0, "every wave runs the same number of MFMA tiles");
   static_assert(S >= 1, "at least one radix-16 stage");
   static_assert((Q * L) % 512 == 0, "tile must hold a multiple of 32 sixteen-point columns");
   static_assert(T == 1 || T == 2 || T == 4 || T == 8, "tail radix");
@@ -327,22 +330,6 @@ MCFFT_DEV GBuf clamp_view(const void* base, long long first, long long total, in
   return make_gbuf((const char*)base + first * esz, (unsigned)(rem > 0xffffffffLL ? 0xffffffffLL : rem));
 }
 
-// Input view of a tile.  Column kinds always run full tiles (the planner
-// guarantees it).  A ROW tile's instances are contiguous, so its view ends at
-// the last valid instance: loads beyond read 0.
-template <class C, int KIND>
-MCFFT_DEV GBuf tile_in_view(const PassArgs& a, int tile, int esz_in) {
-  const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
-  const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
-  long long in_end = a.in_elems;
-  if constexpr (KIND == KIND_ROW) {
-    const int inst0 = til * C::Q_;
-    const long long vq = (long long)(a.n_inst - inst0 < C::Q_ ? a.n_inst - inst0 : C::Q_);
-    in_end = ib + vq * C::L < in_end ? ib + vq * C::L : in_end;
-  }
-  return clamp_view(a.in, ib, in_end, esz_in);
-}
-
 // Stage-0 inputs of one tile: MFMA tile k = wave + k*NW, lane inputs r = 8h + t.
 template <class C, class Tr>
 MCFFT_DEV void load_stage0(const PassArgs& a, GBuf gin, int wave, int lane, typename C::El (*x)[8]) {
@@ -361,12 +348,9 @@ MCFFT_DEV void load_stage0(const PassArgs& a, GBuf gin, int wave, int lane, type
 }
 
 // One tile of a pass: stages on the tile's Q instances, ending with a barrier
-// (LDS is reused by the workgroup's next tile).  With PF the tile's stage-0
-// inputs arrive in `pre` (loaded earlier) and, once stage 0 has consumed them,
-// `pre` is refilled with the inputs of tile `next` (when next >= 0).
-template <int S, int T, int Q, int NW, int CM, int KIND, bool PF = false>
-MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
-                        typename PassCfg<S, T, Q, NW, CM, KIND>::El (*pre)[8] = nullptr, int next = -1) {
+// (LDS is reused by the workgroup's next tile).
+template <int S, int T, int Q, int NW, int CM, int KIND>
+MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
@@ -432,12 +416,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
       constexpr int TPW = C::TPW;
       El x[TPW][8];
       if constexpr (P == 0) {
-        if constexpr (PF) {
-          for (int k = 0; k < TPW; ++k)
-            for (int t = 0; t < 8; ++t) x[k][t] = pre[k][t];
-        } else {
-          load_stage0<C, Tr>(a, gin, wave, lane, x);
-        }
+        load_stage0<C, Tr>(a, gin, wave, lane, x);
       } else {
         for (int k = 0; k < TPW; ++k) {
           const int ti = wave + k * NW;
@@ -490,7 +469,17 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
               f32x2 v;
               v.x = y[s];
               v.y = y[8 + s];
-              if constexpr (!LAST) {
+              if constexpr (C::FUSE2 && P == S - 1) {
+                // radix-2 tail: lane j < 16 holds x0 = X[m], lane j + 16 holds
+                // x1 W = X[m + 256] W; outputs x0 + x1 W at m, x0 - x1 W at m + 256
+                const f32x2 w = cmul(v, tw[u]);
+                f32x2 o;
+                o.x = lane_xor(w.x, 16);
+                o.y = lane_xor(w.y, 16);
+                const f32x2 r = (lane & 16) ? o - w : w + o;
+                if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(finish(r, q, mo)));
+                else store_out(q, mo, Tr::pack(finish(r, q, mo)));
+              } else if constexpr (!LAST) {
                 lds_st(lds, C::lidx(q, mo), Tr::pack(cmul(v, tw[u])));
               } else if constexpr (STAGED) {
                 lds_st(lds, C::lidx(q, mo), Tr::pack(finish(v, q, mo)));
@@ -517,17 +506,13 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A,
           }
         }
       }
-      if constexpr (PF && P == 0) {
-        // the next tile's inputs: in flight while this tile's remaining stages run
-        if (next >= 0) load_stage0<C, Tr>(a, tile_in_view<C, KIND>(a, next, esz_in), wave, lane, pre);
-      }
       if constexpr (!LAST || STAGED) block_barrier();
     });
 
     // ---------------- radix-T tail stage (VALU, FP32) ----------------
     // Column (q, j) reads and writes the same T positions j + r*L/T, so with
     // the staged store the results go back in place without a barrier.
-    if constexpr (T > 1) {
+    if constexpr (T > 1 && !C::FUSE2) {
       constexpr int NCOL = Q * L / T;
       constexpr int CPT = (NCOL + C::NT - 1) / C::NT;
       for (int k = 0; k < CPT; ++k) {
@@ -579,16 +564,7 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   if constexpr (XCD_GROUP) {
     if (b < (g & ~31)) b = (b & ~31) + 4 * (b & 7) + ((b >> 3) & 3);
   }
-  typedef PassCfg<S, T, Q, NW, CM, KIND> C;
-  if constexpr (C::PREFETCH) {
-    typename C::El pre[C::TPW][8];
-    const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
-    if (b < a.n_tiles) load_stage0<C, Traits<CM>>(a, tile_in_view<C, KIND>(a, b, esz_in), thread_id() >> 6, lane_id(), pre);
-    for (int tile = b; tile < a.n_tiles; tile += g)
-      fft_tile<S, T, Q, NW, CM, KIND, true>(a, tile, A, pre, tile + g < a.n_tiles ? tile + g : -1);
-  } else {
-    for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
-  }
+  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
 }
 
 }  // namespace mcfft

@@ -59,12 +59,19 @@
   X(2, 2, 16, 8, CM, 1)            \
   X(2, 2, 16, 4, CM, 1)
 
+// 4-byte elements only: 256-byte output segments per row of the transposed store
+#define MCFFT_ROWT_TUNE_WIDE(X, CM) \
+  X(2, 1, 64, 8, CM, 1)             \
+  X(2, 2, 64, 8, CM, 1)             \
+  X(2, 2, 64, 16, CM, 1)
+
 #define MCFFT_ROWT_LIST(X)      \
   MCFFT_ROWT_LIST_CM(X, 0, 16)  \
   MCFFT_ROWT_LIST_CM(X, 1, 16)  \
   MCFFT_ROWT_LIST_CM(X, 2, 8)   \
   MCFFT_ROWT_TUNE_CM(X, 0)      \
-  MCFFT_ROWT_TUNE_CM(X, 2)
+  MCFFT_ROWT_TUNE_CM(X, 2)      \
+  MCFFT_ROWT_TUNE_WIDE(X, 0)
 
 // ---- COL / COLF: Q consecutive columns ----
 #define MCFFT_COL_LIST_CM(X, CM, K, Q512, Q1024, Q2048, Q4096) \

@@ -47,6 +47,7 @@ struct Block {
   std::vector<std::barrier<>*> wbar;
   // per-wave MFMA exchange buffers: [wave][lane][8] for A and B
   std::vector<float> xa, xb;
+  std::vector<float> xs;  // per-wave lane exchange: [wave][lane]
 };
 inline thread_local Block* tl_blk = nullptr;
 inline thread_local int tl_tid = 0;
@@ -70,6 +71,7 @@ inline void launch_concurrent(int nblocks, int nt, size_t lds, const std::functi
     for (int w = 0; w < nt / 64; ++w) blk->wbar.push_back(new std::barrier<>(64));
     blk->xa.assign((size_t)(nt / 64) * 64 * 8, 0.f);
     blk->xb.assign((size_t)(nt / 64) * 64 * 8, 0.f);
+    blk->xs.assign((size_t)nt, 0.f);
     blks.push_back(blk);
   }
   for (int b = 0; b < nblocks; ++b)
@@ -98,6 +100,7 @@ inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>&
     blk.wbar = wb;
     blk.xa.assign((size_t)nw * 64 * 8, 0.f);
     blk.xb.assign((size_t)nw * 64 * 8, 0.f);
+    blk.xs.assign((size_t)nt, 0.f);
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t)
       th.emplace_back([&, t] { tl_blk = &blk; tl_tid = t; body(); });
@@ -115,6 +118,16 @@ inline int grid_size() { return emu::tl_blk->nblocks; }
 inline int uniform(int v) { return v; }
 inline const void* uniform_ptr(const void* p) { return p; }
 inline int opaque(int v) { return v; }
+// value of lane (lane ^ mask) of the same wave (all 64 lanes take part)
+inline float lane_xor(float v, int mask) {
+  emu::Block* b = emu::tl_blk;
+  const int t = emu::tl_tid, w = t / 64, l = t % 64;
+  b->xs[(size_t)w * 64 + l] = v;
+  b->wbar[w]->arrive_and_wait();
+  const float r = b->xs[(size_t)w * 64 + (l ^ mask)];
+  b->wbar[w]->arrive_and_wait();
+  return r;
+}
 inline f32x2 cis_rev(float x) {
   f32x2 r;
   r.x = (float)std::cos(2.0 * M_PI * (double)x);
