@@ -86,16 +86,19 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
         if constexpr (P < 3) {
           // element written at m' = jb + rr * NS is input m' >> 12 of the next
           // stage's column m' & 4095: twiddle W_{256 NS}^((m' >> 12) * (m' % 16NS))
+          // r2 = m' >> 12 is the same for the lane's outputs and m' % 16NS grows
+          // by NS per u: W^E (one sin/cos) times powers of W^(r2 NS).  The last
+          // stage's scale rides on the twiddles of stage 2.
           constexpr int NS = ipow16(P);
           constexpr float INV_M = 1.0f / (float)(256 * NS);
           const int j = low + 256 * k;
           const int mo0 = (j / NS) * NS * 16 + (j % NS) + (4 * h + 8 * g) * NS;
-          const float sc = a.st_scale[P];
+          const int r2 = mo0 >> 12;
+          const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : a.st_scale[P];
+          const f32x2 d = twiddle_rev(r2 * NS, INV_M, a.inverse != 0, 1.0f);
+          tw[0] = twiddle_rev(r2 * (mo0 & (16 * NS - 1)), INV_M, a.inverse != 0, sc);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int mo = mo0 + u * NS;
-            tw[u] = twiddle_rev((mo >> 12) * (mo & (16 * NS - 1)), INV_M, a.inverse != 0, sc);
-          }
+          for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], d);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -106,7 +109,7 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
           if constexpr (P < 3) {
             x[k][s] = Tr::pack(cmul(v, tw[u]));
           } else {
-            buf_st32s(gout, vout, (k * 256 + B::rrc(s) * B::LQ) * 4, Tr::pack(v * a.out_scale));
+            buf_st32s(gout, vout, (k * 256 + B::rrc(s) * B::LQ) * 4, Tr::pack(v));
           }
         }
       }

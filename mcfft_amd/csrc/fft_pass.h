@@ -386,6 +386,15 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
     const GBuf gin = clamp_view(a.in, ib, in_end, esz_in);
     const GBuf gout = clamp_view(a.out, ob, out_end, esz_out);
 
+    // Scale factors.  With computed twiddles the last stage's power-of-two scale
+    // rides on the twiddle the stage before it writes (the stages are linear), so
+    // the outputs need no extra multiply; the split tables carry each stage's own.
+    constexpr bool FOLD = (CM != CM_SPLIT) && C::NSTAGE >= 2;
+    const float last_scale = FOLD ? 1.0f : a.out_scale;
+    auto stage_scale = [&](int P) -> float {  // scale carried by the twiddle stage P writes
+      return (FOLD && P == C::NSTAGE - 2) ? a.st_scale[P] * a.st_scale[P + 1] : a.st_scale[P];
+    };
+
     // output post-processing of the pass's last stage: four-step twiddle or scale
     // four-step twiddle W_N^(m * column), N = L * fs_cols (m < L, column < fs_cols)
     const float inv_n = 1.0f / ((float)L * (float)a.fs_cols);
@@ -393,12 +402,14 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       if constexpr (CM == CM_SPLIT) {
         return __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8));
       } else {
-        return twiddle_rev(m * (inst0 + q), inv_n, inv, a.out_scale);
+        return twiddle_rev(m * (inst0 + q), inv_n, inv, last_scale);
       }
     };
     auto finish = [&](f32x2 v, int q, int m) -> f32x2 {
       if constexpr (FOURSTEP) {
         return cmul(v, fs_twiddle(q, m));
+      } else if constexpr (FOLD) {
+        return v;
       } else {
         return v * a.out_scale;
       }
@@ -438,6 +449,23 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           C::template colmap<16>(ti * 32 + (lane & 31), q, j);
           const f32x16 y = dft16<CM>(x[k], A);
           const int jb = (j / NS) * NS * 16 + (j % NS);
+          // Computed twiddles of the lane's outputs rr = 4h + 8g + u, written at
+          // m = jb + rr*NS: r2 = m >> lg(L/RN) is the same for all eight and
+          // m % 16NS = j % NS + rr*NS, so the exponent is E + u*D with
+          // D = r2*NS: W^E (one sin/cos per g) times powers of W^D.
+          constexpr int RN = (P + 1 < S) ? 16 : T;        // radix of the next stage
+          constexpr int LGQ2 = ilog2c(L / (RN > 1 ? RN : 1));
+          constexpr int NSN = 16 * NS;
+          constexpr float INV_M = 1.0f / (float)(NSN * (RN > 1 ? RN : 1));
+          f32x2 tw_d;
+          int r2 = 0;
+          if constexpr (!LAST && CM != CM_SPLIT) {
+            r2 = (jb + 4 * h * NS) >> LGQ2;
+            tw_d = twiddle_rev(r2 * NS, INV_M, inv, 1.0f);
+          }
+          // four-step twiddles of the last stage: W_N^(m * column), m = mo0 + u*NS
+          f32x2 fs_d;
+          if constexpr (LAST && !STAGED && FOURSTEP && CM != CM_SPLIT) fs_d = twiddle_rev(NS * (inst0 + q), inv_n, inv, 1.0f);
           for (int g = 0; g < 2; ++g) {
             f32x2 tw[4];
             const int mo0 = jb + (4 * h + 8 * g) * NS;
@@ -452,15 +480,8 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
                 tw[2] = t23.xy;
                 tw[3] = t23.zw;
               } else {
-                constexpr int RN = (P + 1 < S) ? 16 : T;      // radix of the next stage
-                constexpr int LGQ2 = ilog2c(L / RN);          // its column count L/RN
-                constexpr int NSN = 16 * NS;
-                constexpr float INV_M = 1.0f / (float)(NSN * RN);
-                const float sc = a.st_scale[P];
-                for (int u = 0; u < 4; ++u) {
-                  const int mo = mo0 + u * NS;
-                  tw[u] = twiddle_rev((mo >> LGQ2) * (mo & (NSN - 1)), INV_M, inv, sc);
-                }
+                tw[0] = twiddle_rev(r2 * (mo0 & (NSN - 1)), INV_M, inv, stage_scale(P));
+                for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], tw_d);
               }
             }
             for (int u = 0; u < 4; ++u) {
@@ -489,14 +510,19 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
               // direct store: offsets and four-step exponents advance linearly in u
               {
                 const int o0 = C::out_off(q, mo0, sout), ostep = C::out_off(0, NS, sout);
+                f32x2 fw;
+                if constexpr (FOURSTEP && CM != CM_SPLIT) fw = fs_twiddle(q, mo0);
                 for (int u = 0; u < 4; ++u) {
                   const int s = 4 * g + u;
                   f32x2 v;
                   v.x = y[s];
                   v.y = y[8 + s];
-                  if constexpr (FOURSTEP) {
+                  if constexpr (FOURSTEP && CM != CM_SPLIT) {
+                    if (u > 0) fw = cmul(fw, fs_d);
+                    v = cmul(v, fw);
+                  } else if constexpr (FOURSTEP) {
                     v = cmul(v, fs_twiddle(q, mo0 + u * NS));
-                  } else {
+                  } else if constexpr (!FOLD) {
                     v = v * a.out_scale;
                   }
                   Tr::store(gout, o0 + u * ostep, Tr::pack(v));
