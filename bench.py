@@ -195,6 +195,32 @@ def main():
                "sample": f"FP64 forward+inverse of {n_tr} of the {BATCH} transforms (N={N}), {reps} repetitions, "
                          f"{reps * tc:.1f} s"}
 
+    # same-run device copy bandwidth (the practical HBM ceiling) and the vendor
+    # library (rocFFT, half precision, same data and step) for comparison
+    ca = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    cb = torch.empty_like(ca)
+    ms_copy = timed(lambda i: cb.copy_(ca), 10, 2)
+    copy_gbs = 2.0 * ca.numel() / (ms_copy * 1e-3) / 1e9
+    del ca, cb
+    vendor = None
+    try:
+        from mcfft_amd import vendor_baseline as vb
+        rf = vb.RocfftPlan([N], BATCH, "half")
+        ri = vb.RocfftPlan([N], BATCH, "half", inverse=True)
+        st = stream.cuda_stream
+
+        def vstep(i):
+            rf.exec(ins[i % n_rot].data_ptr(), mid.data_ptr(), st)
+            ri.exec(mid.data_ptr(), out.data_ptr(), st)
+
+        ms_v = timed(vstep, args.steps, args.warmup)
+        vendor = {"library": "rocFFT", "precision": "half", "ms_per_step": round(ms_v, 4),
+                  "gflops": round(flops_step / (ms_v * 1e-3) / 1e9, 1), "speedup": round(ms_v / ms_plain, 3)}
+        rf.destroy()
+        ri.destroy()
+    except Exception as e:  # noqa: BLE001 -- a missing vendor library is reported, not fatal
+        vendor = {"library": "rocFFT", "error": str(e)}
+
     value = flops_step / (ms_plain * 1e-3) / 1e9
     job_bytes = 2 * 8.0 * elems
     line = {
@@ -214,6 +240,9 @@ def main():
                                "re/im iid N(0,1) seed 2",
                    "n": N, "batch": BATCH, "precision": "plain fp16 (value); split fp16 hi+lo in 'split'"},
         "hbm_frac_job": round(job_bytes / (ms_plain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "gflops_12nlogn": round(value * 12.0 / 5.0, 1),   # the reference's FLOP convention (test_1d.cpp)
+        "copy_gbs": round(copy_gbs, 1),
+        "vendor_baseline": vendor,
         "plain": plain_err,
         "split": split,
         "kernels": kernels,

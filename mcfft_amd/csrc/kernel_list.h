@@ -70,6 +70,7 @@
   MCFFT_ROWT_LIST_CM(X, 1, 16)  \
   MCFFT_ROWT_LIST_CM(X, 2, 8)   \
   MCFFT_ROWT_TUNE_CM(X, 0)      \
+  MCFFT_ROWT_TUNE_CM(X, 1)      \
   MCFFT_ROWT_TUNE_CM(X, 2)      \
   MCFFT_ROWT_TUNE_WIDE(X, 0)
 

@@ -32,9 +32,27 @@ bool split_len(long long L, int& S, int& T) {
 
 enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
 
+// Default tile shapes that measured fastest on MI355X (autotuner sweep over
+// batched 1D/2D/3D plans); other combinations use the first compiled variant.
+struct PreferredShape {
+  int kind, S, T, cm, Q, nw;
+};
+static const PreferredShape kPreferred[] = {
+    {KIND_ROWT, 2, 2, 0, 16, 8}, {KIND_ROWT, 2, 2, 1, 16, 8}, {KIND_ROWT, 2, 2, 2, 16, 8},
+    {KIND_ROW, 2, 2, 0, 8, 8},   {KIND_ROW, 2, 2, 1, 8, 8},
+    {KIND_COL, 2, 1, 0, 64, 8},  {KIND_COL, 2, 1, 1, 64, 8},
+};
+
 static const KernelInfo* find_kernel(int S, int T, int cm, int kind) {
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  for (const PreferredShape& p : kPreferred) {
+    if (p.kind != kind || p.S != S || p.T != T || p.cm != cm) continue;
+    for (int i = 0; i < counts[kind]; ++i) {
+      const KernelInfo& k = lists[kind][i];
+      if (k.S == S && k.T == T && k.cm == cm && k.Q == p.Q && k.nw == p.nw) return &k;
+    }
+  }
   for (int i = 0; i < counts[kind]; ++i) {
     const KernelInfo& k = lists[kind][i];
     if (k.S == S && k.T == T && k.cm == cm) return &k;
