@@ -129,17 +129,32 @@ def main():
     plain_err = {"rel_l2_fwd": oracle.rel_l2(y_f, ref_f), "max_abs_fwd": oracle.max_abs(y_f, ref_f),
                  "rel_l2_roundtrip": oracle.rel_l2(y_rt, xq[:k]), "checked_transforms": 8}
 
-    # per-kernel durations of the plain path (HIP events on the launch stream)
+    # per-kernel durations of the plain path: the same forward+inverse steps, one
+    # launch at a time between HIP events on the launch stream, averaged per
+    # kernel over its forward and inverse launches (as rocprofv3 --stats does)
     n_pass = pf.pass_count()
     desc = pf.describe().strip().splitlines()
+    names = [desc[1 + p_i].split()[1] for p_i in range(n_pass)]
+    launches = []   # (kernel name, start event, end event)
+    for i in range(args.warmup + args.steps):
+        for src, dst, d in ((ins[i % n_rot], mid, mc.FORWARD), (mid, out, mc.INVERSE)):
+            for p_i in range(n_pass):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                pf.exec_passes(src, dst, d, p_i, 1)
+                e1.record(stream)
+                if i >= args.warmup:
+                    launches.append((names[p_i], e0, e1))
+    torch.cuda.synchronize()
+    per = {}
+    for name, e0, e1 in launches:
+        per.setdefault(name, []).append(e0.elapsed_time(e1))
     kernels = []
-    for p_i in range(n_pass):
-        def one(i, p_i=p_i):
-            pf.exec_passes(ins[i % n_rot], mid, mc.FORWARD, p_i, 1)
-        ms_k = timed(one, args.steps, 2)
-        name = desc[1 + p_i].split()[1]
-        alg_bytes = 8.0 * elems
-        kernels.append({"pass": p_i, "kernel": name, "ms": ms_k, "alg_gbs": alg_bytes / (ms_k * 1e-3) / 1e9})
+    for p_i, name in enumerate(names):
+        ms_k = sum(per[name]) / len(per[name])
+        kernels.append({"pass": p_i, "kernel": name, "ms": ms_k, "launches": len(per[name]),
+                        "alg_gbs": 8.0 * elems / (ms_k * 1e-3) / 1e9})
     dom = max(kernels, key=lambda r: r["ms"])
     traffic = traffic_from_profiles(dom["kernel"])
     roofline = {"bound": "hbm", "achieved": round(dom["alg_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -25,9 +25,9 @@
  * roughly constant (E ~ log2(N)/2), FULL divides by N (E = log2 N; never
  * overflows for inputs that fit the format).
  *
- * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D sides
- * 16..4096 (the number of columns an outer axis has must be a multiple of the
- * column tile, 32 or less).
+ * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D outer sides
+ * 16..4096 and innermost side up to 2^15 (2^16 for FP16/BF16) (the number of
+ * columns an outer axis has must be a multiple of the column tile, 64 or less).
  *
  * Reference interface replaced: mcfftCreate / mcfftExec of
  * include/mcfft_1d_device.h:37-38, include/mcfft_2d_device.h:37-38 and
@@ -72,8 +72,10 @@ typedef enum mcfftScaling_t {
 typedef struct mcfftPlan_st* mcfftPlan;
 
 /* Create a plan for `batch` transforms of shape dims[0..rank-1] (rank 1..3).
- * Allocates the plan's twiddle tables (generated on the GPU in FP64) and, for
- * 1D lengths above 4096, a workspace; both live on the current device.
+ * Allocates the plan's tables (DFT matrices and, for split precision, the
+ * twiddles; generated on the GPU in FP64) and, for 1D lengths above the
+ * single-pass limit (2^15; 2^16 for FP16/BF16), a workspace; both live on the
+ * current device.  FP16/BF16 passes evaluate their twiddles in the kernel.
  * Replaces mcfftCreate (source/mcfft_1d_device.cpp:680-790). */
 mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
                             mcfftFormat out_fmt, mcfftScaling scaling);
@@ -92,9 +94,10 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
 
 /* Ragged batch: n_items transforms of any supported shapes (rank 1..3; 1D
  * lengths 16..2^18, sides 16..4096), each at its own element offset in the
- * exec input and output buffers, run by ONE persistent kernel launch over a
- * device-side work queue (plus a tiny queue-reset kernel), so the whole batch is
- * a single graph-capturable exec.  Back-to-back 1D items of one length up to
+ * exec input and output buffers, run by one persistent kernel launch per pass
+ * level (at most three: every item's first passes, then second, then third)
+ * over a device-side work queue, plus a tiny queue-reset kernel; the whole batch
+ * is one graph-capturable exec.  Back-to-back 1D items of one length up to
  * 4096 are processed as one multi-row unit.  Items may have different output
  * exponents: see mcfft_get_item_exponents.  No reference counterpart. */
 typedef struct mcfftRaggedItem_t {
