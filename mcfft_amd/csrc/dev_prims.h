@@ -49,8 +49,11 @@ MCFFT_DEV f32x2 cis_rev(float x) {
   r.y = __builtin_amdgcn_sinf(x);
   return r;
 }
-// value of lane (lane ^ mask) of the same wave (all 64 lanes take part)
-MCFFT_DEV float lane_xor(float v, int mask) { return __shfl_xor(v, mask, 64); }
+// value of lane (lane ^ 16) of the same wave (ds_swizzle bit mode: and 0x1f,
+// xor 0x10 within each 32-lane half; all 64 lanes take part)
+MCFFT_DEV float lane_xor16(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401f));
+}
 // a copy of v the compiler cannot see through (keeps values derived from it
 // from being hoisted out of a loop)
 MCFFT_DEV int opaque(int v) {

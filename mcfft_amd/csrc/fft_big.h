@@ -92,7 +92,7 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
           constexpr int NS = ipow16(P);
           constexpr float INV_M = 1.0f / (float)(256 * NS);
           const int j = low + 256 * k;
-          const int mo0 = (j / NS) * NS * 16 + (j % NS) + (4 * h + 8 * g) * NS;
+          const int mo0 = (int)(((unsigned)j / NS) * NS * 16 + ((unsigned)j % NS)) + (4 * h + 8 * g) * NS;
           const int r2 = mo0 >> 12;
           const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : a.st_scale[P];
           const f32x2 d = twiddle_rev(r2 * NS, INV_M, a.inverse != 0, 1.0f);

@@ -311,9 +311,10 @@ struct PassCfg {
     else return q * ROW + m + (m >> 4);
   }
   template <int R>
-  static MCFFT_DEV void colmap(int c, int& q, int& j) {
-    if constexpr (QFAST) { q = c % Q; j = c / Q; }
-    else { q = c / (L / R); j = c % (L / R); }
+  static MCFFT_DEV void colmap(int c, int& q, int& j) {  // c >= 0: unsigned division is a shift
+    const unsigned u = (unsigned)c;
+    if constexpr (QFAST) { q = (int)(u % (unsigned)Q); j = (int)(u / (unsigned)Q); }
+    else { q = (int)(u / (unsigned)(L / R)); j = (int)(u % (unsigned)(L / R)); }
   }
   static MCFFT_DEV int in_off(int q, int m, int stride) {
     if constexpr (QFAST) return q + m * stride;
@@ -425,6 +426,20 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       constexpr bool LAST = (P == C::NSTAGE - 1);
       constexpr int NCOLT = C::NCOLT;
       constexpr int TPW = C::TPW;
+      // Fused radix-2 tail with the partners in one lane: in this (last radix-16)
+      // stage MFMA tiles 2i and 2i+1 of a wave take columns 0..15 and 16..31 of the
+      // same two instances (lanes 0-15: instance 2p, 16-31: 2p+1), so the outputs
+      // at m and m + 256 meet in the same thread.
+      constexpr bool PAIR = C::FUSE2 && P == S - 1 && TPW % 2 == 0;
+      auto stage_col = [&](int k, int& q, int& j) {
+        if constexpr (PAIR) {
+          const int pidx = wave + (k >> 1) * NW, c5 = lane & 31;
+          q = 2 * pidx + (c5 >> 4);
+          j = (c5 & 15) + 16 * (k & 1);
+        } else {
+          C::template colmap<16>((wave + k * NW) * 32 + (lane & 31), q, j);
+        }
+      };
       El x[TPW][8];
       if constexpr (P == 0) {
         load_stage0<C, Tr>(a, gin, wave, lane, x);
@@ -433,7 +448,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           const int ti = wave + k * NW;
           if (ti < NCOLT) {
             int q, j;
-            C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+            stage_col(k, q, j);
             for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, C::lidx(q, j + (8 * h + t) * (L / 16)));
           }
         }
@@ -442,13 +457,61 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       // (L = 16) reads and writes global memory in one stage: the barrier keeps an
       // in-place pass from overwriting elements another wave has not read yet.
       if constexpr (P > 0 || (LAST && !STAGED)) block_barrier();
+      if constexpr (PAIR) {
+        for (int k = 0; k < TPW; k += 2) {
+          int q, j0;
+          stage_col(k, q, j0);                            // j0 < 16; partner column j0 + 16
+          const f32x16 y0 = dft16<CM>(x[k], A);
+          const f32x16 y1 = dft16<CM>(x[k + 1], A);
+          // outputs of column j0 land at m = j0 + 16 rr < 256 (tail input 0, twiddle 1),
+          // those of j0 + 16 at m + 256 (tail input 1, twiddle W_512^m)
+          const float sc = stage_scale(P);
+          f32x2 d;
+          if constexpr (CM != CM_SPLIT) d = twiddle_rev(16, 1.0f / 512.0f, inv, 1.0f);
+          for (int g = 0; g < 2; ++g) {
+            const int m0 = j0 + (4 * h + 8 * g) * 16;
+            f32x2 tw[4];
+            if constexpr (CM == CM_SPLIT) {
+              const int toff = (P * L + (j0 + 16) * 16 + 4 * h + 8 * g) * 8;
+              const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
+              const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
+              tw[0] = t01.xy;
+              tw[1] = t01.zw;
+              tw[2] = t23.xy;
+              tw[3] = t23.zw;
+            } else {
+              tw[0] = twiddle_rev(m0, 1.0f / 512.0f, inv, sc);
+              for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], d);
+            }
+            for (int u = 0; u < 4; ++u) {
+              const int s = 4 * g + u, m = m0 + 16 * u;
+              f32x2 v0, v1;
+              v0.x = y0[s];
+              v0.y = y0[8 + s];
+              v1.x = y1[s];
+              v1.y = y1[8 + s];
+              // split tables carry the scale in both entries (W^0 * scale for column j0)
+              const f32x2 w0 = (CM == CM_SPLIT) ? v0 * a.st_scale[P] : v0 * sc;
+              const f32x2 w1 = cmul(v1, tw[u]);
+              const f32x2 lo = finish(w0 + w1, q, m), hi = finish(w0 - w1, q, m + 256);
+              if constexpr (STAGED) {
+                lds_st(lds, C::lidx(q, m), Tr::pack(lo));
+                lds_st(lds, C::lidx(q, m + 256), Tr::pack(hi));
+              } else {
+                store_out(q, m, Tr::pack(lo));
+                store_out(q, m + 256, Tr::pack(hi));
+              }
+            }
+          }
+        }
+      } else
       for (int k = 0; k < TPW; ++k) {
         const int ti = wave + k * NW;
         if (ti < NCOLT) {
           int q, j;
-          C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+          stage_col(k, q, j);
           const f32x16 y = dft16<CM>(x[k], A);
-          const int jb = (j / NS) * NS * 16 + (j % NS);
+          const int jb = (int)(((unsigned)j / NS) * NS * 16 + ((unsigned)j % NS));
           // Computed twiddles of the lane's outputs rr = 4h + 8g + u, written at
           // m = jb + rr*NS: r2 = m >> lg(L/RN) is the same for all eight and
           // m % 16NS = j % NS + rr*NS, so the exponent is E + u*D with
@@ -495,9 +558,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
                 // x1 W = X[m + 256] W; outputs x0 + x1 W at m, x0 - x1 W at m + 256
                 const f32x2 w = cmul(v, tw[u]);
                 f32x2 o;
-                o.x = lane_xor(w.x, 16);
-                o.y = lane_xor(w.y, 16);
-                const f32x2 r = (lane & 16) ? o - w : w + o;
+                o.x = lane_xor16(w.x);
+                o.y = lane_xor16(w.y);
+                const float sg = (lane & 16) ? -1.0f : 1.0f;
+                f32x2 r;
+                r.x = __builtin_fmaf(sg, w.x, o.x);
+                r.y = __builtin_fmaf(sg, w.y, o.y);
                 if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(finish(r, q, mo)));
                 else store_out(q, mo, Tr::pack(finish(r, q, mo)));
               } else if constexpr (!LAST) {
@@ -566,7 +632,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       // per-lane offset plus a wave-uniform one (scalar register) per i
       static_assert(C::NT % Q == 0 && (Q * L) % C::NT == 0, "staged store covers the tile exactly");
       constexpr int MSTEP = C::NT / Q;
-      const int q = tid % Q, m0 = tid / Q;
+      const int q = (int)((unsigned)tid % Q), m0 = (int)((unsigned)tid / Q);
       const int vo = C::out_off(q, m0, sout);
       for (int i = 0; i < Q * L / C::NT; ++i)
         Tr::store_s(gout, vo, i * MSTEP * sout, lds_ld(lds, C::lidx(q, m0 + i * MSTEP)));

@@ -118,6 +118,9 @@ inline int grid_size() { return emu::tl_blk->nblocks; }
 inline int uniform(int v) { return v; }
 inline const void* uniform_ptr(const void* p) { return p; }
 inline int opaque(int v) { return v; }
+inline float lane_xor(float v, int mask);
+// value of lane (lane ^ 16) of the same wave (all 64 lanes take part)
+inline float lane_xor16(float v) { return lane_xor(v, 16); }
 // value of lane (lane ^ mask) of the same wave (all 64 lanes take part)
 inline float lane_xor(float v, int mask) {
   emu::Block* b = emu::tl_blk;
