@@ -296,12 +296,14 @@ struct PassCfg {
   static constexpr int LDS_BYTES = LDS_ELEMS * (int)sizeof(El);
   static constexpr int NCOLT = Q * L / 512;          // 32-column MFMA tiles of the tile
   static constexpr int TPW = (NCOLT + NW - 1) / NW;  // MFMA tiles per wave
-  // L = 512 in the row layout: the 32 columns of the last radix-16 stage are one
-  // instance's, and the radix-2 partners (positions m, m + 256) sit in lanes l
-  // and l ^ 16 of the same MFMA tile, so the tail is done there by a lane
-  // exchange instead of another pass through LDS.
-  static constexpr bool FUSE2 = (S == 2 && T == 2 && !QFAST);
-  static_assert(!FUSE2 || NCOLT % NW == 0, "every wave runs the same number of MFMA tiles");
+  // L = 512: the radix-2 tail is done right after the last radix-16 stage
+  // instead of in another pass through LDS.  Its partners (positions m and
+  // m + 256) come from columns j and j + 16 of that stage, which the stage
+  // places in the same lane of two MFMA tiles of one wave (needs an even number
+  // of tiles per wave); in the row layout with an odd number, in lanes l and
+  // l ^ 16 of one tile (exchanged with ds_swizzle).
+  static constexpr bool FUSE2 =
+      S == 2 && T == 2 && NCOLT % NW == 0 && (!QFAST || (NCOLT / NW) % 2 == 0);
   static_assert(S >= 1, "at least one radix-16 stage");
   static_assert((Q * L) % 512 == 0, "tile must hold a multiple of 32 sixteen-point columns");
   static_assert(T == 1 || T == 2 || T == 4 || T == 8, "tail radix");
@@ -427,15 +429,21 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       constexpr int NCOLT = C::NCOLT;
       constexpr int TPW = C::TPW;
       // Fused radix-2 tail with the partners in one lane: in this (last radix-16)
-      // stage MFMA tiles 2i and 2i+1 of a wave take columns 0..15 and 16..31 of the
-      // same two instances (lanes 0-15: instance 2p, 16-31: 2p+1), so the outputs
-      // at m and m + 256 meet in the same thread.
+      // stage MFMA tiles 2i and 2i+1 of a wave take the same 32 (instance, column)
+      // slots with column j < 16 and j + 16 (row layout: lanes 0-15 instance 2p,
+      // 16-31 instance 2p+1; column layout: 32 instances of one column), so the
+      // outputs at m and m + 256 meet in the same thread.
       constexpr bool PAIR = C::FUSE2 && P == S - 1 && TPW % 2 == 0;
       auto stage_col = [&](int k, int& q, int& j) {
         if constexpr (PAIR) {
-          const int pidx = wave + (k >> 1) * NW, c5 = lane & 31;
-          q = 2 * pidx + (c5 >> 4);
-          j = (c5 & 15) + 16 * (k & 1);
+          const unsigned cp = (unsigned)((wave + (k >> 1) * NW) * 32 + (lane & 31));
+          if constexpr (C::QFAST) {
+            q = (int)(cp % (unsigned)Q);
+            j = (int)(cp / (unsigned)Q) + 16 * (k & 1);
+          } else {
+            q = (int)(cp >> 4);
+            j = (int)(cp & 15u) + 16 * (k & 1);
+          }
         } else {
           C::template colmap<16>((wave + k * NW) * 32 + (lane & 31), q, j);
         }
@@ -466,23 +474,18 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           // outputs of column j0 land at m = j0 + 16 rr < 256 (tail input 0, twiddle 1),
           // those of j0 + 16 at m + 256 (tail input 1, twiddle W_512^m)
           const float sc = stage_scale(P);
-          f32x2 d;
+          f32x2 d;  // W_512^16; split: table entry (column 16, rr = 1) without its scale
           if constexpr (CM != CM_SPLIT) d = twiddle_rev(16, 1.0f / 512.0f, inv, 1.0f);
+          else d = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + 16 * 16 + 1) * 8)) * (1.0f / a.st_scale[P]);
           for (int g = 0; g < 2; ++g) {
             const int m0 = j0 + (4 * h + 8 * g) * 16;
             f32x2 tw[4];
             if constexpr (CM == CM_SPLIT) {
-              const int toff = (P * L + (j0 + 16) * 16 + 4 * h + 8 * g) * 8;
-              const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
-              const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
-              tw[0] = t01.xy;
-              tw[1] = t01.zw;
-              tw[2] = t23.xy;
-              tw[3] = t23.zw;
+              tw[0] = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + (j0 + 16) * 16 + 4 * h + 8 * g) * 8));
             } else {
               tw[0] = twiddle_rev(m0, 1.0f / 512.0f, inv, sc);
-              for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], d);
             }
+            for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], d);
             for (int u = 0; u < 4; ++u) {
               const int s = 4 * g + u, m = m0 + 16 * u;
               f32x2 v0, v1;
@@ -522,30 +525,34 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           constexpr float INV_M = 1.0f / (float)(NSN * (RN > 1 ? RN : 1));
           f32x2 tw_d;
           int r2 = 0;
-          if constexpr (!LAST && CM != CM_SPLIT) {
+          if constexpr (!LAST) {
             r2 = (jb + 4 * h * NS) >> LGQ2;
-            tw_d = twiddle_rev(r2 * NS, INV_M, inv, 1.0f);
+            if constexpr (CM != CM_SPLIT) {
+              tw_d = twiddle_rev(r2 * NS, INV_M, inv, 1.0f);
+            } else {
+              // split: W^D is the table entry (column r2 * L/(16 RN), rr = 1), whose
+              // column has j % NS = 0 and the same r2; the table carries the scale
+              constexpr int JD = L / (16 * (RN > 1 ? RN : 1));
+              tw_d = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + r2 * JD * 16 + 1) * 8)) * (1.0f / a.st_scale[P]);
+            }
           }
           // four-step twiddles of the last stage: W_N^(m * column), m = mo0 + u*NS
           f32x2 fs_d;
-          if constexpr (LAST && !STAGED && FOURSTEP && CM != CM_SPLIT) fs_d = twiddle_rev(NS * (inst0 + q), inv_n, inv, 1.0f);
+          if constexpr (LAST && !STAGED && FOURSTEP) {
+            if constexpr (CM != CM_SPLIT) fs_d = twiddle_rev(NS * (inst0 + q), inv_n, inv, 1.0f);
+            else fs_d = fs_twiddle(q, NS) * (1.0f / a.out_scale);
+          }
           for (int g = 0; g < 2; ++g) {
             f32x2 tw[4];
             const int mo0 = jb + (4 * h + 8 * g) * NS;
             if constexpr (!LAST) {
               if constexpr (CM == CM_SPLIT) {
-                // table entries rr = 4h + 8g + 0..3 of row j: two 16-byte loads
-                const int toff = (P * L + j * 16 + 4 * h + 8 * g) * 8;
-                const f32x4 t01 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff));
-                const f32x4 t23 = __builtin_bit_cast(f32x4, buf_ld128(gtw, toff + 16));
-                tw[0] = t01.xy;
-                tw[1] = t01.zw;
-                tw[2] = t23.xy;
-                tw[3] = t23.zw;
+                // W^E: table entry rr = 4h + 8g of row j (FP64-generated, scale folded in)
+                tw[0] = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + j * 16 + 4 * h + 8 * g) * 8));
               } else {
                 tw[0] = twiddle_rev(r2 * (mo0 & (NSN - 1)), INV_M, inv, stage_scale(P));
-                for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], tw_d);
               }
+              for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], tw_d);
             }
             for (int u = 0; u < 4; ++u) {
               const int s = 4 * g + u;
@@ -577,17 +584,15 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
               {
                 const int o0 = C::out_off(q, mo0, sout), ostep = C::out_off(0, NS, sout);
                 f32x2 fw;
-                if constexpr (FOURSTEP && CM != CM_SPLIT) fw = fs_twiddle(q, mo0);
+                if constexpr (FOURSTEP) fw = fs_twiddle(q, mo0);
                 for (int u = 0; u < 4; ++u) {
                   const int s = 4 * g + u;
                   f32x2 v;
                   v.x = y[s];
                   v.y = y[8 + s];
-                  if constexpr (FOURSTEP && CM != CM_SPLIT) {
+                  if constexpr (FOURSTEP) {
                     if (u > 0) fw = cmul(fw, fs_d);
                     v = cmul(v, fw);
-                  } else if constexpr (FOURSTEP) {
-                    v = cmul(v, fs_twiddle(q, mo0 + u * NS));
                   } else if constexpr (!FOLD) {
                     v = v * a.out_scale;
                   }

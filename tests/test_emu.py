@@ -57,7 +57,7 @@ def test_four_step(n, batch, ws, fi, fo):
 
 
 @pytest.mark.parametrize("dims,batch", [((64, 32), 2), ((16, 256), 1), ((256, 64), 1), ((16, 16, 32), 1),
-                                        ((32, 64, 32), 1)])
+                                        ((32, 64, 32), 1), ((512, 64), 1), ((512, 16, 32), 1)])
 def test_multi_dim(dims, batch):
     check(dims, batch, emu.FMT_F16, emu.FMT_F16)
     check(dims, batch, emu.FMT_F16, emu.FMT_SPLIT)
