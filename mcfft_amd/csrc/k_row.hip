@@ -5,14 +5,18 @@
 
 MCFFT_ROW_LIST(MCFFT_DEFINE_KERNEL)
 // single-pass 65536 points (one workgroup per transform, two-round LDS exchange)
-MCFFT_KERNEL(512) void mcfft_pass_big_0(mcfft::PassArgs a) { mcfft::fft_big_body<0>(a); }
-MCFFT_KERNEL(512) void mcfft_pass_big_1(mcfft::PassArgs a) { mcfft::fft_big_body<1>(a); }
+MCFFT_KERNEL(512) void mcfft_pass_big_0(mcfft::PassArgs a) { mcfft::fft_big_body<0, 8>(a); }
+MCFFT_KERNEL(512) void mcfft_pass_big_1(mcfft::PassArgs a) { mcfft::fft_big_body<1, 8>(a); }
+MCFFT_KERNEL(1024) void mcfft_pass_big16_0(mcfft::PassArgs a) { mcfft::fft_big_body<0, 16>(a); }
+MCFFT_KERNEL(1024) void mcfft_pass_big16_1(mcfft::PassArgs a) { mcfft::fft_big_body<1, 16>(a); }
 
 namespace mcfft {
 const KernelInfo kRowKernels[] = {
     MCFFT_ROW_LIST(MCFFT_TABLE_ENTRY)
-    {"mcfft_pass_big_0", (const void*)&mcfft_pass_big_0, 4, 1, 1, 8, 0, 0, BigCfg::LDS_BYTES},
-    {"mcfft_pass_big_1", (const void*)&mcfft_pass_big_1, 4, 1, 1, 8, 1, 0, BigCfg::LDS_BYTES},
+    {"mcfft_pass_big_0", (const void*)&mcfft_pass_big_0, 4, 1, 1, 8, 0, 0, BigCfg<8>::LDS_BYTES},
+    {"mcfft_pass_big_1", (const void*)&mcfft_pass_big_1, 4, 1, 1, 8, 1, 0, BigCfg<8>::LDS_BYTES},
+    {"mcfft_pass_big16_0", (const void*)&mcfft_pass_big16_0, 4, 1, 1, 16, 0, 0, BigCfg<16>::LDS_BYTES},
+    {"mcfft_pass_big16_1", (const void*)&mcfft_pass_big16_1, 4, 1, 1, 16, 1, 0, BigCfg<16>::LDS_BYTES},
 };
 const int kRowKernelCount = (int)(sizeof(kRowKernels) / sizeof(kRowKernels[0]));
 }  // namespace mcfft

@@ -151,3 +151,17 @@ def ragged_check(shapes, in_fmt, out_fmt, scaling=1):
         raise RuntimeError(f"ragged_build status {-bad}")
     keys = ("tasks", "passes", "phases", "counters", "ws_elems", "tw_entries")
     return int(bad), {k: int(v) for k, v in zip(keys, st)}
+
+
+def set_kernel_override(name: str | None):
+    """Run `name` for every pass it can serve (None: the planner's choice)."""
+    L = lib()
+    L.emu_set_kernel_override.argtypes = [ctypes.c_char_p]
+    L.emu_set_kernel_override((name or "").encode())
+
+
+def last_kernels() -> str:
+    """Kernel names of the passes of the last exec_fft call."""
+    L = lib()
+    L.emu_last_kernels.restype = ctypes.c_char_p
+    return L.emu_last_kernels().decode()

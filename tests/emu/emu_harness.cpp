@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "fft_pass.h"
@@ -62,11 +63,23 @@ extern "C" int emu_plan_describe(int rank, const long long* dims, long long batc
 }
 
 // Run a whole plan on host buffers.  ws_limit bounds the four-step workspace.
+// Kernel override for tests: passes whose candidates include a kernel of this
+// name run that kernel instead of the planner's default (empty: no override).
+static std::string g_kernel_override, g_last_kernels;
+extern "C" void emu_set_kernel_override(const char* name) { g_kernel_override = name ? name : ""; }
+extern "C" const char* emu_last_kernels() { return g_last_kernels.c_str(); }
+
 extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
                         int inverse, const void* in, void* out, long long ws_limit, int* out_exponent) {
   PlanCore p;
   int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit);
   if (st) return st;
+  if (!g_kernel_override.empty())
+    for (PassPlan& pp : p.passes)
+      for (const KernelInfo* k : pass_candidates(pp, p.cm))
+        if (g_kernel_override == k->name) set_pass_kernel(pp, k);
+  g_last_kernels.clear();
+  for (const PassPlan& pp : p.passes) g_last_kernels += std::string(pp.k->name) + " ";
   if (out_exponent) *out_exponent = p.out_exponent;
   const int sign = inverse ? 1 : -1;
   std::vector<f32x2> tw(p.tw.size() + 1);
