@@ -222,6 +222,20 @@ void apply_cached(mcfftPlan p) {
   }
 }
 
+// MCFFT_FORCE_KERNEL=<kernel name>: every pass of a new plan that can run that
+// kernel runs it (pins a variant for testing and benchmarking; overrides the
+// tuning cache)
+void apply_forced(mcfftPlan p) {
+  const char* env = getenv("MCFFT_FORCE_KERNEL");
+  if (!env || !*env) return;
+  for (PassPlan& pp : p->core.passes)
+    for (const KernelInfo* k : pass_candidates(pp, p->core.cm))
+      if (std::string(env) == k->name) {
+        PassPlan trial = pp;
+        if (set_pass_kernel(trial, k)) pp = trial;
+      }
+}
+
 void free_plan(mcfftPlan p) {
   if (!p) return;
   for (int d = 0; d < 2; ++d) {
@@ -293,6 +307,7 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
   }
   p->arch = arch_of(p->device);
   apply_cached(p);
+  apply_forced(p);
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);
     int per_cu = 0;

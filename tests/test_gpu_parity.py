@@ -62,7 +62,8 @@ def test_1d_forward(torch_cuda, n, batch, fmt):
 
 
 @pytest.mark.parametrize("dims,batch", [((64, 32), 3), ((256, 256), 2), ((1024, 512), 1), ((4096, 4096), 1),
-                                        ((16, 16, 32), 2), ((64, 32, 128), 1), ((128, 128, 128), 1)])
+                                        ((512, 256), 2), ((16, 65536), 1), ((16, 16, 32), 2), ((64, 32, 128), 1),
+                                        ((128, 128, 128), 1), ((512, 16, 64), 1)])
 @pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
 def test_nd_forward(torch_cuda, dims, batch, fmt):
     in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else (fmt, fmt)
@@ -142,3 +143,20 @@ def test_gpu_error_norms_match_host(torch_cuda, fmt):
     y = mc.decode(mc.encode(x, fmt), fmt, 3)
     assert abs(r - oracle.rel_l2(y, ref)) <= 1e-12 * max(1.0, r)
     assert abs(m - oracle.max_abs(y, ref)) <= 1e-12 * max(1.0, m)
+
+
+@pytest.mark.parametrize("kernel,fmt", [("mcfft_pass_big_0", "fp16"), ("mcfft_pass_big16_0", "fp16"),
+                                        ("mcfft_pass_big16_1", "bf16")])
+def test_single_pass_65536_variants(torch_cuda, kernel, fmt, monkeypatch):
+    """Both wave counts of the single-pass 65536 kernel (pinned with
+    MCFFT_FORCE_KERNEL), against the oracle."""
+    monkeypatch.setenv("MCFFT_FORCE_KERNEL", kernel)
+    n, batch = 65536, 3
+    x = _input(n * batch, seed=11)
+    p = mc.Plan([n], batch, fmt, fmt)
+    assert kernel in p.describe()
+    p.destroy()
+    y = run([n], batch, fmt, fmt, x)
+    err = oracle.rel_l2(y, oracle.fft(_quantised(x, fmt), (n,)))
+    print(f"65536 {kernel} rel_l2={err:.3e}")
+    assert err <= TOL[fmt]
