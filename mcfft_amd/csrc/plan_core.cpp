@@ -41,6 +41,7 @@ static const PreferredShape kPreferred[] = {
     {KIND_ROWT, 2, 2, 0, 16, 8}, {KIND_ROWT, 2, 2, 1, 16, 8}, {KIND_ROWT, 2, 2, 2, 16, 8},
     {KIND_ROW, 2, 2, 0, 8, 8},   {KIND_ROW, 2, 2, 1, 8, 8},
     {KIND_COL, 2, 1, 0, 64, 8},  {KIND_COL, 2, 1, 1, 64, 8},
+    {KIND_ROW, 4, 1, 0, 1, 16},  {KIND_ROW, 4, 1, 1, 1, 16},   // single-pass 65536: 16 waves
 };
 
 static const KernelInfo* find_kernel(int S, int T, int cm, int kind) {

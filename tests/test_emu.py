@@ -41,20 +41,20 @@ def test_single_pass_32768():
 def test_single_pass_65536(fi, inverse):
     """65536 points in one workgroup (registers + two-round LDS exchange); 5
     transforms on 3 workgroups, so two workgroups prefetch a second transform."""
-    assert "mcfft_pass_big_" in emu.describe([65536], 5, fi, fi)
+    assert "mcfft_pass_big" in emu.describe([65536], 5, fi, fi)
     check([65536], 5, fi, fi, inverse=inverse)
 
 
-def test_single_pass_65536_16_waves():
-    """The 16-wave variant (64 elements per lane, round taken from bit 0 of the tile)."""
-    emu.set_kernel_override("mcfft_pass_big16_0")
+def test_single_pass_65536_8_waves():
+    """The 8-wave variant (128 elements per lane, round taken from bit 1 of the tile)."""
+    emu.set_kernel_override("mcfft_pass_big_0")
     try:
         check([65536], 5, emu.FMT_F16, emu.FMT_F16)   # 3 workgroups: two prefetch a second transform
-        assert emu.last_kernels().split() == ["mcfft_pass_big16_0"]
+        assert emu.last_kernels().split() == ["mcfft_pass_big_0"]
         check([65536], 3, emu.FMT_F16, emu.FMT_F16, inverse=True)
     finally:
         emu.set_kernel_override(None)
-    emu.set_kernel_override("mcfft_pass_big16_1")
+    emu.set_kernel_override("mcfft_pass_big_1")
     try:
         check([65536], 2, emu.FMT_BF16, emu.FMT_BF16)
     finally:
@@ -62,7 +62,7 @@ def test_single_pass_65536_16_waves():
 
 
 def test_rows_65536_in_2d():
-    assert "mcfft_pass_big_1" in emu.describe([16, 65536], 1, emu.FMT_BF16, emu.FMT_BF16)
+    assert "mcfft_pass_big16_1" in emu.describe([16, 65536], 1, emu.FMT_BF16, emu.FMT_BF16)
     check([16, 65536], 1, emu.FMT_BF16, emu.FMT_BF16)
 
 
