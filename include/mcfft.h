@@ -92,6 +92,23 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
  * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
 
+/* Real-input forward (R2C) and real-output inverse (C2R) 1D transforms; no
+ * reference counterpart (the reference has only the forward C2C).
+ * R2C: `batch` real sequences of length n (n a power of two, 32 <= n <= 2^24)
+ *   -> n/2 + 1 complex outputs each (X[k], k = 0..n/2), exec with MCFFT_FORWARD.
+ * C2R: n/2 + 1 complex inputs per transform (the non-redundant half of a
+ *   Hermitian spectrum) -> n reals (unnormalised inverse), exec with MCFFT_INVERSE.
+ * Real data is stored as pairs (x[2m], x[2m+1]) in the complex format of the
+ * same name (FP16/BF16: 2 bytes per real; split: the split layout of the pair).
+ * Computed as a length-n/2 complex FFT plus an FP32 pass with FP64-generated
+ * twiddles W_n^k; the plan owns an n/2-element-per-transform buffer.  The C2R
+ * plan reports one more power of two in its output exponent than the complex
+ * plan of length n/2.  mcfft_exec_passes / mcfft_plan_tune do not apply. */
+mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt,
+                              mcfftFormat out_fmt, mcfftScaling scaling);
+mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt,
+                              mcfftFormat out_fmt, mcfftScaling scaling);
+
 /* Ragged batch: n_items transforms of any supported shapes (rank 1..3; 1D
  * lengths 16..2^18, sides 16..4096), each at its own element offset in the
  * exec input and output buffers, run by one persistent kernel launch per pass

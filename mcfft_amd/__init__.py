@@ -66,6 +66,8 @@ def lib() -> ctypes.CDLL:
         L.mcfft_status_string.restype = ctypes.c_char_p
         L.mcfft_version.restype = i
         L.mcfft_plan_ragged.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(RaggedItem), i, i, i]
+        L.mcfft_plan_r2c_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
+        L.mcfft_plan_c2r_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
         L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_status_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.mcfft_plan_tune.argtypes = [vp, vp, vp, i, vp, i]
@@ -73,7 +75,8 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_double), vp]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
-                  "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms",
+                  "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_plan_r2c_1d",
+                  "mcfft_plan_c2r_1d",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -189,6 +192,39 @@ class RaggedPlan(Plan):
         f = ctypes.c_uint()
         _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
         return f.value
+
+
+class RealPlan(Plan):
+    """1D real-input forward ("r2c": n reals -> n/2+1 complex per transform,
+    exec with FORWARD) or real-output inverse ("c2r": n/2+1 complex -> n reals,
+    exec with INVERSE).  Real data is stored as complex pairs (x[2m], x[2m+1])."""
+
+    def __init__(self, n: int, batch: int, kind: str = "r2c", in_fmt: str = "fp16", out_fmt: str | None = None,
+                 scaling: str = "balanced"):
+        out_fmt = out_fmt or in_fmt
+        self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = [int(n)], int(batch), in_fmt, out_fmt, scaling
+        self.kind = kind
+        self._h = ctypes.c_void_p()
+        fn = lib().mcfft_plan_r2c_1d if kind == "r2c" else lib().mcfft_plan_c2r_1d
+        _check(fn(ctypes.byref(self._h), int(n), int(batch), FORMATS[in_fmt], FORMATS[out_fmt], SCALINGS[scaling]),
+               f"mcfft_plan_{kind}_1d")
+        e = ctypes.c_int()
+        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
+        self.exponent = e.value
+        self.n = int(n)
+
+
+def real_to_pairs(x: np.ndarray) -> np.ndarray:
+    """batch x n reals -> complex pairs x[2m] + i x[2m+1] (the real storage layout)."""
+    x = np.asarray(x, dtype=np.float64)
+    return (x[..., 0::2] + 1j * x[..., 1::2]).reshape(-1)
+
+
+def pairs_to_real(z: np.ndarray, n: int) -> np.ndarray:
+    z = np.asarray(z).reshape(-1, n // 2)
+    r = np.empty((z.shape[0], n))
+    r[:, 0::2], r[:, 1::2] = z.real, z.imag
+    return r
 
 
 def error_norms(y, fmt: str, exponent: int, ref, stream=None):

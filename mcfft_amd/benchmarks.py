@@ -1,6 +1,6 @@
 """Throughput and accuracy of every workload the library is measured on.
 
-    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c6,ragged_small] [--reps 10]
+    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c6,ragged_small,real] [--reps 10]
 
 One JSON line per (config, precision): GFLOP/s (5 N log2 N, N = elements per
 transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
@@ -259,9 +259,48 @@ def ragged_small(torch, reps, fi="fp16", fo="fp16", seed=1):
     return res
 
 
+def bench_real(torch, reps, n=131072, batch=256):
+    """Real-input forward (R2C) of `batch` FP16 real sequences of length n: this
+    library (length-n/2 complex FFT + FP32 post-pass) next to rocFFT half
+    precision; error against the FP64 oracle on a sample of transforms."""
+    import oracle
+    from mcfft_amd import vendor_baseline as vb
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((batch, n))
+    z = mc.real_to_pairs(x)
+    d_in = mc.to_device(z, "fp16")
+    h = n // 2
+    d_out = mc.empty((h + 1) * batch, "fp16")
+    p = mc.RealPlan(n, batch, "r2c", "fp16", "fp16")
+    ms = _timer(torch, lambda i: p.exec(d_in, d_out, mc.FORWARD), reps)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    k = 4
+    y = mc.from_device(d_out[: k * (h + 1)], "fp16", p.exponent)
+    xq = mc.pairs_to_real(mc.decode(mc.encode(z[: k * h], "fp16"), "fp16"), n)
+    ref = oracle.fft(xq.reshape(-1).astype(np.complex128), (n,)).reshape(k, n)[:, : h + 1].reshape(-1)
+    flops = 2.5 * n * math.log2(n) * batch   # half of the complex transform's 5 N log2 N
+    res = {"config": "r2c", "n": n, "batch": batch, "in": "fp16", "out": "fp16", "ms": round(ms, 4),
+           "gflops_2.5nlogn": round(flops / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round((2 * n + 4 * (h + 1)) * batch / (ms * 1e-3) / 1e9 / HBM, 4),
+           "rel_l2": oracle.rel_l2(y, ref), "checked": k}
+    p.destroy()
+    try:
+        rp = vb.RocfftPlan([n], batch, "half", real=True)
+        xr = torch.from_numpy(x.astype(np.float16)).to("cuda")
+        vout = torch.empty(((h + 1) * batch, 2), dtype=torch.float16, device="cuda")
+        st = torch.cuda.current_stream().cuda_stream
+        vms = _timer(torch, lambda i: rp.exec(xr.data_ptr(), vout.data_ptr(), st), reps)
+        res.update({"rocfft_half_ms": round(vms, 4), "speedup_vs_rocfft": round(vms / ms, 3)})
+        rp.destroy()
+    except Exception as e:  # noqa: BLE001
+        res["rocfft"] = f"unavailable: {e}"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c1,c2,c3,c4,c6,ragged_small")
+    ap.add_argument("--config", default="c1,c2,c3,c4,c6,ragged_small,real")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     import torch
@@ -275,6 +314,9 @@ def main():
         "c6": ([65536], 4096, "balanced", 16),
     }
     for cfg in cfgs:
+        if cfg == "real":
+            print(json.dumps(bench_real(torch, args.reps)), flush=True)
+            continue
         if cfg == "ragged_small":
             for fi, fo in (("fp16", "fp16"), ("fp16", "split")):
                 print(json.dumps(ragged_small(torch, args.reps, fi, fo)), flush=True)

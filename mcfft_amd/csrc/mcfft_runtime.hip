@@ -19,6 +19,7 @@
 #include "fft_pass.h"
 #include "plan_core.h"
 #include "ragged.h"
+#include "real_pass.h"
 #include "ragged_plan.h"
 
 using namespace mcfft;
@@ -40,7 +41,20 @@ struct mcfftPlan_st {
   RaggedTask* rtasks = nullptr;
   unsigned* rcounters = nullptr;
   int rgrid = 0;
+  // real-input / real-output plans: `core` is the length-H complex plan
+  int real_kind = 0;          // 0 complex, 1 R2C, 2 C2R
+  long long real_h = 0;       // H = N / 2
+  void* real_tmp = nullptr;   // H complex per transform (C2C output / input)
+  f32x2* real_tw[2] = {nullptr, nullptr};  // W_N^k, k = 0..H, per direction
+  int real_tmp_fmt = 0;
+  int real_in_fmt = 0;        // C2R: storage format of the caller's spectrum
+  int real_grid = 0;
 };
+
+namespace mcfft {
+extern const void* const kR2CPost[3];
+extern const void* const kC2RPre[3];
+}  // namespace mcfft
 
 namespace {
 
@@ -248,6 +262,9 @@ void free_plan(mcfftPlan p) {
     if (p->rpasses[d]) (void)hipFree(p->rpasses[d]);
   if (p->rtasks) (void)hipFree(p->rtasks);
   if (p->rcounters) (void)hipFree(p->rcounters);
+  if (p->real_tmp) (void)hipFree(p->real_tmp);
+  for (int d = 0; d < 2; ++d)
+    if (p->real_tw[d]) (void)hipFree(p->real_tw[d]);
   delete p;
 }
 
@@ -333,6 +350,67 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
   }
   *plan = p;
   return MCFFT_SUCCESS;
+}
+
+// Real-input forward / real-output inverse plans (see include/mcfft.h).
+static mcfftStatus plan_real(mcfftPlan* plan, int kind, long long n, long long batch, mcfftFormat in_fmt,
+                             mcfftFormat out_fmt, mcfftScaling scaling) {
+  if (!plan) return MCFFT_INVALID_VALUE;
+  *plan = nullptr;
+  if (n < 32 || (n & (n - 1)) || batch < 1) return n < 1 || batch < 1 ? MCFFT_INVALID_VALUE : MCFFT_UNSUPPORTED;
+  const long long h = n / 2;
+  const int cm = compute_mode((int)in_fmt, (int)out_fmt);
+  if (cm < 0) return MCFFT_UNSUPPORTED;
+  // C2R keeps its pre-pass output in the complex plan's input format
+  const int tmp_fmt = kind == 1 ? (int)out_fmt : (cm == CM_SPLIT ? FMT_SPLIT : (int)in_fmt);
+  mcfftPlan p = nullptr;
+  mcfftStatus st = mcfft_plan_many(&p, 1, &h, batch, kind == 1 ? in_fmt : (mcfftFormat)tmp_fmt, out_fmt, scaling);
+  if (st != MCFFT_SUCCESS) return st;
+  p->real_kind = kind;
+  p->real_h = h;
+  p->real_tmp_fmt = tmp_fmt;
+  p->real_in_fmt = (int)in_fmt;
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) st = MCFFT_HIP_ERROR;
+  p->real_grid = 8 * (n_cu > 0 ? n_cu : 1);
+  if (st == MCFFT_SUCCESS && hipMalloc(&p->real_tmp, (size_t)(h * batch) * (size_t)elem_bytes(tmp_fmt)) != hipSuccess)
+    st = MCFFT_ALLOC_FAILED;
+  // W_N^k, k = 0..H, generated in FP64 (forward and inverse signs)
+  TwSpec* dspec = nullptr;
+  if (st == MCFFT_SUCCESS) {
+    std::vector<TwSpec> spec((size_t)h + 1);
+    for (long long k = 0; k <= h; ++k) spec[(size_t)k] = TwSpec{k, n, 1.0f};
+    if (hipMalloc(&dspec, sizeof(TwSpec) * spec.size()) != hipSuccess ||
+        hipMemcpy(dspec, spec.data(), sizeof(TwSpec) * spec.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = MCFFT_HIP_ERROR;
+    for (int d = 0; d < 2 && st == MCFFT_SUCCESS; ++d) {
+      if (hipMalloc(&p->real_tw[d], sizeof(f32x2) * spec.size()) != hipSuccess) {
+        st = MCFFT_ALLOC_FAILED;
+        break;
+      }
+      gen_twiddles<<<(unsigned)((spec.size() + 255) / 256), 256>>>(dspec, (long long)spec.size(), d == 0 ? -1 : 1,
+                                                                   p->real_tw[d]);
+      if (hipGetLastError() != hipSuccess) st = MCFFT_HIP_ERROR;
+    }
+    if (st == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) st = MCFFT_HIP_ERROR;
+  }
+  if (dspec) (void)hipFree(dspec);
+  if (st != MCFFT_SUCCESS) {
+    free_plan(p);
+    return st;
+  }
+  *plan = p;
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt,
+                              mcfftScaling scaling) {
+  return plan_real(plan, 1, n, batch, in_fmt, out_fmt, scaling);
+}
+
+mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt,
+                              mcfftScaling scaling) {
+  return plan_real(plan, 2, n, batch, in_fmt, out_fmt, scaling);
 }
 
 mcfftStatus mcfft_plan_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt) {
@@ -493,6 +571,40 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
   return MCFFT_SUCCESS;
 }
 
+static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
+                                int count);
+
+// R2C: complex FFT of length H into the plan's buffer, then the post-pass;
+// C2R: the pre-pass into the buffer, then the inverse complex FFT.
+static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int direction, hipStream_t s) {
+  if (!in || !out) return MCFFT_INVALID_VALUE;
+  if ((plan->real_kind == 1) != (direction == MCFFT_FORWARD)) return MCFFT_INVALID_VALUE;
+  const int cm = plan->core.cm;
+  const int np = (int)plan->core.passes.size();
+  RealArgs ra;
+  ra.h = plan->real_h;
+  ra.batch = plan->core.batch;
+  ra.tw = plan->real_tw[direction == MCFFT_FORWARD ? 0 : 1];
+  const long long total = plan->real_kind == 1 ? (ra.h / 2 + 1) * ra.batch : ra.h * ra.batch;
+  const long long need = (total + 1023) / 1024;
+  const unsigned grid = (unsigned)(need < plan->real_grid ? need : plan->real_grid);
+  void* args[] = {&ra};
+  if (plan->real_kind == 1) {
+    mcfftStatus st = exec_complex(plan, in, plan->real_tmp, direction, s, 0, np);
+    if (st != MCFFT_SUCCESS) return st;
+    ra.in = plan->real_tmp;
+    ra.out = out;
+    ra.in_fmt = plan->core.out_fmt;
+    if (hipLaunchKernel(kR2CPost[cm], dim3(grid), dim3(1024), args, 0, s) != hipSuccess) return MCFFT_HIP_ERROR;
+    return MCFFT_SUCCESS;
+  }
+  ra.in = in;
+  ra.out = plan->real_tmp;
+  ra.in_fmt = plan->real_in_fmt;
+  if (hipLaunchKernel(kC2RPre[cm], dim3(grid), dim3(1024), args, 0, s) != hipSuccess) return MCFFT_HIP_ERROR;
+  return exec_complex(plan, plan->real_tmp, out, direction, s, 0, np);
+}
+
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (plan->ragged) {
@@ -500,11 +612,12 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
     if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
     return exec_ragged(plan, in, out, direction, (hipStream_t)stream);
   }
-  return mcfft_exec_passes(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
+  if (plan->real_kind) return exec_real(plan, in, out, direction, (hipStream_t)stream);
+  return exec_complex(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
 }
 
 mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
-  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
+  if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
   if (!in || !out || reps < 1) return MCFFT_INVALID_VALUE;
   hipStream_t s = (hipStream_t)stream;
   int n_cu = 0;
@@ -622,7 +735,12 @@ mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count) {
 
 mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
                               int count) {
-  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
+  if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
+  return exec_complex(plan, in, out, direction, stream, first, count);
+}
+
+static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
+                                int count) {
   if (first < 0 || count < 0 || first + count > (int)plan->core.passes.size()) return MCFFT_INVALID_VALUE;
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
@@ -690,14 +808,16 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
 mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!exponent) return MCFFT_INVALID_VALUE;
-  *exponent = plan->core.out_exponent;
+  *exponent = plan->core.out_exponent + (plan->real_kind == 2 ? 1 : 0);
   return MCFFT_SUCCESS;
 }
 
 mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!bytes) return MCFFT_INVALID_VALUE;
-  *bytes = plan->ws_bytes;
+  size_t b = plan->ws_bytes;
+  if (plan->real_kind) b += (size_t)(plan->real_h * plan->core.batch) * (size_t)elem_bytes(plan->real_tmp_fmt);
+  *bytes = b;
   return MCFFT_SUCCESS;
 }
 

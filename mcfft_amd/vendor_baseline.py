@@ -54,12 +54,14 @@ def _check(st, what):
 class RocfftPlan:
     """Batched C2C plan (row-major dims, last contiguous), out of place."""
 
-    def __init__(self, dims, batch: int, precision: str = "half", inverse: bool = False):
+    def __init__(self, dims, batch: int, precision: str = "half", inverse: bool = False, real: bool = False):
+        """real: real-input forward (inverse=False) or real-output inverse transform."""
         import torch
         L = lib()
         lengths = (ctypes.c_size_t * len(dims))(*reversed([int(d) for d in dims]))  # fastest first
         self._plan = ctypes.c_void_p()
-        _check(L.rocfft_plan_create(ctypes.byref(self._plan), 1, 1 if inverse else 0, PRECISIONS[precision],
+        ttype = (3 if inverse else 2) if real else (1 if inverse else 0)
+        _check(L.rocfft_plan_create(ctypes.byref(self._plan), 1, ttype, PRECISIONS[precision],
                                     len(dims), lengths, int(batch), None), "rocfft_plan_create")
         ws = ctypes.c_size_t()
         _check(L.rocfft_plan_get_work_buffer_size(self._plan, ctypes.byref(ws)), "rocfft_plan_get_work_buffer_size")

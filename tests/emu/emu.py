@@ -165,3 +165,32 @@ def last_kernels() -> str:
     L = lib()
     L.emu_last_kernels.restype = ctypes.c_char_p
     return L.emu_last_kernels().decode()
+
+
+def real_exec(kind: str, data, n: int, batch: int, in_fmt: int, out_fmt: int, scaling: int = 1):
+    """kind "r2c": data = batch x n reals -> batch x (n/2+1) complex (times 2^e);
+    kind "c2r": data = batch x (n/2+1) complex -> batch x n reals (times 2^e)."""
+    h = n // 2
+    if kind == "r2c":
+        x = np.asarray(data, dtype=np.float64).reshape(batch, n)
+        src = to_storage((x[:, 0::2] + 1j * x[:, 1::2]).reshape(-1), in_fmt)
+        out = np.zeros((h + 1) * batch * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    else:
+        src = to_storage(np.asarray(data, dtype=np.complex128).reshape(-1), in_fmt)
+        out = np.zeros(h * batch * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    e = ctypes.c_int(0)
+    L = lib()
+    L.emu_real_exec.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    L.emu_real_exec.restype = ctypes.c_int
+    rc = L.emu_real_exec(1 if kind == "r2c" else 2, n, batch, in_fmt, out_fmt, scaling, src.ctypes.data,
+                         out.ctypes.data, ctypes.byref(e))
+    if rc != 0:
+        raise RuntimeError(f"emu_real_exec status {rc}")
+    y = from_storage(out, out_fmt) * (2.0 ** e.value)
+    if kind == "r2c":
+        return y.reshape(batch, h + 1)
+    z = y.reshape(batch, h)
+    r = np.empty((batch, n))
+    r[:, 0::2], r[:, 1::2] = z.real, z.imag
+    return r

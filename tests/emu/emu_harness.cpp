@@ -2,6 +2,7 @@
 // (test infrastructure).  The kernel sources are compiled unchanged with
 // MCFFT_EMU: every LDS and global access is bounds-checked, so a wrong index
 // aborts here instead of faulting a GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -13,6 +14,7 @@
 #include "plan_core.h"
 #include "ragged.h"
 #include "ragged_plan.h"
+#include "real_pass.h"
 
 using namespace mcfft;
 
@@ -300,4 +302,61 @@ extern "C" long long emu_ragged_check(int n_items, const long long* spec, int in
   stats[4] = r.ws_elems;
   stats[5] = (long long)r.tw.size();
   return bad;
+}
+
+// Real transforms through the emulated kernels, mirroring exec_real in the
+// runtime: kind 1 = R2C (complex FFT of length n/2, then the post-pass), kind 2 =
+// C2R (pre-pass, then the inverse complex FFT).  Returns the output exponent.
+extern "C" void mcfft_r2c_post_0(RealArgs a);
+extern "C" void mcfft_r2c_post_1(RealArgs a);
+extern "C" void mcfft_r2c_post_2(RealArgs a);
+extern "C" void mcfft_c2r_pre_0(RealArgs a);
+extern "C" void mcfft_c2r_pre_1(RealArgs a);
+extern "C" void mcfft_c2r_pre_2(RealArgs a);
+
+extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt, int out_fmt, int scaling,
+                             const void* in, void* out, int* out_exponent) {
+  const long long h = n / 2;
+  const int cm = compute_mode(in_fmt, out_fmt);
+  if (cm < 0 || n < 32 || (n & (n - 1))) return ST_UNSUPPORTED;
+  const int tmp_fmt = kind == 1 ? out_fmt : (cm == 2 ? 2 : in_fmt);
+  std::vector<unsigned char> tmp((size_t)(h * batch * elem_bytes(tmp_fmt)) + 16);
+  const int sign = kind == 1 ? -1 : 1;
+  std::vector<f32x2> tw((size_t)h + 1);
+  for (long long k = 0; k <= h; ++k) twiddle_host(TwSpec{k, n, 1.0f}, sign, tw[(size_t)k]);
+  void (*post[3])(RealArgs) = {mcfft_r2c_post_0, mcfft_r2c_post_1, mcfft_r2c_post_2};
+  void (*pre[3])(RealArgs) = {mcfft_c2r_pre_0, mcfft_c2r_pre_1, mcfft_c2r_pre_2};
+  RealArgs ra;
+  ra.h = h;
+  ra.batch = batch;
+  ra.tw = tw.data();
+  const long long total = kind == 1 ? (h / 2 + 1) * batch : h * batch;
+  const int grid = (int)std::min<long long>((total + 1023) / 1024, 3);
+  int e = 0;
+  auto regs = [&](const void* a, size_t abytes, const void* b, size_t bbytes) {
+    emu::clear_ranges();
+    emu::register_range(a, abytes);
+    emu::register_range(b, bbytes);
+    emu::register_range(tw.data(), tw.size() * sizeof(f32x2));
+  };
+  if (kind == 1) {
+    int st = emu_exec(1, &h, batch, in_fmt, out_fmt, scaling, 0, in, tmp.data(), 0, &e);
+    if (st) return st;
+    regs(tmp.data(), (size_t)(h * batch * elem_bytes(tmp_fmt)), out, (size_t)((h + 1) * batch * elem_bytes(out_fmt)));
+    ra.in = tmp.data();
+    ra.out = out;
+    ra.in_fmt = out_fmt;
+    emu::launch(grid, 1024, 0, [&] { post[cm](ra); });
+  } else {
+    regs(in, (size_t)((h + 1) * batch * elem_bytes(in_fmt)), tmp.data(), (size_t)(h * batch * elem_bytes(tmp_fmt)));
+    ra.in = in;
+    ra.out = tmp.data();
+    ra.in_fmt = in_fmt;
+    emu::launch(grid, 1024, 0, [&] { pre[cm](ra); });
+    int st = emu_exec(1, &h, batch, tmp_fmt, out_fmt, scaling, 1, tmp.data(), out, 0, &e);
+    if (st) return st;
+    e += 1;
+  }
+  if (out_exponent) *out_exponent = e;
+  return 0;
 }

@@ -180,3 +180,36 @@ def test_multi_dim_chunked(fi, fo):
     transform per chunk and a partial last chunk."""
     check([64, 32], 3, fi, fo, ws_limit=64 * 32 * 8)
     check([32, 16, 32], 2, fi, fo, ws_limit=1)
+
+
+@pytest.mark.parametrize("n,batch", [(32, 4), (256, 3), (8192, 2), (131072, 1)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_r2c(n, batch, fi, fo):
+    """Real-input forward: X[k], k = 0..n/2, of the complex FFT of the real input."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((batch, n))
+    xq = emu.from_storage(emu.to_storage((x[:, 0::2] + 1j * x[:, 1::2]).reshape(-1), fi), fi).reshape(batch, n // 2)
+    xr = np.empty((batch, n))
+    xr[:, 0::2], xr[:, 1::2] = xq.real, xq.imag
+    y = emu.real_exec("r2c", x, n, batch, fi, fo)
+    ref = oracle.fft(xr.reshape(-1).astype(np.complex128), (n,)).reshape(batch, n)[:, : n // 2 + 1]
+    err = oracle.rel_l2(y.reshape(-1), ref.reshape(-1))
+    assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (n, err)
+
+
+@pytest.mark.parametrize("n,batch", [(32, 4), (1024, 2), (65536, 1)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_c2r(n, batch, fi, fo):
+    """Real-output inverse of a Hermitian spectrum given by its first n/2 + 1 terms."""
+    rng = np.random.default_rng(n + 1)
+    h = n // 2
+    spec = rng.standard_normal((batch, h + 1)) + 1j * rng.standard_normal((batch, h + 1))
+    spec[:, 0] = spec[:, 0].real
+    spec[:, h] = spec[:, h].real
+    sq = emu.from_storage(emu.to_storage(spec.reshape(-1), fi), fi).reshape(batch, h + 1)
+    full = np.concatenate([sq, np.conj(sq[:, h - 1:0:-1])], axis=1)
+    ref = oracle.fft(full.reshape(-1), (n,), sign=+1).reshape(batch, n)
+    assert np.abs(ref.imag).max() <= 1e-9 * np.abs(ref).max()
+    y = emu.real_exec("c2r", spec, n, batch, fi, fo)
+    err = oracle.rel_l2(y.reshape(-1).astype(np.complex128), ref.real.reshape(-1).astype(np.complex128))
+    assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (n, err)

@@ -1,0 +1,79 @@
+"""GPU: real-input forward (R2C) and real-output inverse (C2R) 1D transforms
+against the FP64 oracle (the complex FFT of the real input / of the
+Hermitian-extended spectrum)."""
+import numpy as np
+import pytest
+
+import mcfft_amd as mc
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = {"fp16": 1.5e-3, "bf16": 1.2e-2, "split": 1e-6}
+FMTS = [("fp16", "fp16"), ("bf16", "bf16"), ("fp16", "split")]
+
+
+@pytest.mark.parametrize("n,batch", [(32, 64), (256, 512), (4096, 16), (65536, 4), (131072, 2), (1 << 20, 1)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_r2c(torch_cuda, n, batch, fi, fo):
+    torch = torch_cuda
+    rng = np.random.default_rng(n + batch)
+    x = rng.standard_normal((batch, n))
+    z = mc.real_to_pairs(x)
+    zq = mc.decode(mc.encode(z, fi), fi)
+    xq = mc.pairs_to_real(zq, n)
+    p = mc.RealPlan(n, batch, "r2c", fi, fo)
+    d_in = mc.to_device(z, fi)
+    d_out = mc.empty((n // 2 + 1) * batch, fo)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    y = mc.from_device(d_out, fo, p.exponent)
+    p.destroy()
+    ref = oracle.fft(xq.reshape(-1).astype(np.complex128), (n,)).reshape(batch, n)[:, : n // 2 + 1]
+    err = oracle.rel_l2(y, ref.reshape(-1))
+    print(f"r2c n={n} b={batch} {fi}->{fo} rel_l2={err:.3e}")
+    assert err <= TOL[fo if fo == "split" else fi]
+
+
+@pytest.mark.parametrize("n,batch", [(32, 64), (1024, 32), (65536, 4), (131072, 2)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_c2r(torch_cuda, n, batch, fi, fo):
+    torch = torch_cuda
+    rng = np.random.default_rng(n + 3)
+    h = n // 2
+    spec = rng.standard_normal((batch, h + 1)) + 1j * rng.standard_normal((batch, h + 1))
+    spec[:, 0] = spec[:, 0].real
+    spec[:, h] = spec[:, h].real
+    sq = mc.decode(mc.encode(spec.reshape(-1), fi), fi).reshape(batch, h + 1)
+    full = np.concatenate([sq, np.conj(sq[:, h - 1:0:-1])], axis=1)
+    ref = oracle.fft(full.reshape(-1), (n,), sign=+1).real.reshape(batch, n)
+    p = mc.RealPlan(n, batch, "c2r", fi, fo)
+    d_in = mc.to_device(spec.reshape(-1), fi)
+    d_out = mc.empty(h * batch, fo)
+    p.exec(d_in, d_out, mc.INVERSE)
+    torch.cuda.synchronize()
+    y = mc.pairs_to_real(mc.from_device(d_out, fo, p.exponent), n)
+    p.destroy()
+    err = oracle.rel_l2(y.reshape(-1).astype(np.complex128), ref.reshape(-1).astype(np.complex128))
+    print(f"c2r n={n} b={batch} {fi}->{fo} rel_l2={err:.3e}")
+    assert err <= TOL[fo if fo == "split" else fi]
+
+
+def test_r2c_then_c2r_round_trip(torch_cuda):
+    """c2r(r2c(x)) = n x (split both ways)."""
+    torch = torch_cuda
+    n, batch = 8192, 8
+    x = np.random.default_rng(4).standard_normal((batch, n))
+    z = mc.real_to_pairs(x)
+    xq = mc.pairs_to_real(mc.decode(mc.encode(z, "fp16"), "fp16"), n)
+    pf = mc.RealPlan(n, batch, "r2c", "fp16", "split")
+    pi = mc.RealPlan(n, batch, "c2r", "split", "split")
+    d_in = mc.to_device(z, "fp16")
+    d_mid = mc.empty((n // 2 + 1) * batch, "split")
+    d_out = mc.empty(n // 2 * batch, "split")
+    pf.exec(d_in, d_mid, mc.FORWARD)
+    pi.exec(d_mid, d_out, mc.INVERSE)
+    torch.cuda.synchronize()
+    y = mc.pairs_to_real(mc.from_device(d_out, "split", pf.exponent + pi.exponent), n) / n
+    pf.destroy()
+    pi.destroy()
+    assert oracle.rel_l2(y.reshape(-1).astype(np.complex128), xq.reshape(-1).astype(np.complex128)) <= 1e-6
