@@ -113,6 +113,8 @@ def test_plan_describe_and_errors():
         emu.describe([8192, 16], 1, 0, 0)    # strided side above 4096
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([256], 1, 2, 0)         # split in, fp16 out
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([16, 65536], 1, 0, 2)   # split rows of 65536 (single-pass kernel is FP16/BF16 only)
     with pytest.raises(RuntimeError, match="status 1"):
         emu.describe([256], 0, 0, 0)         # batch 0
 
