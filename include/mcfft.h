@@ -109,6 +109,40 @@ mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcf
 mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt,
                               mcfftFormat out_fmt, mcfftScaling scaling);
 
+/* Advanced data layouts; no reference counterpart (the reference takes
+ * contiguous interleaved data only, source/mcfft_1d_utils.cpp:37-66).
+ * Element (b, i_0, .., i_{rank-1}) of transform b of a side is at
+ *     b * dist + sum_d i_d * strides[d]
+ * counted in elements of that side's form:
+ *   planar = 0: interleaved complex elements of the side's format (4 B; split 8 B);
+ *   planar = 1: two arrays, real parts and imaginary parts, of scalars
+ *               (FP16/BF16: 2 B; split: binary16 hi then lo, 4 B).
+ * Strides and dist are >= 1; output positions must not coincide.  A NULL
+ * layout is the contiguous interleaved layout of mcfft_plan_many. */
+typedef struct mcfftLayout_t {
+  long long strides[3]; /* per dimension, dims order (only the first `rank` are read) */
+  long long dist;       /* between consecutive transforms */
+  int planar;
+} mcfftLayout;
+
+/* A plan whose exec reads and writes the given layouts.  A side that is not
+ * the contiguous interleaved layout is staged through a plan-owned contiguous
+ * buffer by a gather (input) or scatter (output) kernel: one extra read and
+ * write of that side's data; a contiguous side costs nothing extra.  Tuning
+ * (mcfft_plan_tune) and mcfft_exec_passes act on the contiguous transform, with
+ * contiguous buffers; the tuning cache entry of the same shape applies. */
+mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* dims, long long batch,
+                                   const mcfftLayout* in_layout, const mcfftLayout* out_layout,
+                                   mcfftFormat in_fmt, mcfftFormat out_fmt, mcfftScaling scaling);
+/* Execute a layout plan.  *_im are the imaginary-part arrays of planar sides
+ * and must be NULL for interleaved sides.  mcfft_exec(plan, in, out, ...) on a
+ * layout plan equals mcfft_exec_layout(plan, in, NULL, out, NULL, ...).
+ * Input and output may overlap in any way (the staging orders the accesses),
+ * except that an interleaved-contiguous input may alias a contiguous output
+ * only exactly (in place). */
+mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
+                              int direction, void* stream);
+
 /* Ragged batch: n_items transforms of any supported shapes (rank 1..3; 1D
  * lengths 16..2^18, sides 16..4096), each at its own element offset in the
  * exec input and output buffers, run by one persistent kernel launch per pass

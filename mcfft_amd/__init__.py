@@ -37,6 +37,19 @@ class RaggedItem(ctypes.Structure):
                 ("out_offset", ctypes.c_longlong)]
 
 
+class Layout(ctypes.Structure):
+    """mcfftLayout: element (b, i_0, ..) at b*dist + sum_d i_d*strides[d]."""
+    _fields_ = [("strides", ctypes.c_longlong * 3), ("dist", ctypes.c_longlong), ("planar", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, strides, dist: int, planar: bool = False) -> "Layout":
+        v = cls()
+        for j, s in enumerate(strides):
+            v.strides[j] = int(s)
+        v.dist, v.planar = int(dist), int(bool(planar))
+        return v
+
+
 class McfftError(RuntimeError):
     def __init__(self, status: int, what: str):
         super().__init__(f"{what}: {STATUS.get(status, status)} (status {status})")
@@ -66,6 +79,9 @@ def lib() -> ctypes.CDLL:
         L.mcfft_status_string.restype = ctypes.c_char_p
         L.mcfft_version.restype = i
         L.mcfft_plan_ragged.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(RaggedItem), i, i, i]
+        L.mcfft_plan_many_layout.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(ll), ll, ctypes.POINTER(Layout),
+                                             ctypes.POINTER(Layout), i, i, i]
+        L.mcfft_exec_layout.argtypes = [vp, vp, vp, vp, vp, i, vp]
         L.mcfft_plan_r2c_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
         L.mcfft_plan_c2r_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
         L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
@@ -76,7 +92,7 @@ def lib() -> ctypes.CDLL:
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
                   "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_plan_r2c_1d",
-                  "mcfft_plan_c2r_1d",
+                  "mcfft_plan_c2r_1d", "mcfft_plan_many_layout", "mcfft_exec_layout",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -212,6 +228,37 @@ class RealPlan(Plan):
         _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
         self.exponent = e.value
         self.n = int(n)
+
+
+class LayoutPlan(Plan):
+    """A plan reading and writing strided and/or planar layouts (Layout, or
+    None for contiguous interleaved) on each side."""
+
+    def __init__(self, dims, batch: int, in_layout: Layout | None, out_layout: Layout | None,
+                 in_fmt: str = "fp16", out_fmt: str | None = None, scaling: str = "balanced"):
+        dims = [int(d) for d in (dims if hasattr(dims, "__len__") else [dims])]
+        out_fmt = out_fmt or in_fmt
+        self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = dims, int(batch), in_fmt, out_fmt, scaling
+        self.in_layout, self.out_layout = in_layout, out_layout
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_longlong * len(dims))(*dims)
+        ref = lambda l: ctypes.byref(l) if l is not None else None
+        _check(lib().mcfft_plan_many_layout(ctypes.byref(self._h), len(dims), arr, self.batch, ref(in_layout),
+                                            ref(out_layout), FORMATS[in_fmt], FORMATS[out_fmt], SCALINGS[scaling]),
+               "mcfft_plan_many_layout")
+        e = ctypes.c_int()
+        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
+        self.exponent = e.value
+        self.n = int(np.prod(dims))
+
+    def exec_layout(self, in_re, in_im, out_re, out_im, direction: int = FORWARD, stream=None):
+        """torch device tensors; *_im None for interleaved sides."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(in_re.device).cuda_stream
+        p = lambda t: t.data_ptr() if t is not None else None
+        _check(lib().mcfft_exec_layout(self._h, p(in_re), p(in_im), p(out_re), p(out_im), direction, stream),
+               "mcfft_exec_layout")
 
 
 def real_to_pairs(x: np.ndarray) -> np.ndarray:

@@ -298,9 +298,39 @@ def bench_real(torch, reps, n=131072, batch=256):
     return res
 
 
+def bench_layout(torch, reps, n=131072, batch=256):
+    """Planar FP16 input and output (separate re / im arrays) of the C2 shape:
+    gather + transform + scatter next to the contiguous interleaved exec, and
+    the HBM rate of the staging kernels (4 B read + 4 B written per element
+    each)."""
+    g = torch.Generator(device="cuda").manual_seed(3)
+    inter = torch.randn((n * batch, 2), device="cuda", generator=g).half()
+    re, im = inter[:, 0].contiguous(), inter[:, 1].contiguous()
+    o_re, o_im = torch.empty_like(re), torch.empty_like(im)
+    out = torch.empty_like(inter)
+    lay = mc.Layout.make([1], n, True)
+    p = mc.LayoutPlan([n], batch, lay, lay, "fp16", "fp16")
+    q = mc.Plan([n], batch, "fp16", "fp16")
+    ms_l = _timer(torch, lambda i: p.exec_layout(re, im, o_re, o_im, mc.FORWARD), reps)
+    ms_c = _timer(torch, lambda i: q.exec(inter, out, mc.FORWARD), reps)
+    q.exec(inter, out, mc.FORWARD)
+    p.exec_layout(re, im, o_re, o_im, mc.FORWARD)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(o_re.view(torch.int16), out[:, 0].contiguous().view(torch.int16)) and
+                torch.equal(o_im.view(torch.int16), out[:, 1].contiguous().view(torch.int16)))
+    stage_ms = ms_l - ms_c
+    res = {"config": "layout_planar", "n": n, "batch": batch, "fmt": "fp16", "ms_layout": round(ms_l, 4),
+           "ms_contiguous": round(ms_c, 4), "staging_ms": round(stage_ms, 4),
+           "staging_gbs": round(2 * 8 * n * batch / (stage_ms * 1e-3) / 1e9, 1) if stage_ms > 0 else None,
+           "bit_identical": same}
+    p.destroy()
+    q.destroy()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c1,c2,c3,c4,c6,ragged_small,real")
+    ap.add_argument("--config", default="c1,c2,c3,c4,c6,ragged_small,real,layout")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     import torch
@@ -314,6 +344,9 @@ def main():
         "c6": ([65536], 4096, "balanced", 16),
     }
     for cfg in cfgs:
+        if cfg == "layout":
+            print(json.dumps(bench_layout(torch, args.reps)), flush=True)
+            continue
         if cfg == "real":
             print(json.dumps(bench_real(torch, args.reps)), flush=True)
             continue

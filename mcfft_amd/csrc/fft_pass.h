@@ -365,7 +365,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   // out of the tile loop and held in registers (or spilled) across it
   const int tid = opaque(thread_id()), lane = tid & 63, wave = tid >> 6, h = lane >> 5;
   const bool inv = a.inverse != 0;
-  const int sin_ = a.stride_in, sout = a.stride_out;
+  const int sout = a.stride_out;
   const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
   constexpr int esz_out = (int)sizeof(El);
   const GBuf gtw = make_gbuf(a.stage_tw, (unsigned)a.stage_tw_entries * 8u);

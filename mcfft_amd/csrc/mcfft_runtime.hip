@@ -20,6 +20,7 @@
 #include "plan_core.h"
 #include "ragged.h"
 #include "real_pass.h"
+#include "layout_pass.h"
 #include "ragged_plan.h"
 
 using namespace mcfft;
@@ -49,11 +50,24 @@ struct mcfftPlan_st {
   int real_tmp_fmt = 0;
   int real_in_fmt = 0;        // C2R: storage format of the caller's spectrum
   int real_grid = 0;
+  // advanced layouts (mcfft_plan_many_layout): caller side [0] input, [1] output
+  bool layout = false;
+  int lay_rank = 0;
+  long long lay_dims[3] = {1, 1, 1};
+  long long lay_strides[2][3] = {};
+  long long lay_dist[2] = {0, 0};
+  int lay_planar[2] = {0, 0};
+  bool lay_dense[2] = {true, true};
+  void* lay_buf[2] = {nullptr, nullptr};  // contiguous staging per side (one shared buffer when both stage)
+  size_t lay_bytes = 0;
+  int lay_grid = 0;
 };
 
 namespace mcfft {
 extern const void* const kR2CPost[3];
 extern const void* const kC2RPre[3];
+extern const void* const kLayoutGather;
+extern const void* const kLayoutScatter;
 }  // namespace mcfft
 
 namespace {
@@ -265,6 +279,8 @@ void free_plan(mcfftPlan p) {
   if (p->real_tmp) (void)hipFree(p->real_tmp);
   for (int d = 0; d < 2; ++d)
     if (p->real_tw[d]) (void)hipFree(p->real_tw[d]);
+  if (p->lay_buf[0]) (void)hipFree(p->lay_buf[0]);
+  if (p->lay_buf[1] && p->lay_buf[1] != p->lay_buf[0]) (void)hipFree(p->lay_buf[1]);
   delete p;
 }
 
@@ -411,6 +427,70 @@ mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcf
 mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt,
                               mcfftScaling scaling) {
   return plan_real(plan, 2, n, batch, in_fmt, out_fmt, scaling);
+}
+
+mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* dims, long long batch,
+                                   const mcfftLayout* in_layout, const mcfftLayout* out_layout, mcfftFormat in_fmt,
+                                   mcfftFormat out_fmt, mcfftScaling scaling) {
+  if (!plan || !dims) return MCFFT_INVALID_VALUE;
+  *plan = nullptr;
+  if (rank < 1 || rank > 3 || batch < 1) return MCFFT_INVALID_VALUE;
+  const mcfftLayout* lay[2] = {in_layout, out_layout};
+  for (int sd = 0; sd < 2; ++sd)
+    if (lay[sd]) {
+      const int v = layout_validate(rank, dims, batch, lay[sd]->strides, lay[sd]->dist, lay[sd]->planar);
+      if (v != ST_OK) return (mcfftStatus)v;
+    }
+  mcfftPlan p = nullptr;
+  mcfftStatus st = mcfft_plan_many(&p, rank, dims, batch, in_fmt, out_fmt, scaling);
+  if (st != MCFFT_SUCCESS) return st;
+  p->layout = true;
+  p->lay_rank = rank;
+  long long n = 1;
+  for (int d = 0; d < rank; ++d) {
+    p->lay_dims[d] = dims[d];
+    n *= dims[d];
+  }
+  for (int sd = 0; sd < 2; ++sd) {
+    if (lay[sd]) {
+      for (int d = 0; d < rank; ++d) p->lay_strides[sd][d] = lay[sd]->strides[d];
+      p->lay_dist[sd] = lay[sd]->dist;
+      p->lay_planar[sd] = lay[sd]->planar;
+    } else {
+      long long s = 1;
+      for (int d = rank - 1; d >= 0; --d) {
+        p->lay_strides[sd][d] = s;
+        s *= dims[d];
+      }
+      p->lay_dist[sd] = n;
+    }
+    p->lay_dense[sd] = layout_is_dense(rank, dims, p->lay_strides[sd], p->lay_dist[sd], p->lay_planar[sd]);
+  }
+  // staging: one buffer serves both sides when both stage and the formats match
+  // (the transform then runs in place on it)
+  const size_t bytes[2] = {(size_t)(n * batch) * (size_t)elem_bytes(in_fmt),
+                           (size_t)(n * batch) * (size_t)elem_bytes(out_fmt)};
+  const bool share = !p->lay_dense[0] && !p->lay_dense[1] && in_fmt == out_fmt;
+  for (int sd = 0; sd < 2 && st == MCFFT_SUCCESS; ++sd) {
+    if (p->lay_dense[sd]) continue;
+    if (sd == 1 && share) {
+      p->lay_buf[1] = p->lay_buf[0];
+      continue;
+    }
+    if (hipMalloc(&p->lay_buf[sd], bytes[sd]) != hipSuccess) st = MCFFT_ALLOC_FAILED;
+    else p->lay_bytes += bytes[sd];
+  }
+  int n_cu = 0;
+  if (st == MCFFT_SUCCESS &&
+      (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || n_cu < 1))
+    st = MCFFT_HIP_ERROR;
+  p->lay_grid = 32 * (n_cu > 0 ? n_cu : 1);
+  if (st != MCFFT_SUCCESS) {
+    free_plan(p);
+    return st;
+  }
+  *plan = p;
+  return MCFFT_SUCCESS;
 }
 
 mcfftStatus mcfft_plan_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt) {
@@ -605,8 +685,61 @@ static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int dire
   return exec_complex(plan, plan->real_tmp, out, direction, s, 0, np);
 }
 
+// Gather a non-contiguous input into the staging buffer, run the transform,
+// scatter the result to a non-contiguous output.
+static mcfftStatus exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
+                               int direction, hipStream_t s) {
+  if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
+  if (!in_re || !out_re) return MCFFT_INVALID_VALUE;
+  if ((in_im != nullptr) != (plan->lay_planar[0] != 0) || (out_im != nullptr) != (plan->lay_planar[1] != 0))
+    return MCFFT_INVALID_VALUE;
+  const int fmt[2] = {plan->core.in_fmt, plan->core.out_fmt};
+  const void* ptr[2][2] = {{in_re, in_im}, {out_re, out_im}};
+  for (int sd = 0; sd < 2; ++sd) {
+    const size_t al = (size_t)elem_bytes(fmt[sd]) / (plan->lay_planar[sd] ? 2 : 1);
+    for (int c = 0; c < 2; ++c)
+      if ((size_t)ptr[sd][c] % al) return MCFFT_MISALIGNED;
+  }
+  const int np = (int)plan->core.passes.size();
+  auto launch = [&](int sd, const void* k, LayoutArgs& la) -> mcfftStatus {
+    layout_fill(la, plan->lay_rank, plan->lay_dims, plan->core.batch, plan->lay_strides[sd], plan->lay_dist[sd],
+                plan->lay_planar[sd], fmt[sd]);
+    const long long need = (la.total + kLayoutThreads - 1) / kLayoutThreads;
+    const unsigned grid = (unsigned)(need < plan->lay_grid ? need : plan->lay_grid);
+    void* args[] = {&la};
+    return hipLaunchKernel(k, dim3(grid), dim3(kLayoutThreads), args, 0, s) == hipSuccess ? MCFFT_SUCCESS
+                                                                                        : MCFFT_HIP_ERROR;
+  };
+  const void* src = in_re;
+  void* dst = out_re;
+  if (!plan->lay_dense[0]) {
+    LayoutArgs la{};
+    la.src[0] = in_re;
+    la.src[1] = in_im;
+    la.dst[0] = plan->lay_buf[0];
+    const mcfftStatus st = launch(0, kLayoutGather, la);
+    if (st != MCFFT_SUCCESS) return st;
+    src = plan->lay_buf[0];
+  }
+  if (!plan->lay_dense[1]) dst = plan->lay_buf[1];
+  const mcfftStatus st = exec_complex(plan, src, dst, direction, s, 0, np);
+  if (st != MCFFT_SUCCESS || plan->lay_dense[1]) return st;
+  LayoutArgs la{};
+  la.src[0] = plan->lay_buf[1];
+  la.dst[0] = out_re;
+  la.dst[1] = out_im;
+  return launch(1, kLayoutScatter, la);
+}
+
+mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
+                              int direction, void* stream) {
+  if (!plan || !plan->layout) return MCFFT_INVALID_PLAN;
+  return exec_layout(plan, in_re, in_im, out_re, out_im, direction, (hipStream_t)stream);
+}
+
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
+  if (plan->layout) return exec_layout(plan, in, nullptr, out, nullptr, direction, (hipStream_t)stream);
   if (plan->ragged) {
     if (!in || !out) return MCFFT_INVALID_VALUE;
     if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
@@ -817,6 +950,7 @@ mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
   if (!bytes) return MCFFT_INVALID_VALUE;
   size_t b = plan->ws_bytes;
   if (plan->real_kind) b += (size_t)(plan->real_h * plan->core.batch) * (size_t)elem_bytes(plan->real_tmp_fmt);
+  b += plan->lay_bytes;
   *bytes = b;
   return MCFFT_SUCCESS;
 }

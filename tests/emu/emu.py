@@ -194,3 +194,86 @@ def real_exec(kind: str, data, n: int, batch: int, in_fmt: int, out_fmt: int, sc
     r = np.empty((batch, n))
     r[:, 0::2], r[:, 1::2] = z.real, z.imag
     return r
+
+
+def layout_encode(x, fmt: int, planar: int, positions, extent: int, fill: int = 0x7E00):
+    """Place complex values x[k] at element positions[k] of a caller-side buffer
+    of `extent` elements: interleaved -> one uint16 array (2 or 4 components per
+    element); planar -> (re, im) arrays of scalars (1 or 2 uint16 per scalar).
+    Unused positions hold `fill` (a NaN pattern) so stray reads show."""
+    v = to_storage(np.asarray(x).reshape(-1), fmt).reshape(-1, 4 if fmt == FMT_SPLIT else 2)
+    pos = np.asarray(positions, dtype=np.int64).reshape(-1)
+    if not planar:
+        buf = np.full((extent, v.shape[1]), fill, dtype=np.uint16)
+        buf[pos] = v
+        return buf.reshape(-1), None
+    w = 2 if fmt == FMT_SPLIT else 1
+    re = np.full((extent, w), fill, dtype=np.uint16)
+    im = np.full((extent, w), fill, dtype=np.uint16)
+    if fmt == FMT_SPLIT:
+        re[pos] = v[:, [0, 2]]
+        im[pos] = v[:, [1, 3]]
+    else:
+        re[pos, 0] = v[:, 0]
+        im[pos, 0] = v[:, 1]
+    return re.reshape(-1), im.reshape(-1)
+
+
+def layout_decode(re, im, fmt: int, planar: int, positions) -> np.ndarray:
+    """Inverse of layout_encode at `positions` (complex128, unscaled)."""
+    pos = np.asarray(positions, dtype=np.int64).reshape(-1)
+    c = 4 if fmt == FMT_SPLIT else 2
+    if not planar:
+        return from_storage(re.reshape(-1, c)[pos].reshape(-1), fmt)
+    w = c // 2
+    r, i = re.reshape(-1, w)[pos], im.reshape(-1, w)[pos]
+    v = np.empty((pos.size, c), dtype=np.uint16)
+    if fmt == FMT_SPLIT:
+        v[:, [0, 2]] = r
+        v[:, [1, 3]] = i
+    else:
+        v[:, 0], v[:, 1] = r[:, 0], i[:, 0]
+    return from_storage(v.reshape(-1), fmt)
+
+
+def layout_positions(dims, batch, strides, dist) -> np.ndarray:
+    """Element offset of every (b, i_0, ..) in row-major order."""
+    idx = np.zeros((batch,) + tuple(dims), dtype=np.int64)
+    idx += (np.arange(batch, dtype=np.int64) * dist).reshape((batch,) + (1,) * len(dims))
+    for d, (nd, sd) in enumerate(zip(dims, strides)):
+        shape = [1] * (len(dims) + 1)
+        shape[d + 1] = nd
+        idx += (np.arange(nd, dtype=np.int64) * sd).reshape(shape)
+    return idx.reshape(-1)
+
+
+def layout_exec(x, dims, batch, in_layout, out_layout, in_fmt, out_fmt, scaling=1, inverse=False):
+    """x: batch x prod(dims) complex.  in_layout / out_layout: (strides, dist,
+    planar).  Returns (y as batch x prod(dims) complex times 2^e, output buffers,
+    output positions) so callers can also check untouched positions."""
+    rank = len(dims)
+    L = lib()
+    LL3 = ctypes.c_longlong * 3
+    vp = ctypes.c_void_p
+    L.emu_layout_exec.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                  ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.emu_layout_exec.restype = ctypes.c_int
+    (is_, idist, ipl), (os_, odist, opl) = in_layout, out_layout
+    ipos = layout_positions(dims, batch, is_, idist)
+    opos = layout_positions(dims, batch, os_, odist)
+    iext = int(ipos.max()) + 1
+    oext = int(opos.max()) + 1
+    in_re, in_im = layout_encode(x, in_fmt, ipl, ipos, iext)
+    out_re, out_im = layout_encode(np.zeros(0), out_fmt, opl, np.zeros(0, dtype=np.int64), oext, fill=0x7C01)
+    e = ctypes.c_int(0)
+    ptr = lambda a: a.ctypes.data if a is not None else None
+    rc = L.emu_layout_exec(rank, (ctypes.c_longlong * rank)(*dims), batch, LL3(*(list(is_) + [0] * (3 - rank))),
+                           idist, ipl, LL3(*(list(os_) + [0] * (3 - rank))), odist, opl, in_fmt, out_fmt, scaling,
+                           1 if inverse else 0, ptr(in_re), ptr(in_im), ptr(out_re), ptr(out_im), ctypes.byref(e))
+    if rc != 0:
+        raise RuntimeError(f"emu_layout_exec status {rc}")
+    y = layout_decode(out_re, out_im, out_fmt, opl, opos) * (2.0 ** e.value)
+    return y.reshape(batch, -1), (out_re, out_im), opos

@@ -15,6 +15,7 @@
 #include "ragged.h"
 #include "ragged_plan.h"
 #include "real_pass.h"
+#include "layout_pass.h"
 
 using namespace mcfft;
 
@@ -358,5 +359,72 @@ extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt,
     e += 1;
   }
   if (out_exponent) *out_exponent = e;
+  return 0;
+}
+
+// ---- advanced layouts: gather -> transform -> scatter, as the runtime's
+// exec_layout (staging shared between the sides when both stage and the
+// formats match) ----
+extern "C" void mcfft_layout_gather(LayoutArgs a);
+extern "C" void mcfft_layout_scatter(LayoutArgs a);
+
+extern "C" int emu_layout_exec(int rank, const long long* dims, long long batch, const long long* in_strides,
+                               long long in_dist, int in_planar, const long long* out_strides, long long out_dist,
+                               int out_planar, int in_fmt, int out_fmt, int scaling, int inverse, const void* in_re,
+                               const void* in_im, void* out_re, void* out_im, int* out_exponent) {
+  const long long* st[2] = {in_strides, out_strides};
+  const long long dist[2] = {in_dist, out_dist};
+  const int planar[2] = {in_planar, out_planar}, fmt[2] = {in_fmt, out_fmt};
+  for (int sd = 0; sd < 2; ++sd) {
+    const int v = layout_validate(rank, dims, batch, st[sd], dist[sd], planar[sd]);
+    if (v) return v;
+  }
+  if ((in_im != nullptr) != (in_planar != 0) || (out_im != nullptr) != (out_planar != 0)) return ST_INVALID_VALUE;
+  long long n = 1;
+  for (int d = 0; d < rank; ++d) n *= dims[d];
+  bool dense[2];
+  for (int sd = 0; sd < 2; ++sd) dense[sd] = layout_is_dense(rank, dims, st[sd], dist[sd], planar[sd]);
+  const bool share = !dense[0] && !dense[1] && in_fmt == out_fmt;
+  std::vector<unsigned char> buf[2];
+  for (int sd = 0; sd < 2; ++sd)
+    if (!dense[sd] && !(sd == 1 && share)) buf[sd].resize((size_t)(n * batch * elem_bytes(fmt[sd])));
+  void* stage[2] = {buf[0].data(), share ? buf[0].data() : buf[1].data()};
+  auto run = [&](int sd, bool gather, const void* s0, const void* s1, void* d0, void* d1) {
+    LayoutArgs la{};
+    layout_fill(la, rank, dims, batch, st[sd], dist[sd], planar[sd], fmt[sd]);
+    la.src[0] = s0;
+    la.src[1] = s1;
+    la.dst[0] = d0;
+    la.dst[1] = d1;
+    const long long ext = layout_extent(rank, dims, batch, st[sd], dist[sd]);
+    const size_t side_bytes = (size_t)ext * (size_t)(elem_bytes(fmt[sd]) / (planar[sd] ? 2 : 1));
+    const size_t dense_bytes = (size_t)(n * batch * elem_bytes(fmt[sd]));
+    emu::clear_ranges();
+    if (gather) {
+      emu::register_range(s0, side_bytes);
+      if (s1) emu::register_range(s1, side_bytes);
+      emu::register_range(d0, dense_bytes);
+    } else {
+      emu::register_range(s0, dense_bytes);
+      emu::register_range(d0, side_bytes);
+      if (d1) emu::register_range(d1, side_bytes);
+    }
+    const long long need = (la.total + kLayoutThreads - 1) / kLayoutThreads;
+    const int grid = (int)std::min<long long>(need, 5);  // fewer blocks than elements: grid-stride loop
+    emu::launch(grid, kLayoutThreads, 0, [&] {
+      if (gather) mcfft_layout_gather(la);
+      else mcfft_layout_scatter(la);
+    });
+  };
+  const void* src = in_re;
+  void* dst = out_re;
+  if (!dense[0]) {
+    run(0, true, in_re, in_im, stage[0], nullptr);
+    src = stage[0];
+  }
+  if (!dense[1]) dst = stage[1];
+  const int rc = emu_exec(rank, dims, batch, in_fmt, out_fmt, scaling, inverse, src, dst, 0, out_exponent);
+  if (rc) return rc;
+  if (!dense[1]) run(1, false, stage[1], nullptr, out_re, out_im);
   return 0;
 }
