@@ -33,18 +33,25 @@ HBM = 8000.0  # GB/s
 MALL = 256 << 20
 
 
-def _timer(torch, fn, reps, warmup=2):
+def _timer(torch, fn, reps, warmup=2, min_ms=2.0):
+    """Mean time per call of `reps` back-to-back calls, with the rep count raised
+    until the timed region lasts at least `min_ms` (a few-microsecond exec is
+    otherwise dominated by the jitter of its first launches)."""
     for i in range(warmup):
         fn(i)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for i in range(reps):
-        fn(i)
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    while True:
+        e0.record()
+        for i in range(reps):
+            fn(i)
+        e1.record()
+        torch.cuda.synchronize()
+        total = e0.elapsed_time(e1)
+        if total >= min_ms or reps >= 4096:
+            return total / reps
+        reps = min(4096, max(reps * 2, int(reps * min_ms / max(total, 1e-3)) + 1))
 
 
 def _bytes(fi, fo):
