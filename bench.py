@@ -129,24 +129,35 @@ def main():
     plain_err = {"rel_l2_fwd": oracle.rel_l2(y_f, ref_f), "max_abs_fwd": oracle.max_abs(y_f, ref_f),
                  "rel_l2_roundtrip": oracle.rel_l2(y_rt, xq[:k]), "checked_transforms": 8}
 
-    # per-kernel durations of the plain path: the same forward+inverse steps, one
-    # launch at a time between HIP events on the launch stream, averaged per
-    # kernel over its forward and inverse launches (as rocprofv3 --stats does)
+    # per-kernel durations of the plain path: the same forward+inverse steps, each
+    # pass launched with a start/stop HIP event pair that the runtime attaches to
+    # the kernel dispatch itself (mcfft_exec_timed), on the launch stream;
+    # averaged per kernel over its forward and inverse launches (as rocprofv3
+    # --stats does).  The events are created before the timed region.
     n_pass = pf.pass_count()
     desc = pf.describe().strip().splitlines()
     names = [desc[1 + p_i].split()[1] for p_i in range(n_pass)]
-    launches = []   # (kernel name, start event, end event)
-    for i in range(args.warmup + args.steps):
-        for src, dst, d in ((ins[i % n_rot], mid, mc.FORWARD), (mid, out, mc.INVERSE)):
-            for p_i in range(n_pass):
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                pf.exec_passes(src, dst, d, p_i, 1)
-                e1.record(stream)
-                if i >= args.warmup:
-                    launches.append((names[p_i], e0, e1))
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(n_pass)] for _ in range(2)]
+          for _ in range(2 * (args.warmup + args.steps))]
+    for group in ev:
+        for row in group:
+            for e in row:
+                e.record(stream)    # creates the event
     torch.cuda.synchronize()
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    launches = []   # (kernel name, start event, stop event)
+    for i in range(args.warmup + args.steps):
+        if i == args.warmup:
+            s0.record(stream)
+        for j, (src, dst, d) in enumerate(((ins[i % n_rot], mid, mc.FORWARD), (mid, out, mc.INVERSE))):
+            starts, stops = ev[2 * i + j]
+            pf.exec_timed(src, dst, d, starts, stops)
+            if i >= args.warmup:
+                launches += [(names[p_i], starts[p_i], stops[p_i]) for p_i in range(n_pass)]
+    s1.record(stream)
+    torch.cuda.synchronize()
+    ms_timed_step = s0.elapsed_time(s1) / args.steps
     per = {}
     for name, e0, e1 in launches:
         per.setdefault(name, []).append(e0.elapsed_time(e1))
@@ -261,6 +272,7 @@ def main():
         "plain": plain_err,
         "split": split,
         "kernels": kernels,
+        "ms_per_step_kernel_timed": round(ms_timed_step, 4),   # the steps of the per-kernel timing loop
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -190,6 +190,14 @@ mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, cons
 mcfftStatus mcfft_get_pass_count(mcfftPlan plan, int* count);
 mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
                               int count);
+/* mcfft_exec with every pass's kernel timed on the GPU: pass i is launched
+ * with hipExtLaunchKernel, which records start_events[i] and stop_events[i]
+ * (hipEvent_t, created by the caller) when that kernel starts and ends, with
+ * no extra stream packets between the passes.  Arrays of
+ * mcfft_get_pass_count() events; plans whose batch runs in one chunk
+ * (MCFFT_UNSUPPORTED otherwise; not for ragged or real plans). */
+mcfftStatus mcfft_exec_timed(mcfftPlan plan, const void* in, void* out, int direction, void* stream,
+                             void* const* start_events, void* const* stop_events);
 
 /* E such that true result = stored * 2^E (same for both directions). */
 mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent);

@@ -70,6 +70,7 @@ def lib() -> ctypes.CDLL:
         L.mcfft_plan_3d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, ll, i, i]
         L.mcfft_exec.argtypes = [vp, vp, vp, i, vp]
         L.mcfft_exec_passes.argtypes = [vp, vp, vp, i, vp, i, i]
+        L.mcfft_exec_timed.argtypes = [vp, vp, vp, i, vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L.mcfft_get_pass_count.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_output_exponent.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_workspace_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
@@ -90,7 +91,7 @@ def lib() -> ctypes.CDLL:
         L.mcfft_error_norms.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), vp]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
-                  "mcfft_exec_passes", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
+                  "mcfft_exec_passes", "mcfft_exec_timed", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
                   "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_plan_r2c_1d",
                   "mcfft_plan_c2r_1d", "mcfft_plan_many_layout", "mcfft_exec_layout",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
@@ -169,6 +170,19 @@ class Plan:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         _check(lib().mcfft_exec_passes(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, first, count),
                "mcfft_exec_passes")
+
+    def exec_timed(self, inp, out, direction, start_events, stop_events, stream=None):
+        """exec with each pass's kernel bracketed on the GPU by its own pair of
+        torch.cuda.Events (already created, e.g. recorded once): elapsed_time
+        between start_events[i] and stop_events[i] is pass i's kernel duration."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(inp.device).cuda_stream
+        n = len(start_events)
+        st = (ctypes.c_void_p * n)(*[e.cuda_event for e in start_events])
+        sp = (ctypes.c_void_p * n)(*[e.cuda_event for e in stop_events])
+        _check(lib().mcfft_exec_timed(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, st, sp),
+               "mcfft_exec_timed")
 
     def destroy(self):
         if self._h and self._h.value:

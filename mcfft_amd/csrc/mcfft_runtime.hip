@@ -1,5 +1,6 @@
 // mcfft_runtime.hip -- the C ABI (include/mcfft.h): plan objects, FP64 table
 // generation on the GPU, and pass launches on the caller's stream.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
@@ -652,7 +653,7 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
 }
 
 static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
-                                int count);
+                                int count, void* const* start_events = nullptr, void* const* stop_events = nullptr);
 
 // R2C: complex FFT of length H into the plan's buffer, then the post-pass;
 // C2R: the pre-pass into the buffer, then the inverse complex FFT.
@@ -872,8 +873,20 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
   return exec_complex(plan, in, out, direction, stream, first, count);
 }
 
+mcfftStatus mcfft_exec_timed(mcfftPlan plan, const void* in, void* out, int direction, void* stream,
+                             void* const* start_events, void* const* stop_events) {
+  if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
+  if (!start_events || !stop_events) return MCFFT_INVALID_VALUE;
+  if (plan->core.n_chunks != 1) return MCFFT_UNSUPPORTED;
+  return exec_complex(plan, in, out, direction, stream, 0, (int)plan->core.passes.size(), start_events,
+                      stop_events);
+}
+
+// start_events / stop_events (one per pass, single-chunk plans): each pass is
+// launched with hipExtLaunchKernel, which records the events at the start and
+// the end of that kernel's execution.
 static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
-                                int count) {
+                                int count, void* const* start_events, void* const* stop_events) {
   if (first < 0 || count < 0 || first + count > (int)plan->core.passes.size()) return MCFFT_INVALID_VALUE;
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
@@ -930,9 +943,15 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
       a.n_tiles = (int)tiles;
       const long long grid = tiles < plan->grid_cap[pi] ? tiles : plan->grid_cap[pi];
       void* args[] = {&a};
-      if (hipLaunchKernel(pp.k->fn, dim3((unsigned)grid), dim3((unsigned)(pp.k->nw * 64)), args,
-                          (size_t)pp.k->lds_bytes, s) != hipSuccess)
+      if (start_events) {
+        if (hipExtLaunchKernel(pp.k->fn, dim3((unsigned)grid), dim3((unsigned)(pp.k->nw * 64)), args,
+                               (size_t)pp.k->lds_bytes, s, (hipEvent_t)start_events[pi],
+                               (hipEvent_t)stop_events[pi], 0) != hipSuccess)
+          return MCFFT_HIP_ERROR;
+      } else if (hipLaunchKernel(pp.k->fn, dim3((unsigned)grid), dim3((unsigned)(pp.k->nw * 64)), args,
+                                 (size_t)pp.k->lds_bytes, s) != hipSuccess) {
         return MCFFT_HIP_ERROR;
+      }
     }
   }
   return MCFFT_SUCCESS;
