@@ -24,6 +24,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #define MCFFT_DEV __device__ __forceinline__
 #define MCFFT_NOINLINE __device__ __attribute__((noinline))
 #define MCFFT_KERNEL(nt) extern "C" __global__ __launch_bounds__(nt)
+// a kernel whose registers must leave room for w waves per SIMD
+#define MCFFT_WAVES_PER_SIMD(w) __attribute__((amdgpu_waves_per_eu(w)))
 #define MCFFT_TKERNEL(nt) __global__ __launch_bounds__(nt)
 #define MCFFT_DYN_LDS(type, name) \
   extern __shared__ __attribute__((aligned(16))) unsigned char mcfft_lds_raw[]; \

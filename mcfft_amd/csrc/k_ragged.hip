@@ -3,8 +3,10 @@
 #include "kernel_list.h"
 #include "ragged.h"
 
-#define MCFFT_RAGGED_KERNEL(CM)                                                          \
-  MCFFT_KERNEL(mcfft::kRaggedNW * 64) void mcfft_ragged_##CM(mcfft::RaggedArgs r) {      \
+// two 8-wave workgroups per CU: at most 128 VGPRs (4 waves per SIMD)
+#define MCFFT_RAGGED_KERNEL(CM)                                                                   \
+  MCFFT_KERNEL(mcfft::kRaggedNW * 64) MCFFT_WAVES_PER_SIMD(4)                                       \
+  void mcfft_ragged_##CM(mcfft::RaggedArgs r) {                                                   \
     mcfft::ragged_body<CM>(r, mcfft::ragged_max_lds<CM>());                              \
   }
 MCFFT_RAGGED_KERNEL(0)

@@ -105,12 +105,14 @@ constexpr int ragged_lds_bytes() {
   return ragged_max_lds<CM>() + 16;
 }
 
-// Each variant is an out-of-line call, so its address arithmetic is not hoisted
-// into the task loop next to every other variant's (which would spill).  The
-// pass arguments are read from the descriptor in global memory through a
-// wave-uniform pointer (scalar loads), not passed through private memory.
+// Every variant is inlined into the task loop (an out-of-line call would keep a
+// call frame in scratch memory and raise the register count to the calling
+// convention's); fft_tile recomputes its index arithmetic per tile, so one
+// variant's addresses are not hoisted next to another's.  The pass arguments are
+// read from the descriptor in global memory through a wave-uniform pointer
+// (scalar loads).
 template <int S, int T, int Q, int NW, int CM, int KIND>
-MCFFT_NOINLINE void ragged_tile(const PassArgs* gpa, const void* in, void* out, int tile, AOps A) {
+MCFFT_DEV void ragged_tile(const PassArgs* gpa, const void* in, void* out, int tile, AOps A) {
   PassArgs a = *(const PassArgs*)uniform_ptr(gpa);
   a.in = uniform_ptr(in);
   a.out = (void*)uniform_ptr(out);

@@ -17,6 +17,7 @@
 #define MCFFT_DEV inline
 #define MCFFT_NOINLINE __attribute__((noinline))
 #define MCFFT_KERNEL(nt) extern "C"
+#define MCFFT_WAVES_PER_SIMD(w)
 #define MCFFT_TKERNEL(nt)
 #define MCFFT_DYN_LDS(type, name) type* name = (type*)::emu::lds_base()
 #define __restrict__
