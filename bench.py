@@ -102,20 +102,26 @@ def main():
         pf.exec(ins[i % n_rot], mid, mc.FORWARD)
         pf.exec(mid, out, mc.INVERSE)
 
-    def timed(fn, steps, warmup):
+    def timed(fn, steps, warmup, per_step=None):
+        """Mean ms per step over `steps` steps between two events; with
+        `per_step` (a list), also the duration of every step (an event at each
+        step boundary)."""
         for i in range(warmup):
             fn(i)
         torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(stream)
         for i in range(steps):
             fn(i)
-        e1.record(stream)
+            if per_step is not None or i == steps - 1:
+                ev[i + 1].record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
+        if per_step is not None:
+            per_step.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+        return ev[0].elapsed_time(ev[steps]) / steps
 
-    ms_plain = timed(step, args.steps, args.warmup)
+    step_ms = []
+    ms_plain = timed(step, args.steps, args.warmup, step_ms)
 
     # errors (plain): forward output of the first 8 transforms vs FP64, and the round trip
     pf.exec(ins[0], mid, mc.FORWARD)
@@ -257,6 +263,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_plain, 4),
+        "ms_per_step_median": round(float(np.median(step_ms)), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

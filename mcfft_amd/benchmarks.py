@@ -1,6 +1,6 @@
 """Throughput and accuracy of every workload the library is measured on.
 
-    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c6,ragged_small,real] [--reps 10]
+    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c6,ragged_small,real,layout] [--reps 10]
 
 One JSON line per (config, precision): GFLOP/s (5 N log2 N, N = elements per
 transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
@@ -8,7 +8,9 @@ transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
 and rel-L2 / max-abs error against an FP64 FFT of the same quantised input
 (computed on a sample of transforms where the full batch would take long on the
 host).  `ragged_small` compares the persistent ragged launches with per-shape
-batched launches over the same data, eagerly and captured in a hipGraph.
+batched launches (eagerly and captured in a hipGraph) and with one launch per
+transform, over the same data; `real` is the 1D R2C path and `layout` the
+planar-layout exec.
 
 The FP64 reference used for the error columns is the repository's CPU oracle
 (oracle/), imported only here, as the checker.
@@ -211,8 +213,9 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
 
 def ragged_small(torch, reps, fi="fp16", fo="fp16", seed=1):
     """205 mixed items (1D N = 2^k, k in 4..18, plus five 2D/3D shapes) as one
-    phased ragged launch, as per-shape batched launches, and both captured in a
-    hipGraph; the item set of the GPU ragged tests."""
+    phased ragged launch, as per-shape batched launches, as one launch per
+    transform, and the first two captured in a hipGraph; the item set of the GPU
+    ragged tests."""
     rng = np.random.default_rng(seed)
     shapes = [[int(2 ** k)] for k in rng.integers(4, 19, 200)]
     shapes += [[256, 512], [64, 64], [16, 16, 32], [1024, 256], [32, 2048]]
@@ -240,8 +243,18 @@ def ragged_small(torch, reps, fi="fp16", fo="fp16", seed=1):
         for p, o, cnt in plans:
             p.exec_ptr(d_in.data_ptr() + o * esz_in, d_out.data_ptr() + o * esz_out, mc.FORWARD, st)
 
+    # one launch per transform: a batch-1 plan per distinct shape, executed item by item
+    single = {s: mc.Plan(list(s), 1, fi, fo) for s in groups}
+    items = [(single[tuple(s)], int(o)) for s, o in zip(shapes, offs)]
+
+    def per_item(i):
+        st = torch.cuda.current_stream().cuda_stream
+        for p, o in items:
+            p.exec_ptr(d_in.data_ptr() + o * esz_in, d_out.data_ptr() + o * esz_out, mc.FORWARD, st)
+
     ms_r = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
     ms_p = _timer(torch, per_shape, reps)
+    ms_i = _timer(torch, per_item, max(2, reps // 4))
     graphs = {}
     for name, fn in (("ragged", lambda: rp.exec(d_in, d_out, mc.FORWARD)), ("per_shape", lambda: per_shape(0))):
         s = torch.cuda.Stream()
@@ -259,8 +272,12 @@ def ragged_small(torch, reps, fi="fp16", fo="fp16", seed=1):
            "ragged_graph_ms": round(graphs["ragged"], 4), "per_shape_graph_ms": round(graphs["per_shape"], 4),
            "speedup_vs_per_shape": round(ms_p / ms_r, 3),
            "speedup_graph_vs_per_shape_graph": round(graphs["per_shape"] / graphs["ragged"], 3),
+           "per_item_launches": len(items), "per_item_ms": round(ms_i, 4),
+           "speedup_vs_per_item": round(ms_i / ms_r, 3),
            "status_flags": flags}
     for p, _, _ in plans:
+        p.destroy()
+    for p in single.values():
         p.destroy()
     rp.destroy()
     return res
