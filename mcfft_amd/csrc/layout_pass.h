@@ -27,6 +27,7 @@ struct LayoutArgs {
   int lg[3];             // log2 of the (padded) dimensions
   int planar;
   int split;             // 1: 8-byte split elements / 4-byte planar scalars
+  int vec;               // 1: four consecutive elements per thread (layout_vec_ok)
 };
 
 MCFFT_DEV long long layout_offset(const LayoutArgs& a, long long e) {
@@ -86,12 +87,97 @@ MCFFT_DEV void layout_scatter_one(const LayoutArgs& a, long long e) {
   }
 }
 
+// Four consecutive elements e..e+3 (e a multiple of 4, so one innermost row:
+// dimensions are >= 16): with an innermost stride of 1 they are consecutive on
+// the caller's side too, and every access is a 8- to 32-byte vector.
+MCFFT_DEV void layout_gather_four(const LayoutArgs& a, long long e) {
+  const long long s = layout_offset(a, e);
+  if (!a.split) {
+    u32x4 v;
+    if (!a.planar) {
+      v = g_ld((const u32x4*)((const uint32_t*)a.src[0] + s), 0);
+    } else {
+      const u32x2 re = g_ld((const u32x2*)((const uint16_t*)a.src[0] + s), 0);
+      const u32x2 im = g_ld((const u32x2*)((const uint16_t*)a.src[1] + s), 0);
+      v.x = (re.x & 0xffffu) | (im.x << 16);
+      v.y = (re.x >> 16) | (im.x & 0xffff0000u);
+      v.z = (re.y & 0xffffu) | (im.y << 16);
+      v.w = (re.y >> 16) | (im.y & 0xffff0000u);
+    }
+    g_st((u32x4*)((uint32_t*)a.dst[0] + e), 0, v);
+  } else {
+    u32x4 v0, v1;  // elements e, e+1 and e+2, e+3 (x, y of each)
+    if (!a.planar) {
+      v0 = g_ld((const u32x4*)((const u32x2*)a.src[0] + s), 0);
+      v1 = g_ld((const u32x4*)((const u32x2*)a.src[0] + s + 2), 0);
+    } else {
+      const u32x4 re = g_ld((const u32x4*)((const uint32_t*)a.src[0] + s), 0);
+      const u32x4 im = g_ld((const u32x4*)((const uint32_t*)a.src[1] + s), 0);
+      v0.x = (re.x & 0xffffu) | (im.x << 16);
+      v0.y = (re.x >> 16) | (im.x & 0xffff0000u);
+      v0.z = (re.y & 0xffffu) | (im.y << 16);
+      v0.w = (re.y >> 16) | (im.y & 0xffff0000u);
+      v1.x = (re.z & 0xffffu) | (im.z << 16);
+      v1.y = (re.z >> 16) | (im.z & 0xffff0000u);
+      v1.z = (re.w & 0xffffu) | (im.w << 16);
+      v1.w = (re.w >> 16) | (im.w & 0xffff0000u);
+    }
+    g_st((u32x4*)((u32x2*)a.dst[0] + e), 0, v0);
+    g_st((u32x4*)((u32x2*)a.dst[0] + e + 2), 0, v1);
+  }
+}
+
+MCFFT_DEV void layout_scatter_four(const LayoutArgs& a, long long e) {
+  const long long d = layout_offset(a, e);
+  if (!a.split) {
+    const u32x4 v = g_ld((const u32x4*)((const uint32_t*)a.src[0] + e), 0);
+    if (!a.planar) {
+      g_st((u32x4*)((uint32_t*)a.dst[0] + d), 0, v);
+    } else {
+      u32x2 re, im;
+      re.x = (v.x & 0xffffu) | (v.y << 16);
+      re.y = (v.z & 0xffffu) | (v.w << 16);
+      im.x = (v.x >> 16) | (v.y & 0xffff0000u);
+      im.y = (v.z >> 16) | (v.w & 0xffff0000u);
+      g_st((u32x2*)((uint16_t*)a.dst[0] + d), 0, re);
+      g_st((u32x2*)((uint16_t*)a.dst[1] + d), 0, im);
+    }
+  } else {
+    const u32x4 v0 = g_ld((const u32x4*)((const u32x2*)a.src[0] + e), 0);
+    const u32x4 v1 = g_ld((const u32x4*)((const u32x2*)a.src[0] + e + 2), 0);
+    if (!a.planar) {
+      g_st((u32x4*)((u32x2*)a.dst[0] + d), 0, v0);
+      g_st((u32x4*)((u32x2*)a.dst[0] + d + 2), 0, v1);
+    } else {
+      u32x4 re, im;  // element k = (x, y): scalar re = x.lo | y.lo << 16, im = x.hi | y.hi << 16
+      re.x = (v0.x & 0xffffu) | (v0.y << 16);
+      re.y = (v0.z & 0xffffu) | (v0.w << 16);
+      re.z = (v1.x & 0xffffu) | (v1.y << 16);
+      re.w = (v1.z & 0xffffu) | (v1.w << 16);
+      im.x = (v0.x >> 16) | (v0.y & 0xffff0000u);
+      im.y = (v0.z >> 16) | (v0.w & 0xffff0000u);
+      im.z = (v1.x >> 16) | (v1.y & 0xffff0000u);
+      im.w = (v1.z >> 16) | (v1.w & 0xffff0000u);
+      g_st((u32x4*)((uint32_t*)a.dst[0] + d), 0, re);
+      g_st((u32x4*)((uint32_t*)a.dst[1] + d), 0, im);
+    }
+  }
+}
+
 constexpr int kLayoutThreads = 256;
 
 template <bool GATHER>
 MCFFT_DEV void layout_body(const LayoutArgs& a) {
   const long long step = (long long)grid_size() * kLayoutThreads;
-  for (long long e = (long long)block_id() * kLayoutThreads + thread_id(); e < a.total; e += step) {
+  const long long first = (long long)block_id() * kLayoutThreads + thread_id();
+  if (a.vec) {
+    for (long long q = first; q < a.total / 4; q += step) {
+      if constexpr (GATHER) layout_gather_four(a, 4 * q);
+      else layout_scatter_four(a, 4 * q);
+    }
+    return;
+  }
+  for (long long e = first; e < a.total; e += step) {
     if constexpr (GATHER) layout_gather_one(a, e);
     else layout_scatter_one(a, e);
   }
@@ -132,6 +218,16 @@ inline long long layout_extent(int rank, const long long* dims, long long batch,
   return m + 1;
 }
 
+// Four elements per thread when the side's innermost stride is 1, every group
+// of four starts on a multiple of 4 elements (other strides and the distance)
+// and the caller's arrays are aligned to four elements (scalars when planar).
+inline bool layout_vec_ok(const LayoutArgs& a, const void* p0, const void* p1) {
+  if (a.strides[2] != 1 || a.strides[0] % 4 || a.strides[1] % 4 || a.dist % 4) return false;
+  const size_t scalar = (size_t)(a.split ? 8 : 4) / (a.planar ? 2 : 1);
+  const size_t align = 4 * scalar < 16 ? 4 * scalar : 16;
+  return (size_t)p0 % align == 0 && (!p1 || (size_t)p1 % align == 0);
+}
+
 inline void layout_fill(LayoutArgs& a, int rank, const long long* dims, long long batch, const long long* strides,
                         long long dist, int planar, int fmt) {
   long long n = 1;
@@ -149,6 +245,7 @@ inline void layout_fill(LayoutArgs& a, int rank, const long long* dims, long lon
   a.total = n * batch;
   a.planar = planar;
   a.split = fmt == FMT_SPLIT ? 1 : 0;
+  a.vec = 0;
 }
 
 }  // namespace mcfft

@@ -705,7 +705,8 @@ static mcfftStatus exec_layout(mcfftPlan plan, const void* in_re, const void* in
   auto launch = [&](int sd, const void* k, LayoutArgs& la) -> mcfftStatus {
     layout_fill(la, plan->lay_rank, plan->lay_dims, plan->core.batch, plan->lay_strides[sd], plan->lay_dist[sd],
                 plan->lay_planar[sd], fmt[sd]);
-    const long long need = (la.total + kLayoutThreads - 1) / kLayoutThreads;
+    la.vec = layout_vec_ok(la, ptr[sd][0], ptr[sd][1]) ? 1 : 0;
+    const long long need = ((la.vec ? la.total / 4 : la.total) + kLayoutThreads - 1) / kLayoutThreads;
     const unsigned grid = (unsigned)(need < plan->lay_grid ? need : plan->lay_grid);
     void* args[] = {&la};
     return hipLaunchKernel(k, dim3(grid), dim3(kLayoutThreads), args, 0, s) == hipSuccess ? MCFFT_SUCCESS

@@ -247,6 +247,16 @@ def layout_positions(dims, batch, strides, dist) -> np.ndarray:
     return idx.reshape(-1)
 
 
+def layout_set_scalar_only(v: bool):
+    """Force the one-element-per-thread gather/scatter path (test hook)."""
+    lib().emu_layout_set_scalar_only(1 if v else 0)
+
+
+def layout_vec_used() -> bool:
+    """Whether the last layout_exec used the four-element path on a side."""
+    return bool(lib().emu_layout_vec_used())
+
+
 def layout_exec(x, dims, batch, in_layout, out_layout, in_fmt, out_fmt, scaling=1, inverse=False):
     """x: batch x prod(dims) complex.  in_layout / out_layout: (strides, dist,
     planar).  Returns (y as batch x prod(dims) complex times 2^e, output buffers,

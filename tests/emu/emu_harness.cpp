@@ -368,6 +368,13 @@ extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt,
 extern "C" void mcfft_layout_gather(LayoutArgs a);
 extern "C" void mcfft_layout_scatter(LayoutArgs a);
 
+// test hooks: force the one-element-per-thread path; whether the last exec used
+// the four-element path on either side
+static bool g_layout_scalar_only = false;
+static int g_layout_vec_used = 0;
+extern "C" void emu_layout_set_scalar_only(int v) { g_layout_scalar_only = v != 0; }
+extern "C" int emu_layout_vec_used() { return g_layout_vec_used; }
+
 extern "C" int emu_layout_exec(int rank, const long long* dims, long long batch, const long long* in_strides,
                                long long in_dist, int in_planar, const long long* out_strides, long long out_dist,
                                int out_planar, int in_fmt, int out_fmt, int scaling, int inverse, const void* in_re,
@@ -380,6 +387,7 @@ extern "C" int emu_layout_exec(int rank, const long long* dims, long long batch,
     if (v) return v;
   }
   if ((in_im != nullptr) != (in_planar != 0) || (out_im != nullptr) != (out_planar != 0)) return ST_INVALID_VALUE;
+  g_layout_vec_used = 0;
   long long n = 1;
   for (int d = 0; d < rank; ++d) n *= dims[d];
   bool dense[2];
@@ -396,6 +404,8 @@ extern "C" int emu_layout_exec(int rank, const long long* dims, long long batch,
     la.src[1] = s1;
     la.dst[0] = d0;
     la.dst[1] = d1;
+    la.vec = g_layout_scalar_only ? 0 : (layout_vec_ok(la, gather ? s0 : d0, gather ? s1 : d1) ? 1 : 0);
+    g_layout_vec_used |= la.vec;
     const long long ext = layout_extent(rank, dims, batch, st[sd], dist[sd]);
     const size_t side_bytes = (size_t)ext * (size_t)(elem_bytes(fmt[sd]) / (planar[sd] ? 2 : 1));
     const size_t dense_bytes = (size_t)(n * batch * elem_bytes(fmt[sd]));
@@ -409,7 +419,7 @@ extern "C" int emu_layout_exec(int rank, const long long* dims, long long batch,
       emu::register_range(d0, side_bytes);
       if (d1) emu::register_range(d1, side_bytes);
     }
-    const long long need = (la.total + kLayoutThreads - 1) / kLayoutThreads;
+    const long long need = ((la.vec ? la.total / 4 : la.total) + kLayoutThreads - 1) / kLayoutThreads;
     const int grid = (int)std::min<long long>(need, 5);  // fewer blocks than elements: grid-stride loop
     emu::launch(grid, kLayoutThreads, 0, [&] {
       if (gather) mcfft_layout_gather(la);

@@ -267,6 +267,26 @@ def test_layout_exec(case, inverse):
         assert np.all(w[mask] == 0x7C01)
 
 
+@pytest.mark.parametrize("case", [3, 4, 5])
+def test_layout_four_element_path(case):
+    """The four-element gather/scatter (innermost stride 1) moves the same bits
+    as the one-element path."""
+    dims, batch, li, lo, fi, fo = LAYOUT_CASES[case]
+    st, n = _dense(dims)
+    li = li or (st, n, 0)
+    lo = lo or (st, n, 0)
+    x = np.random.default_rng(11).standard_normal(n * batch) * (1 + 0.5j)
+    y4, _, _ = emu.layout_exec(x, dims, batch, li, lo, fi, fo)
+    assert emu.layout_vec_used()
+    emu.layout_set_scalar_only(True)
+    try:
+        y1, _, _ = emu.layout_exec(x, dims, batch, li, lo, fi, fo)
+        assert not emu.layout_vec_used()
+    finally:
+        emu.layout_set_scalar_only(False)
+    assert np.array_equal(y4, y1)
+
+
 def test_layout_planar_split_bits():
     """Planar split scalars are binary16 (hi, lo): the real-part array of a
     planar split output holds exactly the (re_hi, re_lo) halves of the
