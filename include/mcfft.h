@@ -89,6 +89,9 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
 /* Execute on `stream` (hipStream_t or NULL); asynchronous, no host sync.
  * direction: MCFFT_FORWARD (exp(-2 pi i kn/N)) or MCFFT_INVERSE (exp(+...),
  * unnormalised apart from the plan's scaling).  Capturable into a hipGraph.
+ * A plan's workspace, real-transform buffer, layout staging buffers and ragged
+ * queue counters belong to the plan: execs of one plan on different streams
+ * must not overlap in time (use one plan per stream).
  * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
 
