@@ -178,6 +178,15 @@ def main():
                 "frac": round(dom["alg_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 4),
                 "alg_bytes_per_launch": 8 * elems}
+    # MFMA work of the same kernel: 128 flop per complex element per radix-16
+    # stage (two 32x32x16 MFMAs per 16-point DFT of 32 columns); the C2 passes
+    # have two radix-16 stages each (256 = 16^2, 512 = 16^2 x 2 with a VALU
+    # radix-2 tail).  rocprofv3's SQ_INSTS_VALU_MFMA_MOPS_F16 x 512 gives the
+    # same count per launch.
+    mfma_flops = 128.0 * 2 * elems
+    mfma = {"flops_per_launch": mfma_flops, "achieved_tflops": round(mfma_flops / (dom["ms"] * 1e-3) / 1e12, 1),
+            "peak_tflops": 2500.0, "frac": round(mfma_flops / (dom["ms"] * 1e-3) / 1e12 / 2500.0, 4),
+            "kernel": dom["kernel"]}
 
     # split precision leg: forward FP16 -> split, inverse split -> split
     split = None
@@ -281,6 +290,7 @@ def main():
         "kernels": kernels,
         "ms_per_step_kernel_timed": round(ms_timed_step, 4),   # the steps of the per-kernel timing loop
         "roofline": roofline,
+        "mfma": mfma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
