@@ -70,7 +70,11 @@ def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transf
     rot = max(1, min(4, (2 * MALL) // max(1, d_in.numel() * d_in.element_size()) + 1))
     ins = [d_in] + [d_in.clone() for _ in range(rot - 1)]
     d_out = mc.empty(n * batch, fo)
-    ms = _timer(torch, lambda i: p.exec(ins[i % rot], d_out, direction), reps)
+    # pointers and stream resolved once, as for the rocFFT leg: a few-microsecond
+    # exec would otherwise time the Python-side lookups
+    st = torch.cuda.current_stream().cuda_stream
+    in_ptrs, out_ptr = [t.data_ptr() for t in ins], d_out.data_ptr()
+    ms = _timer(torch, lambda i: p.exec_ptr(in_ptrs[i % rot], out_ptr, direction, st), reps)
     flops = 5.0 * n * math.log2(n) * batch
     res = {"config": name, "dims": list(dims), "batch": batch, "in": fi, "out": fo, "scaling": scaling,
            "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
