@@ -10,6 +10,10 @@ FP64 FFT of the same quantised input.
 `value` is GFLOP/s with the 5 N log2 N convention over the forward + inverse
 transforms of the whole batch, inputs resident in HBM.  Inputs rotate over
 several buffers (> 2x the 256 MB Infinity Cache) so the timed region reads HBM.
+Warm-up is at least `--warmup` steps and at least `--warmup-ms` of GPU time
+(default 100 ms): the GPU's clocks take about 40 ms of sustained load to settle
+(a step from idle takes ~12 % longer), and every timed leg, rocFFT's included,
+is measured after the same warm-up.
 
 `roofline` is for the dominant kernel of the plain path: algorithmic bytes per
 launch (8 B per complex element: one FP16 read + one FP16 write, for every
@@ -41,6 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup-ms", type=float, default=100.0,
+                    help="minimum GPU time of the untimed warm-up before each timed leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
@@ -102,13 +108,31 @@ def main():
         pf.exec(ins[i % n_rot], mid, mc.FORWARD)
         pf.exec(mid, out, mc.INVERSE)
 
-    def timed(fn, steps, warmup, per_step=None):
-        """Mean ms per step over `steps` steps between two events; with
-        `per_step` (a list), also the duration of every step (an event at each
-        step boundary)."""
+    def warm(fn, warmup, min_ms):
+        """Untimed: `warmup` calls, then more (in groups of 10) until at least
+        `min_ms` of GPU time has passed."""
+        w0 = torch.cuda.Event(enable_timing=True)
+        w1 = torch.cuda.Event(enable_timing=True)
+        w0.record(stream)
         for i in range(warmup):
             fn(i)
+        w1.record(stream)
         torch.cuda.synchronize()
+        spent, i = w0.elapsed_time(w1), warmup
+        while spent < min_ms:
+            w0.record(stream)
+            for _ in range(10):
+                fn(i)
+                i += 1
+            w1.record(stream)
+            torch.cuda.synchronize()
+            spent += w0.elapsed_time(w1)
+
+    def timed(fn, steps, warmup, per_step=None, min_warm_ms=None):
+        """Mean ms per step over `steps` steps between two events, after the
+        warm-up above; with `per_step` (a list), also the duration of every step
+        (an event at each step boundary)."""
+        warm(fn, warmup, args.warmup_ms if min_warm_ms is None else min_warm_ms)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
         ev[0].record(stream)
         for i in range(steps):
@@ -150,6 +174,7 @@ def main():
             for e in row:
                 e.record(stream)    # creates the event
     torch.cuda.synchronize()
+    warm(step, 0, args.warmup_ms)   # the error check above left the GPU idle
     s0 = torch.cuda.Event(enable_timing=True)
     s1 = torch.cuda.Event(enable_timing=True)
     launches = []   # (kernel name, start event, stop event)
@@ -240,7 +265,7 @@ def main():
     # library (rocFFT, half precision, same data and step) for comparison
     ca = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
     cb = torch.empty_like(ca)
-    ms_copy = timed(lambda i: cb.copy_(ca), 10, 2)
+    ms_copy = timed(lambda i: cb.copy_(ca), 10, 2, min_warm_ms=0.0)
     copy_gbs = 2.0 * ca.numel() / (ms_copy * 1e-3) / 1e9
     del ca, cb
     vendor = None
@@ -271,6 +296,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_ms_min": args.warmup_ms,
         "ms_per_step": round(ms_plain, 4),
         "ms_per_step_median": round(float(np.median(step_ms)), 4),
         "higher_is_better": True,
