@@ -35,15 +35,26 @@ HBM = 8000.0  # GB/s
 MALL = 256 << 20
 
 
-def _timer(torch, fn, reps, warmup=2, min_ms=2.0):
+def _timer(torch, fn, reps, warmup=2, min_ms=2.0, warm_ms=60.0):
     """Mean time per call of `reps` back-to-back calls, with the rep count raised
     until the timed region lasts at least `min_ms` (a few-microsecond exec is
-    otherwise dominated by the jitter of its first launches)."""
-    for i in range(warmup):
-        fn(i)
-    torch.cuda.synchronize()
+    otherwise dominated by the jitter of its first launches).  Before timing,
+    at least `warmup` calls and `warm_ms` of GPU time run untimed: the GPU's
+    clocks settle after about 40 ms of sustained load."""
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    spent, n, i = 0.0, warmup, 0
+    while True:
+        e0.record()
+        for _ in range(n):
+            fn(i)
+            i += 1
+        e1.record()
+        torch.cuda.synchronize()
+        spent += e0.elapsed_time(e1)
+        if spent >= warm_ms:
+            break
+        n = min(4096, max(n * 2, 1))
     while True:
         e0.record()
         for i in range(reps):
