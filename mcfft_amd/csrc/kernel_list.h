@@ -33,6 +33,14 @@
   X(3, 1, 1, 4, CM, 0)            \
   X(3, 1, 2, 8, CM, 0)
 
+// split-mode alternatives for short rows (latency-bound small batches)
+#define MCFFT_ROW_TUNE_SPLIT(X) \
+  X(2, 1, 4, 2, 2, 0)           \
+  X(2, 1, 4, 4, 2, 0)           \
+  X(2, 1, 16, 8, 2, 0)          \
+  X(2, 2, 2, 2, 2, 0)           \
+  X(2, 2, 8, 8, 2, 0)
+
 // single-pass 2^15 (LDS 139 KiB at 4 B per element; not for split)
 #define MCFFT_ROW_LIST(X)   \
   MCFFT_ROW_LIST_CM(X, 0)   \
@@ -41,7 +49,8 @@
   X(3, 8, 1, 16, 0, 0)      \
   X(3, 8, 1, 16, 1, 0)      \
   MCFFT_ROW_TUNE_CM(X, 0)   \
-  MCFFT_ROW_TUNE_CM(X, 1)
+  MCFFT_ROW_TUNE_CM(X, 1)   \
+  MCFFT_ROW_TUNE_SPLIT(X)
 
 // ---- ROWT: Q rows stored transposed (segments of Q elements) ----
 #define MCFFT_ROWT_LIST_CM(X, CM, QB)  \
