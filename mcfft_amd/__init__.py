@@ -105,6 +105,23 @@ def _check(st: int, what: str):
         raise McfftError(st, what)
 
 
+ELEM_BYTES = {"fp16": 4, "bf16": 4, "split": 8}
+
+
+def _check_tensor(t, nbytes: int, what: str):
+    """The C ABI takes bare pointers: a device tensor smaller than what the plan
+    reads or writes would be overrun by the kernels, so check it here."""
+    if t is None:
+        raise ValueError(f"{what}: missing tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{what}: not a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: not contiguous")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise ValueError(f"{what}: {have} bytes, the plan needs {nbytes}")
+
+
 class Plan:
     """A plan for `batch` transforms of shape `dims` (1 to 3 dims, last contiguous)."""
 
@@ -137,12 +154,28 @@ class Plan:
         return buf.value.decode()
 
     def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """mcfft_exec on raw device pointers (no size checks)."""
         _check(lib().mcfft_exec(self._h, in_ptr, out_ptr, direction, stream), "mcfft_exec")
+
+    def io_bytes(self):
+        """(input bytes, output bytes) of the contiguous transform: what exec
+        reads and writes, and what tune / exec_passes / exec_timed use."""
+        return self.n * self.batch * ELEM_BYTES[self.in_fmt], self.n * self.batch * ELEM_BYTES[self.out_fmt]
+
+    def exec_io_bytes(self):
+        """(input bytes, output bytes) that exec reads and writes."""
+        return self.io_bytes()
+
+    def _check_io(self, inp, out, contiguous=False):
+        ib, ob = self.io_bytes() if contiguous else self.exec_io_bytes()
+        _check_tensor(inp, ib, "input")
+        _check_tensor(out, ob, "output")
 
     def exec(self, inp, out, direction: int = FORWARD, stream=None):
         """Run on torch device tensors (storage layout above) on `stream`
         (default: torch's current stream)."""
         import torch
+        self._check_io(inp, out)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         elif hasattr(stream, "cuda_stream"):
@@ -153,6 +186,7 @@ class Plan:
         """Time every compiled kernel variant of each pass on these buffers, keep
         the fastest and record it in the tuning cache (blocking; overwrites out)."""
         import torch
+        self._check_io(inp, out, contiguous=True)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         _check(lib().mcfft_plan_tune(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, reps),
@@ -166,6 +200,7 @@ class Plan:
     def exec_passes(self, inp, out, direction: int, first: int, count: int, stream=None):
         """Launch only passes [first, first+count) (profiling; leaves `out` incomplete)."""
         import torch
+        self._check_io(inp, out, contiguous=True)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         _check(lib().mcfft_exec_passes(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, first, count),
@@ -176,6 +211,7 @@ class Plan:
         torch.cuda.Events (already created, e.g. recorded once): elapsed_time
         between start_events[i] and stop_events[i] is pass i's kernel duration."""
         import torch
+        self._check_io(inp, out, contiguous=True)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         n = len(start_events)
@@ -218,6 +254,12 @@ class RaggedPlan(Plan):
         self.exponents = list(ex)
         self.exponent = None
 
+    def io_bytes(self):
+        """Bytes up to the end of the furthest item, in and out."""
+        ie = max(io + int(np.prod(d)) for d, io, _ in self.items)
+        oe = max(oo + int(np.prod(d)) for d, _, oo in self.items)
+        return ie * ELEM_BYTES[self.in_fmt], oe * ELEM_BYTES[self.out_fmt]
+
     def status_flags(self) -> int:
         f = ctypes.c_uint()
         _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
@@ -243,6 +285,12 @@ class RealPlan(Plan):
         self.exponent = e.value
         self.n = int(n)
 
+    def exec_io_bytes(self):
+        h, c = self.n // 2, (self.n // 2 + 1)
+        if self.kind == "r2c":
+            return h * self.batch * ELEM_BYTES[self.in_fmt], c * self.batch * ELEM_BYTES[self.out_fmt]
+        return c * self.batch * ELEM_BYTES[self.in_fmt], h * self.batch * ELEM_BYTES[self.out_fmt]
+
 
 class LayoutPlan(Plan):
     """A plan reading and writing strided and/or planar layouts (Layout, or
@@ -265,9 +313,27 @@ class LayoutPlan(Plan):
         self.exponent = e.value
         self.n = int(np.prod(dims))
 
+    def _side_bytes(self, lay, fmt):
+        """Bytes of one array of a side: its extent in elements (interleaved)
+        or scalars (planar: half an element)."""
+        if lay is None:
+            return self.n * self.batch * ELEM_BYTES[fmt]
+        ext = (self.batch - 1) * lay.dist + sum((d - 1) * lay.strides[i] for i, d in enumerate(self.dims)) + 1
+        return ext * ELEM_BYTES[fmt] // (2 if lay.planar else 1)
+
+    def exec_io_bytes(self):
+        return self._side_bytes(self.in_layout, self.in_fmt), self._side_bytes(self.out_layout, self.out_fmt)
+
     def exec_layout(self, in_re, in_im, out_re, out_im, direction: int = FORWARD, stream=None):
         """torch device tensors; *_im None for interleaved sides."""
         import torch
+        ib, ob = self.exec_io_bytes()
+        _check_tensor(in_re, ib, "input (re)")
+        _check_tensor(out_re, ob, "output (re)")
+        if in_im is not None:
+            _check_tensor(in_im, ib, "input (im)")
+        if out_im is not None:
+            _check_tensor(out_im, ob, "output (im)")
         if stream is None:
             stream = torch.cuda.current_stream(in_re.device).cuda_stream
         p = lambda t: t.data_ptr() if t is not None else None

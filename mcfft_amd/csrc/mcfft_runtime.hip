@@ -842,10 +842,7 @@ mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, cons
 mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!exponents) return MCFFT_INVALID_VALUE;
-  if (!plan->ragged) {
-    exponents[0] = plan->core.out_exponent;
-    return MCFFT_SUCCESS;
-  }
+  if (!plan->ragged) return mcfft_get_output_exponent(plan, exponents);
   for (int i = 0; i < plan->rcore.n_items; ++i) exponents[i] = plan->rcore.exponent[i];
   return MCFFT_SUCCESS;
 }

@@ -77,3 +77,18 @@ def test_r2c_then_c2r_round_trip(torch_cuda):
     pf.destroy()
     pi.destroy()
     assert oracle.rel_l2(y.reshape(-1).astype(np.complex128), xq.reshape(-1).astype(np.complex128)) <= 1e-6
+
+
+def test_real_plan_exponent_queries_agree(torch_cuda):
+    """mcfft_get_item_exponents and mcfft_get_output_exponent report the same
+    exponent for real plans (C2R carries one more power of two)."""
+    import ctypes
+    for kind in ("r2c", "c2r"):
+        p = mc.RealPlan(4096, 4, kind, "fp16", "fp16")
+        e = ctypes.c_int()
+        assert mc.lib().mcfft_get_item_exponents(p._h, ctypes.byref(e)) == 0
+        assert e.value == p.exponent
+        q = mc.Plan([2048], 4, "fp16", "fp16")
+        assert p.exponent == q.exponent + (1 if kind == "c2r" else 0)
+        p.destroy()
+        q.destroy()

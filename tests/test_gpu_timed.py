@@ -50,3 +50,24 @@ def test_exec_timed_rejects_chunked_plans(torch_cuda, monkeypatch):
         p.exec_timed(buf, buf, mc.FORWARD, ev, ev)
     assert e.value.status == 2
     p.destroy()
+
+
+def test_python_binding_rejects_undersized_tensors(torch_cuda):
+    """A tensor smaller than the plan's input or output is refused before any
+    kernel is launched (the C ABI itself only sees pointers)."""
+    torch = torch_cuda
+    p = mc.Plan([4096], 8, "fp16", "split")
+    good_in, good_out = mc.empty(4096 * 8, "fp16"), mc.empty(4096 * 8, "split")
+    with pytest.raises(ValueError, match="output"):
+        p.exec(good_in, mc.empty(4096 * 8, "fp16"))          # split output needs 8 B per element
+    with pytest.raises(ValueError, match="input"):
+        p.exec(good_in[:100], good_out)
+    with pytest.raises(ValueError, match="contiguous"):
+        p.exec(good_in.t(), good_out)
+    p.exec(good_in, good_out)
+    torch.cuda.synchronize()
+    r = mc.RealPlan(4096, 4, "r2c", "fp16", "fp16")
+    with pytest.raises(ValueError, match="output"):
+        r.exec(mc.empty(2048 * 4, "fp16"), mc.empty(2048 * 4, "fp16"))   # needs 2049 per transform
+    p.destroy()
+    r.destroy()
