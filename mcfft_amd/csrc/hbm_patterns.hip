@@ -9,7 +9,8 @@
 //                4W bytes, rows 16 KiB apart), loads first then stores, as a
 //                column pass does; W = 8, 16, 32, 64, with and without the
 //                XCD-grouped tile order of the column passes
-// Prints one line per pattern: time per copy and GB/s (read + write bytes).
+// Prints one line per pattern: time per copy and GB/s (read + write bytes),
+// after 200 ms of warm-up load.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -105,6 +106,20 @@ int main() {
   char nm[128];
   const long long n16 = bytes / 16;
   printf("CUs %d\n", ncu);
+  // the GPU's clocks settle after about 40 ms of sustained load: copy for
+  // 200 ms before the first measurement (the patterns then run back to back)
+  {
+    float spent = 0;
+    while (spent < 200.0f && !bad) {
+      bad |= hipEventRecord(e0, 0) != hipSuccess;
+      for (int r = 0; r < 20; ++r) copy_grid_stride<1><<<(unsigned)(n16 / 256), 256>>>(a, b, n16);
+      bad |= hipEventRecord(e1, 0) != hipSuccess;
+      bad |= hipEventSynchronize(e1) != hipSuccess;
+      float ms = 0;
+      bad |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+      spent += ms;
+    }
+  }
   timeit("contiguous 1 GiB, flat (4 KiB per workgroup)", bytes,
          [&] { copy_grid_stride<1><<<(unsigned)(n16 / 256), 256>>>(a, b, n16); });
   for (int threads : {256, 512, 1024})

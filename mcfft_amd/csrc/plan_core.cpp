@@ -44,6 +44,12 @@ static const PreferredShape kPreferred[] = {
     {KIND_ROW, 4, 1, 0, 1, 16},  {KIND_ROW, 4, 1, 1, 1, 16},   // single-pass 65536: 16 waves
 };
 
+// A column tile's buffer view spans (L - 1) * stride + Q elements: its byte
+// offsets must fit the 32-bit buffer addressing.
+static bool col_view_fits(long long L, long long stride, long long Q, int cm) {
+  return ((L - 1) * stride + Q) * (cm == 2 ? 8 : 4) <= (1LL << 32);
+}
+
 static const KernelInfo* find_kernel(int S, int T, int cm, int kind) {
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
@@ -160,7 +166,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
     const KernelInfo* k = find_kernel(S, T, cm, KIND_COL);
-    if (!k || ncols % k->Q != 0 || stride * L > 0x7fffffffLL) return ST_UNSUPPORTED;
+    if (!k || ncols % k->Q != 0 || !col_view_fits(L, stride, k->Q, cm)) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
     pp.kind = KIND_COL;

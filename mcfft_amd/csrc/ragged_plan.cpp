@@ -58,7 +58,8 @@ int build_shape(RaggedPlanCore& r, int rank, const long long* dims, long long ro
     const int v = find_variant(cm, S, T, K_COL);
     if (v < 0) return ST_UNSUPPORTED;
     const KernelInfo& k = kRaggedVariantInfo[cm][v];
-    if (ncols % k.Q != 0 || stride * L > 0x7fffffffLL) return ST_UNSUPPORTED;
+    // the tile's view spans (L - 1) * stride + Q elements of 32-bit byte offsets
+    if (ncols % k.Q != 0 || ((L - 1) * stride + k.Q) * (cm == 2 ? 8 : 4) > (1LL << 32)) return ST_UNSUPPORTED;
     RaggedPassHost h;
     h.variant = v;
     h.pp.src = src;

@@ -321,3 +321,13 @@ def test_layout_invalid():
     assert L.emu_layout_exec(1, dims, 1, good, 256, 1, good, 256, 0, 0, 0, 1, 0, p, None, p, None, ctypes.byref(e)) == 1
     d3 = (ctypes.c_longlong * 1)(100)
     assert L.emu_layout_exec(1, d3, 1, good, 256, 0, good, 256, 0, 0, 0, 1, 0, p, None, p, None, ctypes.byref(e)) == 2
+
+
+def test_column_views_fit_32bit_byte_offsets():
+    """A column pass addresses (L - 1) * stride + Q elements through one 32-bit
+    buffer view: 1024^3 fits in FP16 (4 GiB) but not in split (8 GiB), which
+    the planner refuses instead of producing a truncated view."""
+    assert "pass" in emu.describe([1024, 1024, 1024], 1, emu.FMT_F16, emu.FMT_F16)
+    with pytest.raises(RuntimeError, match="status 2"):
+        emu.describe([1024, 1024, 1024], 1, emu.FMT_F16, emu.FMT_SPLIT)
+    assert "pass" in emu.describe([512, 1024, 1024], 1, emu.FMT_F16, emu.FMT_SPLIT)
