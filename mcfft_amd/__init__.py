@@ -197,6 +197,13 @@ class Plan:
         _check(lib().mcfft_get_pass_count(self._h, ctypes.byref(c)), "mcfft_get_pass_count")
         return c.value
 
+    def status_flags(self) -> int:
+        """Status bits of the last exec's persistent kernel (ragged plans; 0 = fine,
+        always 0 for other plans; waits for the device)."""
+        f = ctypes.c_uint()
+        _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
+        return f.value
+
     def exec_passes(self, inp, out, direction: int, first: int, count: int, stream=None):
         """Launch only passes [first, first+count) (profiling; leaves `out` incomplete)."""
         import torch
@@ -259,11 +266,6 @@ class RaggedPlan(Plan):
         ie = max(io + int(np.prod(d)) for d, io, _ in self.items)
         oe = max(oo + int(np.prod(d)) for d, _, oo in self.items)
         return ie * ELEM_BYTES[self.in_fmt], oe * ELEM_BYTES[self.out_fmt]
-
-    def status_flags(self) -> int:
-        f = ctypes.c_uint()
-        _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
-        return f.value
 
 
 class RealPlan(Plan):

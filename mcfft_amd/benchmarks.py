@@ -1,15 +1,17 @@
 """Throughput and accuracy of every workload the library is measured on.
 
-    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c6,ragged_small,real,layout] [--reps 10]
+    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c5,c6,ragged_small,real,layout] [--reps 10]
 
 One JSON line per (config, precision): GFLOP/s (5 N log2 N, N = elements per
 transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
 (one read of the input + one write of the output in their storage formats),
 and rel-L2 / max-abs error against an FP64 FFT of the same quantised input
 (computed on a sample of transforms where the full batch would take long on the
-host).  `ragged_small` compares the persistent ragged launches with per-shape
-batched launches (eagerly and captured in a hipGraph) and with one launch per
-transform, over the same data; `real` is the 1D R2C path and `layout` the
+host).  `c5` is the 20064-item ragged batch (about 1e9 elements) as phased
+ragged launches, as one launch with in-kernel dependencies and as per-shape
+batched launches; `ragged_small` compares the persistent ragged launches with
+per-shape batched launches (eagerly and captured in a hipGraph) and with one
+launch per transform, over the same data; `real` is the 1D R2C path and `layout` the
 planar-layout exec.
 
 The FP64 reference used for the error columns is the repository's CPU oracle
@@ -155,14 +157,13 @@ def config_inputs(cfg, rng, dims, batch):
 
 def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
     """20000 1D transforms, N = 2^k with k uniform in 8..18, plus 64 2D transforms
-    with sides from {256, 512, 1024, 2048}; laid out grouped by shape.
+    with sides from {256, 512, 1024, 2048}; laid out grouped by shape.  Timed as
+    phased ragged launches (the default), as one launch with in-kernel pass
+    dependencies (MCFFT_RAGGED_SINGLE=1), and as per-shape batched launches;
+    progress goes to stderr."""
+    def progress(msg):
+        print(f"c5 {fi}->{fo}: {msg}", file=sys.stderr, flush=True)
 
-    Disabled: its one GPU run stopped producing output for minutes (no fault
-    reported) and the cause has not been shown; the ragged plan of this exact
-    item list passes the structural checks of tests/emu and its kernels pass the
-    emulated parity tests.  `ragged_small` below measures the ragged path on the
-    item set of the GPU ragged tests instead."""
-    raise RuntimeError("c5 (20064-item ragged batch) is disabled until its stall on the GPU is explained")
     ks = rng.integers(8, 19, 20000)
     shapes = [[int(2 ** k)] for k in ks]
     sides = [256, 512, 1024, 2048]
@@ -178,12 +179,15 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
     d_in = torch.randn((total, 2), dtype=torch.float16, device="cuda", generator=gen)
     d_out = mc.empty(total, fo)
     rp = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], fi, fo)
+    progress(f"{len(shapes)} items, {total} elements planned")
     ms_r = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
     flags = rp.status_flags()
+    progress(f"phased {ms_r:.3f} ms")
     # the same work as ONE launch with in-kernel pass dependencies
     os.environ["MCFFT_RAGGED_SINGLE"] = "1"
     ms_single = _timer(torch, lambda i: rp.exec(d_in, d_out, mc.FORWARD), reps)
     flags |= rp.status_flags()
+    progress(f"single launch {ms_single:.3f} ms")
     del os.environ["MCFFT_RAGGED_SINGLE"]
     # per-shape batched launches over the same buffers
     groups = {}
@@ -202,6 +206,7 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
                        torch.cuda.current_stream().cuda_stream)
 
     ms_p = _timer(torch, per_shape, reps)
+    progress(f"per-shape {ms_p:.3f} ms")
     # accuracy on a sample of items (ragged output)
     import oracle
     rp.exec(d_in, d_out, mc.FORWARD)
@@ -369,7 +374,7 @@ def bench_layout(torch, reps, n=131072, batch=256):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c1,c2,c3,c4,c6,ragged_small,real,layout")
+    ap.add_argument("--config", default="c1,c2,c3,c4,c5,c6,ragged_small,real,layout")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     import torch

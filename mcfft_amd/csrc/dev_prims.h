@@ -109,6 +109,16 @@ MCFFT_DEV void atomic_or_dev(unsigned* p, unsigned v) {
 MCFFT_DEV void fence_dev() { __threadfence(); }
 MCFFT_DEV void nap() { __builtin_amdgcn_s_sleep(4); }
 
+// Work done for the whole workgroup by one wave inside a loop that also holds
+// barriers: branch on wave_id() (wave-uniform, a scalar branch) and let every
+// lane of that wave run the atomics, lane 0 with the real operand and the
+// others with 0, reading results through wave_first.  A branch on the thread
+// index instead (tid == 0) is divergent inside the wave, and the compiler may
+// then turn the loop into a nest whose inner loop runs the other lanes through
+// further iterations (and barriers) while lane 0 waits outside it.
+MCFFT_DEV int wave_id() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6); }
+MCFFT_DEV unsigned wave_first(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+
 // pack the low halves / high halves of two packed pairs into one pair (v_perm_b32)
 MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x05040100u); }
 MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x07060302u); }

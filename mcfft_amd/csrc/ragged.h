@@ -135,24 +135,29 @@ MCFFT_DEV void ragged_dispatch(int variant, const PassArgs* gpa, const void* in,
 
 template <int CM>
 MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
-  const int tid = thread_id();
+  // queue fetches, waits and counts are done by wave 0 (see wave_id in dev_prims.h)
+  const int wid = wave_id();
+  const unsigned one = lane_id() == 0 ? 1u : 0u;
   const AOps A = load_aops(r.aops, lane_id());
   MCFFT_DYN_LDS(unsigned char, ldsb);
   int* slot = (int*)(ldsb + task_slot_offset);
   unsigned* ctr = r.counters + kRaggedCounterBase;
   for (;;) {
-    if (tid == 0) lds_st(slot, 0, r.task_begin + (int)atomic_add_dev(&r.counters[r.queue], 1u));
+    if (wid == 0) lds_st(slot, 0, r.task_begin + (int)wave_first(atomic_add_dev(&r.counters[r.queue], one)));
     block_barrier();
     const int ti = uniform(lds_ld(slot, 0));
     if (ti >= r.task_end) break;  // every workgroup leaves through here
     const RaggedTask t = r.tasks[ti];
     const RaggedPass* rp = (const RaggedPass*)uniform_ptr(r.passes + t.pass);
-    if (r.use_deps && rp->wait_for > 0) {
-      if (tid == 0) {
+    // the descriptor's fields are the same for every lane: read as wave-uniform
+    // values, so the branches on them are scalar branches
+    const int wait_for = uniform(rp->wait_for), counter = uniform(rp->counter), signal = uniform(rp->signal);
+    if (r.use_deps && wait_for > 0) {
+      if (wid == 0) {
         unsigned it = 0;
-        while (atomic_load_dev(&ctr[rp->counter]) < (unsigned)rp->wait_for) {
+        while (wave_first(atomic_load_dev(&ctr[counter])) < (unsigned)wait_for) {
           if (++it > r.spin_limit) {
-            atomic_or_dev(&r.counters[4], 1u);
+            atomic_or_dev(&r.counters[4], one);
             break;
           }
           nap();
@@ -166,10 +171,10 @@ MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
                      (size_t)rp->args.in;
     char* out = (ob == 1 ? r.out_base : r.ws_base) + (size_t)rp->args.out;
     ragged_dispatch<CM>(rp->variant, &rp->args, in, out, t.tile, A);  // ends with a barrier
-    if (r.use_deps && rp->signal) {
+    if (r.use_deps && signal) {
       fence_dev();
       block_barrier();
-      if (tid == 0) atomic_add_dev(&ctr[rp->counter], 1u);
+      if (wid == 0) atomic_add_dev(&ctr[counter], one);
     }
   }
 }

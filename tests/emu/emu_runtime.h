@@ -193,6 +193,17 @@ inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, _
 inline void atomic_or_dev(unsigned* p, unsigned v) { (void)__atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline void fence_dev() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline void nap() { std::this_thread::yield(); }
+inline int wave_id() { return emu::tl_tid / 64; }
+// lane 0's value, for every lane of the wave (all 64 lanes take part)
+inline unsigned wave_first(unsigned v) {
+  emu::Block* b = emu::tl_blk;
+  const int t = emu::tl_tid, w = t / 64, l = t % 64;
+  b->xs[(size_t)w * 64 + l] = __builtin_bit_cast(float, v);
+  b->wbar[w]->arrive_and_wait();
+  const unsigned r = __builtin_bit_cast(unsigned, b->xs[(size_t)w * 64]);
+  b->wbar[w]->arrive_and_wait();
+  return r;
+}
 
 inline uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
 inline uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }

@@ -48,6 +48,30 @@ def test_ragged_matches_oracle(torch_cuda, fmt, sorted_layout):
     assert worst <= TOL[fmt]
 
 
+@pytest.mark.parametrize("fmt", ["fp16", "split"])
+def test_ragged_single_launch_matches_phased(torch_cuda, monkeypatch, fmt):
+    """MCFFT_RAGGED_SINGLE=1: every pass in one launch, later-pass tiles waiting on
+    their transform's tile counter -- the same tiles and arithmetic as the phased
+    launches, so the same bits; no wait may run out (status 0)."""
+    torch = torch_cuda
+    shapes = _mixed(1)
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    print(f"ragged single launch {fmt}: {len(shapes)} items", flush=True)
+    x = np.random.default_rng(4).standard_normal(total) * (1 + 1j)
+    in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else ("fp16", "fp16")
+    p = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], in_fmt, out_fmt)
+    d_in = mc.to_device(x, in_fmt)
+    phased, single = mc.empty(total, out_fmt), mc.empty(total, out_fmt)
+    p.exec(d_in, phased, mc.FORWARD)
+    monkeypatch.setenv("MCFFT_RAGGED_SINGLE", "1")
+    p.exec(d_in, single, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    assert torch.equal(phased.view(torch.int16), single.view(torch.int16))
+
+
 def test_ragged_inverse_and_gaps(torch_cuda):
     """Items at arbitrary (gapped, out-of-order) offsets, inverse direction."""
     torch = torch_cuda
