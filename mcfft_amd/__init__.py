@@ -260,12 +260,19 @@ class RaggedPlan(Plan):
         _check(lib().mcfft_get_item_exponents(self._h, ex), "mcfft_get_item_exponents")
         self.exponents = list(ex)
         self.exponent = None
+        # extents checked on every exec: computed once here (an O(items) loop per
+        # exec would leave the GPU idle for milliseconds at tens of thousands of items)
+        ie = oe = 0
+        for d, io, oo in self.items:
+            n = 1
+            for v in d:
+                n *= int(v)
+            ie, oe = max(ie, io + n), max(oe, oo + n)
+        self._io_bytes = (ie * ELEM_BYTES[in_fmt], oe * ELEM_BYTES[out_fmt])
 
     def io_bytes(self):
         """Bytes up to the end of the furthest item, in and out."""
-        ie = max(io + int(np.prod(d)) for d, io, _ in self.items)
-        oe = max(oo + int(np.prod(d)) for d, _, oo in self.items)
-        return ie * ELEM_BYTES[self.in_fmt], oe * ELEM_BYTES[self.out_fmt]
+        return self._io_bytes
 
 
 class RealPlan(Plan):
