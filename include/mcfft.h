@@ -152,7 +152,11 @@ mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_
  * level (at most three: every item's first passes, then second, then third)
  * over a device-side work queue, plus a tiny queue-reset kernel; the whole batch
  * is one graph-capturable exec.  Back-to-back 1D items of one length up to
- * 4096 are processed as one multi-row unit.  Items may have different output
+ * 4096 are processed as one multi-row unit.  A run of consecutive items of one
+ * shape lying back to back in both buffers with at least 2^23 elements in all
+ * (enough to fill the GPU; environment MCFFT_RAGGED_BULK=<elements> changes
+ * the bound, 0 turns it off) is run as one batched plan instead, launched on
+ * the same stream before the queue.  Items may have different output
  * exponents: see mcfft_get_item_exponents.  No reference counterpart. */
 typedef struct mcfftRaggedItem_t {
   int rank;

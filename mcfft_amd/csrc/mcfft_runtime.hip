@@ -43,6 +43,15 @@ struct mcfftPlan_st {
   RaggedTask* rtasks = nullptr;
   unsigned* rcounters = nullptr;
   int rgrid = 0;
+  // runs of same-shape items large enough to fill the GPU on their own go to a
+  // batched plan each (bulk); the rest to the persistent queue (has_queue)
+  struct Bulk {
+    mcfftPlan_st* plan;
+    long long in_off, out_off;  // elements
+  };
+  std::vector<Bulk> bulk;
+  bool has_queue = false;
+  std::vector<int> item_exp;    // per item, from whichever path runs it
   // real-input / real-output plans: `core` is the length-H complex plan
   int real_kind = 0;          // 0 complex, 1 R2C, 2 C2R
   long long real_h = 0;       // H = N / 2
@@ -277,6 +286,7 @@ void free_plan(mcfftPlan p) {
     if (p->rpasses[d]) (void)hipFree(p->rpasses[d]);
   if (p->rtasks) (void)hipFree(p->rtasks);
   if (p->rcounters) (void)hipFree(p->rcounters);
+  for (auto& b : p->bulk) free_plan(b.plan);
   if (p->real_tmp) (void)hipFree(p->real_tmp);
   for (int d = 0; d < 2; ++d)
     if (p->real_tw[d]) (void)hipFree(p->real_tw[d]);
@@ -512,25 +522,82 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
                               mcfftFormat out_fmt, mcfftScaling scaling) {
   if (!plan || !items || n_items < 1) return MCFFT_INVALID_VALUE;
   *plan = nullptr;
-  std::vector<RaggedItemSpec> spec(n_items);
-  for (int i = 0; i < n_items; ++i) {
-    spec[i].rank = items[i].rank;
-    for (int d = 0; d < 3; ++d) spec[i].dims[d] = items[i].dims[d];
-    spec[i].in_off = items[i].in_offset;
-    spec[i].out_off = items[i].out_offset;
-  }
+  const int cm = compute_mode((int)in_fmt, (int)out_fmt);
+  if (cm < 0) return (int)in_fmt < 0 || (int)in_fmt > 2 || (int)out_fmt < 0 || (int)out_fmt > 2 ? MCFFT_INVALID_VALUE
+                                                                                             : MCFFT_UNSUPPORTED;
   mcfftPlan p = new (std::nothrow) mcfftPlan_st();
   if (!p) return MCFFT_ALLOC_FAILED;
   p->ragged = true;
-  int st = ragged_build(p->rcore, n_items, spec.data(), (int)in_fmt, (int)out_fmt, (int)scaling);
+  p->core.in_fmt = (int)in_fmt;
+  p->core.out_fmt = (int)out_fmt;
+  p->core.cm = cm;
+  p->item_exp.assign((size_t)n_items, 0);
+  // Bulk runs: consecutive items of one shape lying back to back in both buffers,
+  // at least MCFFT_RAGGED_BULK elements together (default 2^23, enough to fill the
+  // GPU; 0 = none), run as one batched plan each.  The other items (and any run
+  // whose batched plan fails) go to the persistent queue.
+  long long bulk_min = 1LL << 23;
+  if (const char* e = getenv("MCFFT_RAGGED_BULK")) bulk_min = atoll(e);
+  std::vector<RaggedItemSpec> spec;
+  std::vector<int> rem_idx;
+  auto same_shape = [&](const mcfftRaggedItem& a, const mcfftRaggedItem& b) {
+    if (a.rank != b.rank) return false;
+    for (int d = 0; d < a.rank; ++d)
+      if (a.dims[d] != b.dims[d]) return false;
+    return true;
+  };
+  for (int i = 0; i < n_items;) {
+    const mcfftRaggedItem& a = items[i];
+    long long n = 1;
+    bool ok = a.rank >= 1 && a.rank <= 3 && a.in_offset >= 0 && a.out_offset >= 0;
+    for (int d = 0; ok && d < a.rank; ++d) {
+      ok = a.dims[d] >= 1;
+      n *= a.dims[d];
+    }
+    int j = i + 1;
+    if (ok)
+      while (j < n_items && same_shape(items[j], a) && items[j].in_offset == items[j - 1].in_offset + n &&
+             items[j].out_offset == items[j - 1].out_offset + n)
+        ++j;
+    if (ok && bulk_min > 0 && (long long)(j - i) * n >= bulk_min) {
+      mcfftPlan sub = nullptr;
+      const long long dims[3] = {a.dims[0], a.dims[1], a.dims[2]};
+      int e = 0;
+      if (mcfft_plan_many(&sub, a.rank, dims, j - i, in_fmt, out_fmt, scaling) == MCFFT_SUCCESS) {
+        p->bulk.push_back({sub, a.in_offset, a.out_offset});
+        if (mcfft_get_output_exponent(sub, &e) != MCFFT_SUCCESS) e = 0;
+        for (int k = i; k < j; ++k) p->item_exp[(size_t)k] = e;
+        i = j;
+        continue;
+      }
+    }
+    for (int k = i; k < j; ++k) {
+      RaggedItemSpec it;
+      it.rank = items[k].rank;
+      for (int d = 0; d < 3; ++d) it.dims[d] = items[k].dims[d];
+      it.in_off = items[k].in_offset;
+      it.out_off = items[k].out_offset;
+      spec.push_back(it);
+      rem_idx.push_back(k);
+    }
+    i = j;
+  }
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    free_plan(p);
+    return MCFFT_HIP_ERROR;
+  }
+  if (spec.empty()) {
+    *plan = p;
+    return MCFFT_SUCCESS;
+  }
+  p->has_queue = true;
+  int st = ragged_build(p->rcore, (int)spec.size(), spec.data(), (int)in_fmt, (int)out_fmt, (int)scaling);
   if (st != ST_OK) {
-    delete p;
+    free_plan(p);
     return (mcfftStatus)st;
   }
   RaggedPlanCore& r = p->rcore;
-  p->core.in_fmt = r.in_fmt;
-  p->core.out_fmt = r.out_fmt;
-  p->core.cm = r.cm;
+  for (size_t k = 0; k < rem_idx.size(); ++k) p->item_exp[(size_t)rem_idx[k]] = r.exponent[k];
   const KernelInfo* k = &kRaggedKernels[r.cm];
   int n_cu = 0, per_cu = 0;
   mcfftStatus rs = MCFFT_SUCCESS;
@@ -615,8 +682,14 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
 
 static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int direction, hipStream_t s) {
   const RaggedPlanCore& r = plan->rcore;
-  if (((size_t)in % (size_t)elem_bytes(r.in_fmt)) || ((size_t)out % (size_t)elem_bytes(r.out_fmt)))
-    return MCFFT_MISALIGNED;
+  const int ein = elem_bytes(plan->core.in_fmt), eout = elem_bytes(plan->core.out_fmt);
+  if (((size_t)in % (size_t)ein) || ((size_t)out % (size_t)eout)) return MCFFT_MISALIGNED;
+  for (const auto& b : plan->bulk) {
+    const mcfftStatus bs = mcfft_exec(b.plan, (const char*)in + (size_t)b.in_off * (size_t)ein,
+                                      (char*)out + (size_t)b.out_off * (size_t)eout, direction, s);
+    if (bs != MCFFT_SUCCESS) return bs;
+  }
+  if (!plan->has_queue) return MCFFT_SUCCESS;
   const int d = direction == MCFFT_FORWARD ? 0 : 1;
   const int nc = kRaggedCounterBase + r.n_counters;
   reset_counters<<<1, 256, 0, s>>>(plan->rcounters, nc);
@@ -843,7 +916,7 @@ mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!exponents) return MCFFT_INVALID_VALUE;
   if (!plan->ragged) return mcfft_get_output_exponent(plan, exponents);
-  for (int i = 0; i < plan->rcore.n_items; ++i) exponents[i] = plan->rcore.exponent[i];
+  for (size_t i = 0; i < plan->item_exp.size(); ++i) exponents[i] = plan->item_exp[i];
   return MCFFT_SUCCESS;
 }
 
@@ -853,7 +926,7 @@ mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags) {
   *flags = 0;
   if (!plan->ragged) return MCFFT_SUCCESS;
   if (hipDeviceSynchronize() != hipSuccess ||
-      hipMemcpy(flags, plan->rcounters + 4, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+      (plan->rcounters && hipMemcpy(flags, plan->rcounters + 4, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess))
     return MCFFT_HIP_ERROR;
   return MCFFT_SUCCESS;
 }
@@ -968,6 +1041,10 @@ mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
   size_t b = plan->ws_bytes;
   if (plan->real_kind) b += (size_t)(plan->real_h * plan->core.batch) * (size_t)elem_bytes(plan->real_tmp_fmt);
   b += plan->lay_bytes;
+  for (const auto& bk : plan->bulk) {
+    size_t sb = 0;
+    if (mcfft_get_workspace_size(bk.plan, &sb) == MCFFT_SUCCESS) b += sb;
+  }
   *bytes = b;
   return MCFFT_SUCCESS;
 }
@@ -975,7 +1052,17 @@ mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
 mcfftStatus mcfft_plan_describe(mcfftPlan plan, char* buf, size_t len) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!buf || len == 0) return MCFFT_INVALID_VALUE;
-  const std::string s = plan->ragged ? plan->rcore.describe() : plan->core.describe();
+  std::string s;
+  if (plan->ragged) {
+    s = plan->has_queue ? plan->rcore.describe() : std::string("ragged items=0 (every item in a bulk plan)\n");
+    for (const auto& b : plan->bulk) {
+      const std::string d = b.plan->core.describe();
+      s += "  bulk in=" + std::to_string(b.in_off) + " out=" + std::to_string(b.out_off) + ": " +
+           d.substr(0, d.find('\n')) + "\n";
+    }
+  } else {
+    s = plan->core.describe();
+  }
   snprintf(buf, len, "%s", s.c_str());
   return MCFFT_SUCCESS;
 }

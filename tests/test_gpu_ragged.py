@@ -72,6 +72,41 @@ def test_ragged_single_launch_matches_phased(torch_cuda, monkeypatch, fmt):
     assert torch.equal(phased.view(torch.int16), single.view(torch.int16))
 
 
+@pytest.mark.parametrize("bulk", ["default", "0"])
+def test_ragged_bulk_runs(torch_cuda, monkeypatch, bulk):
+    """Runs of same-shape items lying back to back with >= 2^23 elements run as a
+    batched plan each (two here: 70 x 131072 and 40 x 512x512), the other items in
+    the persistent queue; MCFFT_RAGGED_BULK=0 puts every item in the queue.  Every
+    item against the oracle with its own exponent."""
+    torch = torch_cuda
+    if bulk != "default":
+        monkeypatch.setenv("MCFFT_RAGGED_BULK", bulk)
+    shapes = [[131072]] * 70 + [[4096]] * 3 + [[256, 512]] + [[65536]] * 5 + [[512, 512]] * 40 + [[1024]] * 9
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    print(f"ragged bulk={bulk}: {len(shapes)} items, {total} elements", flush=True)
+    x = np.random.default_rng(8).standard_normal(total) + 1j * np.random.default_rng(9).standard_normal(total)
+    p = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], "fp16", "fp16")
+    desc = p.describe()
+    assert desc.count("bulk in=") == (2 if bulk == "default" else 0), desc
+    d_in = mc.to_device(x, "fp16")
+    d_out = mc.empty(total, "fp16")
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, "fp16")
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    worst = 0.0
+    for i in list(range(0, 70, 23)) + list(range(70, len(shapes))):
+        s, o, e = shapes[i], int(offs[i]), p.exponents[i]
+        n = int(np.prod(s))
+        worst = max(worst, oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s))))
+    print(f"  worst rel_l2 {worst:.3e}", flush=True)
+    assert worst <= TOL["fp16"]
+    p.destroy()
+
+
 def test_ragged_inverse_and_gaps(torch_cuda):
     """Items at arbitrary (gapped, out-of-order) offsets, inverse direction."""
     torch = torch_cuda
