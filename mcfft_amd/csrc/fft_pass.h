@@ -45,6 +45,9 @@ enum { FMT_F16 = 0, FMT_BF16 = 1, FMT_SPLIT = 2 }; // global element formats
 // scaled so that it stays in FP16's normal range wherever hi does).
 constexpr float kSplitLoScale = 2048.0f;
 constexpr float kSplitLoInv = 1.0f / 2048.0f;
+// Split passes take their twiddles from FP32 tables generated in FP64 (true) or
+// evaluate them in the kernel like FP16/BF16 passes (false).
+constexpr bool kSplitTableTwiddles = true;
 
 enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
 
@@ -391,8 +394,9 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
 
     // Scale factors.  With computed twiddles the last stage's power-of-two scale
     // rides on the twiddle the stage before it writes (the stages are linear), so
-    // the outputs need no extra multiply; the split tables carry each stage's own.
-    constexpr bool FOLD = (CM != CM_SPLIT) && C::NSTAGE >= 2;
+    // the outputs need no extra multiply; twiddle tables carry each stage's own.
+    constexpr bool TABLE_TW = (CM == CM_SPLIT) && kSplitTableTwiddles;
+    constexpr bool FOLD = !TABLE_TW && C::NSTAGE >= 2;
     const float last_scale = FOLD ? 1.0f : a.out_scale;
     auto stage_scale = [&](int P) -> float {  // scale carried by the twiddle stage P writes
       return (FOLD && P == C::NSTAGE - 2) ? a.st_scale[P] * a.st_scale[P + 1] : a.st_scale[P];
@@ -402,7 +406,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
     // four-step twiddle W_N^(m * column), N = L * fs_cols (m < L, column < fs_cols)
     const float inv_n = 1.0f / ((float)L * (float)a.fs_cols);
     auto fs_twiddle = [&](int q, int m) -> f32x2 {
-      if constexpr (CM == CM_SPLIT) {
+      if constexpr (TABLE_TW) {
         return __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8));
       } else {
         return twiddle_rev(m * (inst0 + q), inv_n, inv, last_scale);
@@ -475,12 +479,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           // those of j0 + 16 at m + 256 (tail input 1, twiddle W_512^m)
           const float sc = stage_scale(P);
           f32x2 d;  // W_512^16; split: table entry (column 16, rr = 1) without its scale
-          if constexpr (CM != CM_SPLIT) d = twiddle_rev(16, 1.0f / 512.0f, inv, 1.0f);
+          if constexpr (!TABLE_TW) d = twiddle_rev(16, 1.0f / 512.0f, inv, 1.0f);
           else d = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + 16 * 16 + 1) * 8)) * (1.0f / a.st_scale[P]);
           for (int g = 0; g < 2; ++g) {
             const int m0 = j0 + (4 * h + 8 * g) * 16;
             f32x2 tw[4];
-            if constexpr (CM == CM_SPLIT) {
+            if constexpr (TABLE_TW) {
               tw[0] = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + (j0 + 16) * 16 + 4 * h + 8 * g) * 8));
             } else {
               tw[0] = twiddle_rev(m0, 1.0f / 512.0f, inv, sc);
@@ -494,7 +498,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
               v1.x = y1[s];
               v1.y = y1[8 + s];
               // split tables carry the scale in both entries (W^0 * scale for column j0)
-              const f32x2 w0 = (CM == CM_SPLIT) ? v0 * a.st_scale[P] : v0 * sc;
+              const f32x2 w0 = TABLE_TW ? v0 * a.st_scale[P] : v0 * sc;
               const f32x2 w1 = cmul(v1, tw[u]);
               const f32x2 lo = finish(w0 + w1, q, m), hi = finish(w0 - w1, q, m + 256);
               if constexpr (STAGED) {
@@ -527,7 +531,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           int r2 = 0;
           if constexpr (!LAST) {
             r2 = (jb + 4 * h * NS) >> LGQ2;
-            if constexpr (CM != CM_SPLIT) {
+            if constexpr (!TABLE_TW) {
               tw_d = twiddle_rev(r2 * NS, INV_M, inv, 1.0f);
             } else {
               // split: W^D is the table entry (column r2 * L/(16 RN), rr = 1), whose
@@ -539,14 +543,14 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
           // four-step twiddles of the last stage: W_N^(m * column), m = mo0 + u*NS
           f32x2 fs_d;
           if constexpr (LAST && !STAGED && FOURSTEP) {
-            if constexpr (CM != CM_SPLIT) fs_d = twiddle_rev(NS * (inst0 + q), inv_n, inv, 1.0f);
+            if constexpr (!TABLE_TW) fs_d = twiddle_rev(NS * (inst0 + q), inv_n, inv, 1.0f);
             else fs_d = fs_twiddle(q, NS) * (1.0f / a.out_scale);
           }
           for (int g = 0; g < 2; ++g) {
             f32x2 tw[4];
             const int mo0 = jb + (4 * h + 8 * g) * NS;
             if constexpr (!LAST) {
-              if constexpr (CM == CM_SPLIT) {
+              if constexpr (TABLE_TW) {
                 // W^E: table entry rr = 4h + 8g of row j (FP64-generated, scale folded in)
                 tw[0] = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + j * 16 + 4 * h + 8 * g) * 8));
               } else {
