@@ -104,6 +104,20 @@ def test_ragged_bulk_runs(torch_cuda, monkeypatch, bulk):
         worst = max(worst, oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s))))
     print(f"  worst rel_l2 {worst:.3e}", flush=True)
     assert worst <= TOL["fp16"]
+    # the exec (batched plans + queue launches) replays from a hipGraph to the same bits
+    o2 = mc.empty(total, "fp16")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            p.exec(d_in, o2, mc.FORWARD, stream=s)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        o2.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(o2.view(torch.int16), d_out.view(torch.int16))
     p.destroy()
 
 
