@@ -159,8 +159,9 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
     """20000 1D transforms, N = 2^k with k uniform in 8..18, plus 64 2D transforms
     with sides from {256, 512, 1024, 2048}; laid out grouped by shape.  Timed as
     phased ragged launches (the default), as one launch with in-kernel pass
-    dependencies (MCFFT_RAGGED_SINGLE=1), and as per-shape batched launches;
-    progress goes to stderr."""
+    dependencies (MCFFT_RAGGED_SINGLE=1), as per-shape batched launches and as one
+    launch per transform, plus the ragged exec replayed from a hipGraph; progress
+    goes to stderr."""
     def progress(msg):
         print(f"c5 {fi}->{fo}: {msg}", file=sys.stderr, flush=True)
 
@@ -207,6 +208,28 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
 
     ms_p = _timer(torch, per_shape, reps)
     progress(f"per-shape {ms_p:.3f} ms")
+    # one launch per transform: a batch-1 plan per distinct shape, item by item
+    single = {s: mc.Plan(list(s), 1, fi, fo) for s in groups}
+    item_ptrs = [(single[tuple(s)], d_in.data_ptr() + int(o) * esz_in * d_in.element_size(),
+                  d_out.data_ptr() + int(o) * esz_out * d_out.element_size()) for s, o in zip(shapes, offs)]
+
+    def per_item(i):
+        st = torch.cuda.current_stream().cuda_stream
+        for p, pi, po in item_ptrs:
+            p.exec_ptr(pi, po, mc.FORWARD, st)
+
+    ms_i = _timer(torch, per_item, 2)
+    progress(f"per-item {ms_i:.3f} ms")
+    # the ragged exec captured in a hipGraph and replayed
+    gs = torch.cuda.Stream()
+    gs.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(gs):
+        with torch.cuda.graph(graph, stream=gs):
+            rp.exec(d_in, d_out, mc.FORWARD, stream=gs)
+    torch.cuda.synchronize()
+    ms_g = _timer(torch, lambda i: graph.replay(), reps)
+    progress(f"ragged graph replay {ms_g:.3f} ms")
     # accuracy on a sample of items (ragged output)
     import oracle
     rp.exec(d_in, d_out, mc.FORWARD)
@@ -223,9 +246,13 @@ def bench_c5(torch, reps, rng, fi="fp16", fo="fp16"):
            "ragged_single_launch_ms": round(ms_single, 4),
            "ragged_hbm_frac": round(total * _bytes(fi, fo) / (ms_r * 1e-3) / 1e9 / HBM, 4),
            "per_shape_launches": len(plans), "per_shape_ms": round(ms_p, 4),
-           "speedup_vs_per_shape": round(ms_p / ms_r, 3), "status_flags": flags,
+           "speedup_vs_per_shape": round(ms_p / ms_r, 3), "ragged_graph_ms": round(ms_g, 4),
+           "per_item_launches": len(item_ptrs), "per_item_ms": round(ms_i, 4),
+           "speedup_vs_per_item": round(ms_i / ms_r, 3), "status_flags": flags,
            "worst_rel_l2_sample": worst}
     for p, _, _ in plans:
+        p.destroy()
+    for p in single.values():
         p.destroy()
     rp.destroy()
     return res
