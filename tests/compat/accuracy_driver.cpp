@@ -3,7 +3,9 @@
 // and benchmark/accuracy_2d.cpp use it), linked against the compatible shim
 // library instead of the reference.  Prints one JSON line with the rel-L2,
 // max-abs and the reference's clipped metric against the FP64 oracle.
-// Built twice: -DDIM=1 (libmcfft_compat_1d) and -DDIM=2 (libmcfft_compat_2d).
+// Built three times: -DDIM=1, 2, 3 (libmcfft_compat_{1,2,3}d); the 3D build calls
+// setup(data, nx, ny, nz, n_batch) when -z is given and the reference's 2D-shaped
+// setup(data, nx, ny, n_batch) otherwise.
 #include <unistd.h>
 
 #include <cmath>
@@ -13,8 +15,10 @@
 
 #if DIM == 1
 #include "mcfft_1d_utils.h"
-#else
+#elif DIM == 2
 #include "mcfft_2d_utils.h"
+#else
+#include "mcfft_3d_utils.h"
 #endif
 
 extern "C" {
@@ -26,12 +30,13 @@ void orc_round_half(double* x, long count);
 }
 
 int main(int argc, char** argv) {
-  int n = 256, nx = 256, ny = 256, n_batch = 1, seed = 42, iters = 1;
+  int n = 256, nx = 256, ny = 256, nz = 0, n_batch = 1, seed = 42, iters = 1;
   int c;
-  while ((c = getopt(argc, argv, "n:x:y:b:s:i:")) != -1) {
+  while ((c = getopt(argc, argv, "n:x:y:z:b:s:i:")) != -1) {
     if (c == 'n') n = atoi(optarg);
     if (c == 'x') nx = atoi(optarg);
     if (c == 'y') ny = atoi(optarg);
+    if (c == 'z') nz = atoi(optarg);
     if (c == 'b') n_batch = atoi(optarg);
     if (c == 's') seed = atoi(optarg);
     if (c == 'i') iters = atoi(optarg);
@@ -42,10 +47,16 @@ int main(int argc, char** argv) {
   generate_data(data.data(), n, n_batch, seed);
   setup(data.data(), n, n_batch);
 #else
-  const long total = (long)nx * ny * n_batch;
+  const long nzz = nz > 0 ? nz : 1;
+  const long total = (long)nx * ny * nzz * n_batch;
   std::vector<double> data(2 * total), tested(2 * total);
-  generate_data(data.data(), nx * ny, n_batch, seed);
-  setup(data.data(), nx, ny, n_batch);
+  generate_data(data.data(), (int)(nx * ny * nzz), n_batch, seed);
+#if DIM == 3
+  if (nz > 0)
+    setup(data.data(), nx, ny, nz, n_batch);
+  else
+#endif
+    setup(data.data(), nx, ny, n_batch);
 #endif
   doit(iters);
   finalize(tested.data());
@@ -58,9 +69,10 @@ int main(int argc, char** argv) {
   orc_fft_many(q.data(), 1, dims, n_batch, -1);
   orc_fft_many(standard.data(), 1, dims, n_batch, -1);
 #else
-  const long dims[2] = {nx, ny};
-  orc_fft_many(q.data(), 2, dims, n_batch, -1);
-  orc_fft_many(standard.data(), 2, dims, n_batch, -1);
+  const long dims[3] = {nx, ny, nzz};
+  const int rank = nz > 0 ? 3 : 2;
+  orc_fft_many(q.data(), rank, dims, n_batch, -1);
+  orc_fft_many(standard.data(), rank, dims, n_batch, -1);
 #endif
   printf("{\"dim\": %d, \"total\": %ld, \"rel_l2\": %.4e, \"max_abs\": %.4e, \"ref_metric\": %.6e, "
          "\"rel_l2_vs_unquantised\": %.4e, \"status\": %d}\n",

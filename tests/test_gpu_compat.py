@@ -11,7 +11,9 @@ BIN = os.path.join(os.path.dirname(__file__), "compat", "_build")
 
 
 @pytest.mark.parametrize("args,dim", [(["-n", "256", "-b", "64"], 1), (["-n", "131072", "-b", "2"], 1),
-                                      (["-n", "4096", "-b", "3"], 1), (["-x", "256", "-y", "512", "-b", "2"], 2)])
+                                      (["-n", "4096", "-b", "3"], 1), (["-x", "256", "-y", "512", "-b", "2"], 2),
+                                      (["-x", "64", "-y", "32", "-z", "128", "-b", "2"], 3),
+                                      (["-x", "256", "-y", "256", "-b", "2"], 3)])
 def test_reference_api_caller(torch_cuda, args, dim):
     exe = os.path.join(BIN, f"accuracy_{dim}d")
     out = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
