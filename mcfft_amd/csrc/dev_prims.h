@@ -105,8 +105,11 @@ MCFFT_DEV unsigned atomic_load_dev(const unsigned* p) {
 MCFFT_DEV void atomic_or_dev(unsigned* p, unsigned v) {
   (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// release/acquire at device scope: writes back / invalidates the XCD's L2 as needed
-MCFFT_DEV void fence_dev() { __threadfence(); }
+// device-scope release / acquire, one side each: a producer's release (writes back
+// its XCD's dirty L2 lines) before it counts a tile, a consumer's acquire (drops
+// stale L1/L2 lines) after its wait
+MCFFT_DEV void release_dev() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+MCFFT_DEV void acquire_dev() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 MCFFT_DEV void nap() { __builtin_amdgcn_s_sleep(4); }
 
 // Work done for the whole workgroup by one wave inside a loop that also holds

@@ -164,7 +164,7 @@ MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
         }
       }
       block_barrier();
-      fence_dev();
+      acquire_dev();
     }
     const int ib = rp->in_buf, ob = rp->out_buf;
     const char* in = (ib == 0 ? r.in_base : ib == 1 ? (const char*)r.out_base : (const char*)r.ws_base) +
@@ -172,7 +172,7 @@ MCFFT_DEV void ragged_body(const RaggedArgs& r, int task_slot_offset) {
     char* out = (ob == 1 ? r.out_base : r.ws_base) + (size_t)rp->args.out;
     ragged_dispatch<CM>(rp->variant, &rp->args, in, out, t.tile, A);  // ends with a barrier
     if (r.use_deps && signal) {
-      fence_dev();
+      release_dev();
       block_barrier();
       if (wid == 0) atomic_add_dev(&ctr[counter], one);
     }

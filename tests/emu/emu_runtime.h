@@ -191,7 +191,8 @@ inline void buf_st64s(GBuf b, int voff, int soff, u32x2 v) { buf_st<u32x2>(b, vo
 inline unsigned atomic_add_dev(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 inline void atomic_or_dev(unsigned* p, unsigned v) { (void)__atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
-inline void fence_dev() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+inline void release_dev() { __atomic_thread_fence(__ATOMIC_RELEASE); }
+inline void acquire_dev() { __atomic_thread_fence(__ATOMIC_ACQUIRE); }
 inline void nap() { std::this_thread::yield(); }
 inline int wave_id() { return emu::tl_tid / 64; }
 // lane 0's value, for every lane of the wave (all 64 lanes take part)
