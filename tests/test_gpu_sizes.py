@@ -1,0 +1,101 @@
+"""GPU: every supported 1D length (16 .. 2^23) in each format, and the NONE and
+FULL scaling modes (plans and ragged batches), against the FP64 oracle.
+
+Tolerances as in the parity tests: rel-L2 fp16 1.5e-3, bf16 1.2e-2, split 1e-6.
+Scaling: NONE stores the unnormalised result (exponent 0), FULL stores it
+divided by N (exponent log2 N)."""
+import math
+
+import numpy as np
+import pytest
+
+import mcfft_amd as mc
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp16": 1.5e-3, "bf16": 1.2e-2, "split": 1e-6}
+
+
+def _fmts(fmt):
+    return ("fp16", "split") if fmt == "split" else (fmt, fmt)
+
+
+def _run(torch, dims, batch, fmt, scaling, seed):
+    in_fmt, out_fmt = _fmts(fmt)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
+    p = mc.Plan(list(dims), batch, in_fmt, out_fmt, scaling)
+    d_in = mc.to_device(x, in_fmt)
+    d_out = mc.empty(n * batch, out_fmt)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    e = p.exponent
+    y = mc.from_device(d_out, out_fmt, e)
+    p.destroy()
+    ref = oracle.fft(mc.decode(mc.encode(x, in_fmt), in_fmt), tuple(dims))
+    return oracle.rel_l2(y, ref), e
+
+
+@pytest.mark.parametrize("lg", list(range(4, 24)))
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_every_1d_length(torch_cuda, lg, fmt):
+    """Each power of two the 1D planner accepts; at least 2^18 elements per case."""
+    n = 1 << lg
+    batch = max(1, (1 << 18) // n)
+    print(f"1d n=2^{lg} b={batch} {fmt}", flush=True)
+    err, _ = _run(torch_cuda, [n], batch, fmt, "balanced", seed=lg)
+    print(f"  rel_l2 {err:.3e}", flush=True)
+    assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("dims,batch", [((256,), 64), ((4096,), 8), ((131072,), 2), ((1 << 20,), 1),
+                                        ((256, 256), 2), ((4096, 4096), 1), ((64, 64, 64), 1)])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_scaling_full(torch_cuda, dims, batch, fmt):
+    n = int(np.prod(dims))
+    print(f"full scaling dims={dims} b={batch} {fmt}", flush=True)
+    err, e = _run(torch_cuda, dims, batch, fmt, "full", seed=n + 1)
+    print(f"  rel_l2 {err:.3e} exponent {e}", flush=True)
+    assert e == int(math.log2(n))
+    assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("dims,batch", [((256,), 64), ((4096,), 8), ((65536,), 2), ((64, 64), 4),
+                                        ((16, 16, 32), 4)])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
+def test_scaling_none(torch_cuda, dims, batch, fmt):
+    """Unnormalised output; N(0,1) input at these sizes stays inside FP16's range."""
+    n = int(np.prod(dims))
+    print(f"no scaling dims={dims} b={batch} {fmt}", flush=True)
+    err, e = _run(torch_cuda, dims, batch, fmt, "none", seed=n + 2)
+    print(f"  rel_l2 {err:.3e}", flush=True)
+    assert e == 0
+    assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("scaling", ["none", "full"])
+def test_ragged_scaling_modes(torch_cuda, scaling):
+    """Per-item exponents of a ragged batch follow the scaling mode (0 / log2 N)."""
+    torch = torch_cuda
+    shapes = [[16], [4096], [256, 512], [131072], [64, 64], [1024], [16, 16, 32], [32768]]
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    print(f"ragged scaling={scaling}: {len(shapes)} items", flush=True)
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    p = mc.RaggedPlan([(s, int(o), int(o)) for s, o in zip(shapes, offs)], "fp16", "split", scaling)
+    d_in = mc.to_device(x, "fp16")
+    d_out = mc.empty(total, "split")
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, "split")
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    for s, n, o, e in zip(shapes, sizes, offs, p.exponents):
+        assert e == (0 if scaling == "none" else int(math.log2(n))), (s, e)
+        err = oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s)))
+        assert err <= TOL["split"], (s, err)
+    p.destroy()
