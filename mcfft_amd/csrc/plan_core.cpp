@@ -50,19 +50,24 @@ static bool col_view_fits(long long L, long long stride, long long Q, int cm) {
   return ((L - 1) * stride + Q) * (cm == 2 ? 8 : 4) <= (1LL << 32);
 }
 
-static const KernelInfo* find_kernel(int S, int T, int cm, int kind) {
+// The preferred tile shape of (S, T, cm, kind), else the first compiled one.
+// `ninst` > 0 (column and transposed-row kinds): the pass has that many
+// instances per group, which the tile's Q must divide, so a narrow outer axis
+// falls back to a variant with a smaller tile.
+static const KernelInfo* find_kernel(int S, int T, int cm, int kind, long long ninst = 0) {
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  auto divides = [&](const KernelInfo& k) { return ninst <= 0 || ninst % k.Q == 0; };
   for (const PreferredShape& p : kPreferred) {
     if (p.kind != kind || p.S != S || p.T != T || p.cm != cm) continue;
     for (int i = 0; i < counts[kind]; ++i) {
       const KernelInfo& k = lists[kind][i];
-      if (k.S == S && k.T == T && k.cm == cm && k.Q == p.Q && k.nw == p.nw) return &k;
+      if (k.S == S && k.T == T && k.cm == cm && k.Q == p.Q && k.nw == p.nw && divides(k)) return &k;
     }
   }
   for (int i = 0; i < counts[kind]; ++i) {
     const KernelInfo& k = lists[kind][i];
-    if (k.S == S && k.T == T && k.cm == cm) return &k;
+    if (k.S == S && k.T == T && k.cm == cm && divides(k)) return &k;
   }
   return nullptr;
 }
@@ -165,7 +170,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
                      int dst, int src_fmt) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, KIND_COL);
+    const KernelInfo* k = find_kernel(S, T, cm, KIND_COL, ncols);
     if (!k || ncols % k->Q != 0 || !col_view_fits(L, stride, k->Q, cm)) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
@@ -207,8 +212,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
       int S1, T1, S2, T2;
       if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
-      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF);
-      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT);
+      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF, N2);
+      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT, N1);
       if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
       long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (256LL << 20);
       long long per = N * elem_bytes(ws_fmt);
@@ -318,7 +323,7 @@ bool set_pass_kernel(PassPlan& pp, const KernelInfo* k) {
       pp.tiles_per_group = (int)(pp.n_inst / Q);
       break;
     default:
-      if (pp.n_inst % Q) return false;
+      if (pp.n_inst % Q || !col_view_fits(pp.L, pp.stride_in, Q, k->cm)) return false;
       pp.in_st = pp.out_st = Q;
       pp.tiles_per_group = (int)(pp.n_inst / Q);
       break;

@@ -79,6 +79,14 @@ def test_multi_dim(dims, batch):
     check(dims, batch, emu.FMT_F16, emu.FMT_SPLIT)
 
 
+@pytest.mark.parametrize("dims,batch,fo", [((256, 16), 4, emu.FMT_F16), ((256, 32), 2, emu.FMT_F16),
+                                           ((256, 32), 2, emu.FMT_SPLIT), ((256, 256, 16), 1, emu.FMT_F16)])
+def test_narrow_outer_axes(dims, batch, fo):
+    """Outer axes with fewer columns than the preferred column tile run on a
+    compiled variant with a narrower tile (16 or 32 columns)."""
+    check(dims, batch, emu.FMT_F16, fo)
+
+
 @pytest.mark.parametrize("dims", [(256,), (512,), (8192,), (64, 64)])
 def test_inverse(dims):
     check(dims, 1, emu.FMT_F16, emu.FMT_SPLIT, inverse=True)

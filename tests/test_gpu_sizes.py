@@ -99,3 +99,16 @@ def test_ragged_scaling_modes(torch_cuda, scaling):
         err = oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s)))
         assert err <= TOL["split"], (s, err)
     p.destroy()
+
+
+@pytest.mark.parametrize("dims,batch,fmt", [((256, 16), 8, "fp16"), ((256, 16), 8, "bf16"), ((256, 32), 4, "fp16"),
+                                            ((256, 32), 4, "bf16"), ((256, 32), 4, "split"),
+                                            ((256, 256, 16), 1, "fp16"), ((256, 256, 16), 1, "bf16"),
+                                            ((2048, 32), 2, "split")])
+def test_narrow_outer_axes(torch_cuda, dims, batch, fmt):
+    """Outer axes with fewer columns than the preferred column tile (64 for
+    256-point columns) run on a compiled variant with a narrower tile."""
+    print(f"narrow dims={dims} b={batch} {fmt}", flush=True)
+    err, _ = _run(torch_cuda, dims, batch, fmt, "balanced", seed=int(np.prod(dims)) + 3)
+    print(f"  rel_l2 {err:.3e}", flush=True)
+    assert err <= TOL[fmt]
