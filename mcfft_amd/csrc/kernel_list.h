@@ -105,7 +105,10 @@
   MCFFT_COL_LIST_CM(X, 1, 2, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 2, 2, 16, 16, 8, 4)         \
   MCFFT_COL_TUNE_CM(X, 0, 2)                       \
-  MCFFT_COL_TUNE_CM(X, 1, 2)
+  MCFFT_COL_TUNE_CM(X, 1, 2)                       \
+  X(2, 1, 16, 4, 2, 2)                             \
+  X(2, 2, 16, 8, 0, 2)                             \
+  X(2, 2, 16, 8, 1, 2)
 
 #define MCFFT_COLF_LIST(X)                         \
   MCFFT_COL_LIST_CM(X, 0, 3, 32, 16, 16, 8)        \

@@ -80,7 +80,8 @@ def test_multi_dim(dims, batch):
 
 
 @pytest.mark.parametrize("dims,batch,fo", [((256, 16), 4, emu.FMT_F16), ((256, 32), 2, emu.FMT_F16),
-                                           ((256, 32), 2, emu.FMT_SPLIT), ((256, 256, 16), 1, emu.FMT_F16)])
+                                           ((256, 32), 2, emu.FMT_SPLIT), ((256, 16), 4, emu.FMT_SPLIT), ((512, 16), 2, emu.FMT_F16),
+                                           ((256, 256, 16), 1, emu.FMT_F16)])
 def test_narrow_outer_axes(dims, batch, fo):
     """Outer axes with fewer columns than the preferred column tile run on a
     compiled variant with a narrower tile (16 or 32 columns)."""

@@ -104,6 +104,8 @@ def test_ragged_scaling_modes(torch_cuda, scaling):
 @pytest.mark.parametrize("dims,batch,fmt", [((256, 16), 8, "fp16"), ((256, 16), 8, "bf16"), ((256, 32), 4, "fp16"),
                                             ((256, 32), 4, "bf16"), ((256, 32), 4, "split"),
                                             ((256, 256, 16), 1, "fp16"), ((256, 256, 16), 1, "bf16"),
+                                            ((256, 16), 8, "split"), ((256, 256, 16), 1, "split"),
+                                            ((512, 16), 4, "fp16"), ((512, 16), 4, "bf16"),
                                             ((2048, 32), 2, "split")])
 def test_narrow_outer_axes(torch_cuda, dims, batch, fmt):
     """Outer axes with fewer columns than the preferred column tile (64 for
