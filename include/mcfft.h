@@ -29,7 +29,8 @@
  * 16..4096 and innermost side up to 2^15 (2^16 for FP16/BF16).  The number of
  * columns an outer axis has (the product of the sides inside it) must be a
  * multiple of the width of a compiled column tile for that axis length (4..64;
- * 32 for 16- and 64-point axes, so e.g. 64 x 16 is MCFFT_UNSUPPORTED).
+ * 16 or more for every axis but the 16-point one, which needs 32: 16 x 16 is
+ * MCFFT_UNSUPPORTED).
  *
  * Reference interface replaced: mcfftCreate / mcfftExec of
  * include/mcfft_1d_device.h:37-38, include/mcfft_2d_device.h:37-38 and

@@ -100,6 +100,12 @@
   X(2, 1, 64, 8, CM, K)              \
   X(3, 1, 4, 8, CM, K)
 
+// 16-column tiles of 32..128-point columns (outer axes over 16 columns)
+#define MCFFT_COL_NARROW_CM(X, CM) \
+  X(1, 2, 16, 4, CM, 2)            \
+  X(1, 4, 16, 4, CM, 2)            \
+  X(1, 8, 16, 4, CM, 2)
+
 #define MCFFT_COL_LIST(X)                          \
   MCFFT_COL_LIST_CM(X, 0, 2, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 1, 2, 32, 16, 16, 8)        \
@@ -108,7 +114,10 @@
   MCFFT_COL_TUNE_CM(X, 1, 2)                       \
   X(2, 1, 16, 4, 2, 2)                             \
   X(2, 2, 16, 8, 0, 2)                             \
-  X(2, 2, 16, 8, 1, 2)
+  X(2, 2, 16, 8, 1, 2)                             \
+  MCFFT_COL_NARROW_CM(X, 0)                        \
+  MCFFT_COL_NARROW_CM(X, 1)                        \
+  MCFFT_COL_NARROW_CM(X, 2)
 
 #define MCFFT_COLF_LIST(X)                         \
   MCFFT_COL_LIST_CM(X, 0, 3, 32, 16, 16, 8)        \

@@ -81,6 +81,8 @@ def test_multi_dim(dims, batch):
 
 @pytest.mark.parametrize("dims,batch,fo", [((256, 16), 4, emu.FMT_F16), ((256, 32), 2, emu.FMT_F16),
                                            ((256, 32), 2, emu.FMT_SPLIT), ((256, 16), 4, emu.FMT_SPLIT), ((512, 16), 2, emu.FMT_F16),
+                                           ((32, 16), 8, emu.FMT_F16), ((64, 16), 4, emu.FMT_SPLIT), ((128, 16), 2, emu.FMT_F16),
+                                           ((128, 64, 16), 1, emu.FMT_SPLIT),
                                            ((256, 256, 16), 1, emu.FMT_F16)])
 def test_narrow_outer_axes(dims, batch, fo):
     """Outer axes with fewer columns than the preferred column tile run on a

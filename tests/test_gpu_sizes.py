@@ -106,10 +106,12 @@ def test_ragged_scaling_modes(torch_cuda, scaling):
                                             ((256, 256, 16), 1, "fp16"), ((256, 256, 16), 1, "bf16"),
                                             ((256, 16), 8, "split"), ((256, 256, 16), 1, "split"),
                                             ((512, 16), 4, "fp16"), ((512, 16), 4, "bf16"),
-                                            ((2048, 32), 2, "split")])
+                                            ((32, 16), 8, "fp16"), ((64, 16), 8, "bf16"), ((64, 16), 8, "split"),
+                                            ((128, 16), 4, "fp16"), ((64, 64, 16), 2, "fp16"),
+                                            ((128, 64, 16), 1, "split"), ((2048, 32), 2, "split")])
 def test_narrow_outer_axes(torch_cuda, dims, batch, fmt):
-    """Outer axes with fewer columns than the preferred column tile (64 for
-    256-point columns) run on a compiled variant with a narrower tile."""
+    """Outer axes with fewer columns than the preferred column tile (32 or 64)
+    run on a compiled variant with a 16-column tile."""
     print(f"narrow dims={dims} b={batch} {fmt}", flush=True)
     err, _ = _run(torch_cuda, dims, batch, fmt, "balanced", seed=int(np.prod(dims)) + 3)
     print(f"  rel_l2 {err:.3e}", flush=True)
