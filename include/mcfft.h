@@ -193,6 +193,18 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
 mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
                               double* rel_l2, double* max_abs, void* stream);
 
+/* mcfft_error_norms without any allocation: no counterpart in the reference.
+ * For callers that cannot call hipMalloc (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid allocation).  The per-block partial sums
+ * go to a fixed library-owned device array (3 x 1024 doubles = 24 KiB per GPU,
+ * held until the process ends) instead of a buffer allocated per call; results
+ * are identical to mcfft_error_norms.  The partials do not grow with n, so every
+ * n up to 2^40 elements fits; larger n returns MCFFT_INVALID_VALUE, as does
+ * mcfft_error_norms.  Calls that overlap in time on one GPU share that array
+ * and race: serialise them (one stream, or host order).  Blocking. */
+mcfftStatus mcfft_error_norms_noalloc(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                                      double* rel_l2, double* max_abs, void* stream);
+
 /* Profiling entry points (no reference counterpart): the number of kernel
  * passes of a plan, and an exec that launches only passes [first, first+count)
  * (for every batch chunk), so a caller can time each kernel on its own stream.

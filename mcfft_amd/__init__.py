@@ -90,9 +90,11 @@ def lib() -> ctypes.CDLL:
         L.mcfft_plan_tune.argtypes = [vp, vp, vp, i, vp, i]
         L.mcfft_error_norms.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), vp]
+        L.mcfft_error_norms_noalloc.argtypes = L.mcfft_error_norms.argtypes
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_passes", "mcfft_exec_timed", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
-                  "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_plan_r2c_1d",
+                  "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_error_norms_noalloc",
+                  "mcfft_plan_r2c_1d",
                   "mcfft_plan_c2r_1d", "mcfft_plan_many_layout", "mcfft_exec_layout",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
@@ -363,16 +365,21 @@ def pairs_to_real(z: np.ndarray, n: int) -> np.ndarray:
     return r
 
 
-def error_norms(y, fmt: str, exponent: int, ref, stream=None):
+def error_norms(y, fmt: str, exponent: int, ref, stream=None, noalloc: bool = False):
     """(rel-L2, max-abs) of device tensor `y` (storage format `fmt`, times
     2^exponent) against `ref`, a device float64 tensor of interleaved complex
-    values -- reduced on the GPU in FP64."""
+    values -- reduced on the GPU in FP64.  noalloc: use the entry point that
+    keeps its partial sums in library-owned device memory (no allocation)."""
     import torch
     if stream is None:
         stream = torch.cuda.current_stream(y.device).cuda_stream
+    n = ref.numel() // 2
+    _check_tensor(y, n * ELEM_BYTES[fmt], "y")
+    _check_tensor(ref, n * 16, "ref")
     r, m = ctypes.c_double(), ctypes.c_double()
-    _check(lib().mcfft_error_norms(y.data_ptr(), FORMATS[fmt], exponent, ref.data_ptr(), ref.numel() // 2,
-                                   ctypes.byref(r), ctypes.byref(m), stream), "mcfft_error_norms")
+    fn = lib().mcfft_error_norms_noalloc if noalloc else lib().mcfft_error_norms
+    _check(fn(y.data_ptr(), FORMATS[fmt], exponent, ref.data_ptr(), n, ctypes.byref(r), ctypes.byref(m), stream),
+           "mcfft_error_norms_noalloc" if noalloc else "mcfft_error_norms")
     return r.value, m.value
 
 

@@ -167,6 +167,39 @@ __global__ void error_norms_kernel(const void* y, int fmt, double scale, const d
   }
 }
 
+// Error-norm reduction grid: kNormBlocks blocks of 256 threads, each writing one
+// (sum |d|^2, sum |ref|^2, max |d|) triple of partials.
+constexpr int kNormBlocks = 1024;
+// largest element count the error-norm entry points accept (far beyond any
+// buffer a GPU holds; the partials do not grow with n)
+constexpr long long kErrorNormsMaxElems = 1LL << 40;
+
+// Partials of mcfft_error_norms_noalloc: device memory owned by the library for
+// the whole process (one copy per GPU), so that entry point allocates nothing.
+// Every block of the reduction writes its own triple before the host reads
+// them, so nothing needs clearing between calls.  Calls that overlap in time on
+// one GPU share this array and race.
+static __device__ __align__(256) double scratch[3 * kNormBlocks];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of `scratch` on that device
+
+// Address of `scratch` on the current device (looked up once per device; the
+// symbol's host-side name is not a device address).
+mcfftStatus scratch_on_current_device(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return MCFFT_HIP_ERROR;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    double* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MCFFT_HIP_ERROR;
+    it = g_scratch_ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return MCFFT_SUCCESS;
+}
+
 std::mutex g_attr_mu;
 std::vector<const void*> g_attr_done;  // kernels whose LDS limit was raised
 
@@ -885,24 +918,18 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
   return save_cache_locked(plan->arch) ? MCFFT_SUCCESS : MCFFT_CACHE_ERROR;
 }
 
-mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
-                              double* rel_l2, double* max_abs, void* stream) {
-  if (!y || !ref || n < 1 || !rel_l2 || !max_abs || (int)fmt < 0 || (int)fmt > 2) return MCFFT_INVALID_VALUE;
-  hipStream_t s = (hipStream_t)stream;
-  const int blocks = 1024;
-  double* part = nullptr;
-  if (hipMalloc(&part, sizeof(double) * 3 * blocks) != hipSuccess) return MCFFT_ALLOC_FAILED;
-  error_norms_kernel<<<blocks, 256, 0, s>>>(y, (int)fmt, ldexp(1.0, exponent), ref, n, part);
-  std::vector<double> h(3 * blocks);
-  mcfftStatus rs = MCFFT_SUCCESS;
-  if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * blocks,
-                                                        hipMemcpyDeviceToHost, s) != hipSuccess ||
+// The error-norm reduction into `part` (3 * kNormBlocks doubles of device
+// memory), then the combine of the partials on the host (blocking).
+static mcfftStatus error_norms_into(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                                    double* rel_l2, double* max_abs, hipStream_t s, double* part) {
+  error_norms_kernel<<<kNormBlocks, 256, 0, s>>>(y, (int)fmt, ldexp(1.0, exponent), ref, n, part);
+  double h[3 * kNormBlocks];
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
-    rs = MCFFT_HIP_ERROR;
-  (void)hipFree(part);
-  if (rs != MCFFT_SUCCESS) return rs;
+    return MCFFT_HIP_ERROR;
   double num = 0, den = 0, mx = 0;
-  for (int b = 0; b < blocks; ++b) {
+  for (int b = 0; b < kNormBlocks; ++b) {
     num += h[3 * b];
     den += h[3 * b + 1];
     if (!(h[3 * b + 2] <= mx)) mx = h[3 * b + 2];
@@ -910,6 +937,30 @@ mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, cons
   *rel_l2 = sqrt(num / (den > 0 ? den : 1.0));
   *max_abs = mx;
   return MCFFT_SUCCESS;
+}
+
+static bool error_norms_args_ok(const void* y, mcfftFormat fmt, const double* ref, long long n, const double* rel_l2,
+                                const double* max_abs) {
+  return y && ref && n >= 1 && n <= kErrorNormsMaxElems && rel_l2 && max_abs && (int)fmt >= 0 && (int)fmt <= 2;
+}
+
+mcfftStatus mcfft_error_norms(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                              double* rel_l2, double* max_abs, void* stream) {
+  if (!error_norms_args_ok(y, fmt, ref, n, rel_l2, max_abs)) return MCFFT_INVALID_VALUE;
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * 3 * kNormBlocks) != hipSuccess) return MCFFT_ALLOC_FAILED;
+  const mcfftStatus rs = error_norms_into(y, fmt, exponent, ref, n, rel_l2, max_abs, (hipStream_t)stream, part);
+  (void)hipFree(part);
+  return rs;
+}
+
+mcfftStatus mcfft_error_norms_noalloc(const void* y, mcfftFormat fmt, int exponent, const double* ref, long long n,
+                                      double* rel_l2, double* max_abs, void* stream) {
+  if (!error_norms_args_ok(y, fmt, ref, n, rel_l2, max_abs)) return MCFFT_INVALID_VALUE;
+  double* part = nullptr;
+  const mcfftStatus st = scratch_on_current_device(&part);
+  if (st != MCFFT_SUCCESS) return st;
+  return error_norms_into(y, fmt, exponent, ref, n, rel_l2, max_abs, (hipStream_t)stream, part);
 }
 
 mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {
