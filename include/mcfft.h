@@ -26,7 +26,9 @@
  * overflows for inputs that fit the format).
  *
  * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D outer sides
- * 16..4096 and innermost side up to 2^15 (2^16 for FP16/BF16).  The number of
+ * 16..4096 and innermost side up to 2^16 for FP16/BF16 output, 2^14 for split
+ * output (the innermost axis runs as one single-pass row kernel, and a split
+ * row of 2^15 points does not fit the LDS).  The number of
  * columns an outer axis has (the product of the sides inside it) must be a
  * multiple of the width of a compiled column tile for that axis length (4..64;
  * 16 or more for every axis but the 16-point one, which needs 32: 16 x 16 is
@@ -77,7 +79,7 @@ typedef struct mcfftPlan_st* mcfftPlan;
 /* Create a plan for `batch` transforms of shape dims[0..rank-1] (rank 1..3).
  * Allocates the plan's tables (DFT matrices and, for split precision, the
  * twiddles; generated on the GPU in FP64) and, for 1D lengths above the
- * single-pass limit (2^15; 2^16 for FP16/BF16), a workspace; both live on the
+ * single-pass limit (2^16 for FP16/BF16 output, 2^14 for split), a workspace; both live on the
  * current device.  FP16/BF16 passes evaluate their twiddles in the kernel.
  * Replaces mcfftCreate (source/mcfft_1d_device.cpp:680-790). */
 mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,

@@ -3,8 +3,9 @@
 // Keeps the signatures of include/mcfft_1d_device.h:18-38 of the reference
 // (mcfftHandle, mcfftCreate, mcfftExec) so the reference's utils layer and
 // drivers compile unchanged.  Differences a caller can see:
-//   * MCTYPE is binary16 (__half) -- the FP16 MFMA path; define
-//     MCFFT_COMPAT_BF16 for bfloat16 storage instead.
+//   * MCTYPE is binary16 (__half) and MCTYPE2 its pair type (__half2), in
+//     place of the reference's double / double2 (include/common.h:4-5) -- the
+//     FP16 MFMA path; define MCFFT_COMPAT_BF16 for bfloat16 storage instead.
 //   * mcfftCreate reports radices = {N}, n_radices = 1, so the reference's
 //     gen_rev permutation is the identity and data is in natural order.
 //   * any power of two 16..2^23 is supported; an unsupported size or a HIP
@@ -23,8 +24,10 @@
 
 #ifdef MCFFT_COMPAT_BF16
 typedef __hip_bfloat16 MCTYPE;
+typedef __hip_bfloat162 MCTYPE2;
 #else
 typedef __half MCTYPE;
+typedef __half2 MCTYPE2;
 #endif
 
 typedef struct {

@@ -13,8 +13,10 @@
 
 #ifdef MCFFT_COMPAT_BF16
 typedef __hip_bfloat16 MCTYPE;
+typedef __hip_bfloat162 MCTYPE2;
 #else
 typedef __half MCTYPE;
+typedef __half2 MCTYPE2;
 #endif
 
 typedef struct {

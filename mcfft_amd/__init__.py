@@ -223,7 +223,10 @@ class Plan:
         self._check_io(inp, out, contiguous=True)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
-        n = len(start_events)
+        n = self.pass_count()
+        if len(start_events) != n or len(stop_events) != n:
+            raise ValueError(f"exec_timed needs {n} start and {n} stop events (one pair per pass), got "
+                             f"{len(start_events)} and {len(stop_events)}")
         st = (ctypes.c_void_p * n)(*[e.cuda_event for e in start_events])
         sp = (ctypes.c_void_p * n)(*[e.cuda_event for e in stop_events])
         _check(lib().mcfft_exec_timed(self._h, inp.data_ptr(), out.data_ptr(), direction, stream, st, sp),

@@ -98,21 +98,27 @@ MCFFT_DEV uint32_t pk_bf16(f32x2 v) { return __builtin_bit_cast(uint32_t, __buil
 MCFFT_DEV f32x2 up_f16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(f16x2, e), f32x2); }
 MCFFT_DEV f32x2 up_bf16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(bf16x2, e), f32x2); }
 
+// Byte offset of element i (esz bytes each) inside a buffer view.  Views span up
+// to 2^32 bytes, so the product is formed in unsigned 32-bit arithmetic (element
+// indices stay below 2^31; their byte offsets need not) and handed to the
+// buffer instruction, whose offsets are unsigned, as the same 32 bits.
+MCFFT_DEV int boff(int i, unsigned esz) { return (int)((unsigned)i * esz); }
+
 template <> struct Traits<CM_F16> {
   typedef uint32_t T;
   static MCFFT_DEV T pack(f32x2 v) { return pk_f16(v); }
   static MCFFT_DEV f32x2 unpack(T e) { return up_f16(e); }
-  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
-  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
-  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, i * 4, si * 4, v); }
+  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, boff(i, 4)); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, boff(i, 4), v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, boff(i, 4), boff(si, 4), v); }
 };
 template <> struct Traits<CM_BF16> {
   typedef uint32_t T;
   static MCFFT_DEV T pack(f32x2 v) { return pk_bf16(v); }
   static MCFFT_DEV f32x2 unpack(T e) { return up_bf16(e); }
-  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, i * 4); }
-  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, i * 4, v); }
-  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, i * 4, si * 4, v); }
+  static MCFFT_DEV T load(GBuf b, int i, int) { return buf_ld32(b, boff(i, 4)); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st32(b, boff(i, 4), v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st32s(b, boff(i, 4), boff(si, 4), v); }
 };
 template <> struct Traits<CM_SPLIT> {
   typedef u32x2 T;  // .x = hi pair (re, im), .y = lo pair
@@ -124,14 +130,14 @@ template <> struct Traits<CM_SPLIT> {
   }
   static MCFFT_DEV f32x2 unpack(T e) { return up_f16(e.y) * kSplitLoInv + up_f16(e.x); }
   static MCFFT_DEV T load(GBuf b, int i, int fmt) {
-    if (fmt == FMT_SPLIT) return buf_ld64(b, i * 8);
+    if (fmt == FMT_SPLIT) return buf_ld64(b, boff(i, 8));
     T e;
-    e.x = buf_ld32(b, i * 4);
+    e.x = buf_ld32(b, boff(i, 4));
     e.y = 0u;
     return e;
   }
-  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st64(b, i * 8, v); }
-  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st64s(b, i * 8, si * 8, v); }
+  static MCFFT_DEV void store(GBuf b, int i, T v) { buf_st64(b, boff(i, 8), v); }
+  static MCFFT_DEV void store_s(GBuf b, int i, int si, T v) { buf_st64s(b, boff(i, 8), boff(si, 8), v); }
 };
 
 // DFT-matrix operands of one lane: A1 = [Fr; Fi], A2 = [-Fi; Fr] (hi and lo parts)

@@ -81,6 +81,28 @@ def exec_fft(x, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, ws_limit
     return from_storage(out, out_fmt) * (2.0 ** e.value), e.value
 
 
+def exec_ptr(in_ptr, out_ptr, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, only_pass=None,
+             blocks=None):
+    """Run the emulated plan on caller-owned host memory (raw addresses of
+    buffers as large as the plan's input and output).  only_pass / blocks
+    (first, count): run only that pass, launched with one workgroup per tile,
+    and of it only those workgroups.  Returns the plan's output exponent."""
+    L = lib()
+    L.emu_restrict.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    e = ctypes.c_int(0)
+    first, count = blocks if blocks is not None else (0, 1 << 62)
+    L.emu_restrict(-1 if only_pass is None else int(only_pass), first, count)
+    try:
+        rc = L.emu_exec(len(dims), d, batch, in_fmt, out_fmt, scaling, int(inverse), in_ptr, out_ptr, 0,
+                        ctypes.byref(e))
+    finally:
+        L.emu_restrict(-1, 0, 0)
+    if rc != 0:
+        raise RuntimeError(f"emu_exec status {rc}")
+    return e.value
+
+
 def describe(dims, batch, in_fmt, out_fmt, scaling=1):
     buf = ctypes.create_string_buffer(8192)
     d = (ctypes.c_longlong * len(dims))(*dims)

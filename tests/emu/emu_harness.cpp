@@ -72,6 +72,17 @@ static std::string g_kernel_override, g_last_kernels;
 extern "C" void emu_set_kernel_override(const char* name) { g_kernel_override = name ? name : ""; }
 extern "C" const char* emu_last_kernels() { return g_last_kernels.c_str(); }
 
+// Restrict emu_exec to pass `pass` of the plan, launched with one workgroup per
+// tile, of which only workgroups [blk_first, blk_first + blk_count) run: one
+// part of a plan too large to run whole on the CPU.  pass < 0 lifts it.
+static int g_only_pass = -1;
+static long long g_blk_first = 0, g_blk_count = 0;
+extern "C" void emu_restrict(int pass, long long blk_first, long long blk_count) {
+  g_only_pass = pass;
+  g_blk_first = blk_first;
+  g_blk_count = blk_count;
+}
+
 extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
                         int inverse, const void* in, void* out, long long ws_limit, int* out_exponent) {
   PlanCore p;
@@ -105,7 +116,9 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   for (long long c = 0; c < p.n_chunks; ++c) {
     const long long t0 = c * p.chunk;
     const long long nt = (t0 + p.chunk <= batch) ? p.chunk : batch - t0;
-    for (const PassPlan& pp : p.passes) {
+    for (size_t pi = 0; pi < p.passes.size(); ++pi) {
+      const PassPlan& pp = p.passes[pi];
+      if (g_only_pass >= 0 && (int)pi != g_only_pass) continue;
       PassArgs a;
       memset(&a, 0, sizeof(a));
       auto bufptr = [&](int b, int fmt) -> char* {
@@ -140,9 +153,14 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * pp.tiles_per_group;
       a.n_tiles = (int)tiles;
+      void (*fn)(PassArgs) = (void (*)(PassArgs))pp.k->fn;
+      if (g_only_pass >= 0) {
+        emu::launch((int)tiles, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); }, (int)g_blk_first,
+                    (int)(g_blk_first + g_blk_count));
+        continue;
+      }
       // fewer workgroups than tiles exercises the persistent tile loop
       const long long grid = tiles > 3 ? (tiles + 1) / 2 : tiles;
-      void (*fn)(PassArgs) = (void (*)(PassArgs))pp.k->fn;
       emu::launch((int)grid, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); });
     }
   }

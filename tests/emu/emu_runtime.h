@@ -85,9 +85,11 @@ inline void launch_concurrent(int nblocks, int nt, size_t lds, const std::functi
   }
 }
 
-// launch `body` as a grid of nblocks workgroups of nt threads with `lds` bytes of LDS
-inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>& body) {
-  for (int b = 0; b < nblocks; ++b) {
+// launch `body` as a grid of nblocks workgroups of nt threads with `lds` bytes of
+// LDS, running only workgroups [b0, b1) of it (default: all)
+inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>& body, int b0 = 0, int b1 = -1) {
+  if (b1 < 0 || b1 > nblocks) b1 = nblocks;
+  for (int b = b0; b < b1; ++b) {
     Block blk;
     blk.nt = nt;
     blk.bid = b;
@@ -163,30 +165,35 @@ struct GBuf {
   unsigned bytes;
 };
 inline GBuf make_gbuf(const void* p, unsigned bytes) { return GBuf{(const char*)p, bytes}; }
+// (the hardware reads a buffer offset as an unsigned 32-bit byte count)
 template <class T> inline T buf_ld(GBuf b, int off) {
   T v;
-  if ((unsigned)off + sizeof(T) > b.bytes || (unsigned)off > b.bytes) {
+  const size_t uo = (unsigned)off;
+  if (uo + sizeof(T) > b.bytes) {
     memset(&v, 0, sizeof(T));
     return v;
   }
-  emu::check_global(b.p + off, sizeof(T), off);
-  memcpy(&v, b.p + off, sizeof(T));
+  emu::check_global(b.p + uo, sizeof(T), (long long)uo);
+  memcpy(&v, b.p + uo, sizeof(T));
   return v;
 }
 template <class T> inline void buf_st(GBuf b, int off, T v) {
-  if ((unsigned)off + sizeof(T) > b.bytes || (unsigned)off > b.bytes) return;
-  emu::check_global(b.p + off, sizeof(T), off);
-  memcpy((char*)b.p + off, &v, sizeof(T));
+  const size_t uo = (unsigned)off;
+  if (uo + sizeof(T) > b.bytes) return;
+  emu::check_global(b.p + uo, sizeof(T), (long long)uo);
+  memcpy((char*)b.p + uo, &v, sizeof(T));
 }
 inline uint32_t buf_ld32(GBuf b, int off) { return buf_ld<uint32_t>(b, off); }
 inline u32x2 buf_ld64(GBuf b, int off) { return buf_ld<u32x2>(b, off); }
 inline u32x4 buf_ld128(GBuf b, int off) { return buf_ld<u32x4>(b, off); }
 inline void buf_st32(GBuf b, int off, uint32_t v) { buf_st<uint32_t>(b, off, v); }
 inline void buf_st64(GBuf b, int off, u32x2 v) { buf_st<u32x2>(b, off, v); }
-inline uint32_t buf_ld32s(GBuf b, int voff, int soff) { return buf_ld<uint32_t>(b, voff + soff); }
-inline u32x4 buf_ld128s(GBuf b, int voff, int soff) { return buf_ld<u32x4>(b, voff + soff); }
-inline void buf_st32s(GBuf b, int voff, int soff, uint32_t v) { buf_st<uint32_t>(b, voff + soff, v); }
-inline void buf_st64s(GBuf b, int voff, int soff, u32x2 v) { buf_st<u32x2>(b, voff + soff, v); }
+// vector + scalar offsets add as unsigned 32-bit values
+inline int uadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+inline uint32_t buf_ld32s(GBuf b, int voff, int soff) { return buf_ld<uint32_t>(b, uadd(voff, soff)); }
+inline u32x4 buf_ld128s(GBuf b, int voff, int soff) { return buf_ld<u32x4>(b, uadd(voff, soff)); }
+inline void buf_st32s(GBuf b, int voff, int soff, uint32_t v) { buf_st<uint32_t>(b, uadd(voff, soff), v); }
+inline void buf_st64s(GBuf b, int voff, int soff, u32x2 v) { buf_st<u32x2>(b, uadd(voff, soff), v); }
 
 inline unsigned atomic_add_dev(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }

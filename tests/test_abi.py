@@ -78,3 +78,39 @@ def test_argument_errors_are_status_codes():
     assert L.mcfft_destroy(None) == 6
     e = ctypes.c_int()
     assert L.mcfft_get_output_exponent(None, ctypes.byref(e)) == 6
+
+
+CALLER = r"""
+#include "mcfft_%dd_utils.h"
+// a caller of the reference's type surface: benchmark/test_1d.cpp:105 declares
+// vkfft_get_result(MCTYPE2* data, MCTYPE* result, int n, int n_batch)
+static void pairs_to_scalars(MCTYPE2* data, MCTYPE* result, int n, int n_batch) {
+  for (long long i = 0; i < (long long)n * n_batch; ++i) {
+    result[2 * i] = data[i].x;
+    result[2 * i + 1] = data[i].y;
+  }
+}
+int main() {
+  static_assert(sizeof(MCTYPE2) == 2 * sizeof(MCTYPE), "pair of scalars");
+  MCTYPE2 d[4];
+  MCTYPE r[8];
+  pairs_to_scalars(d, r, 2, 2);
+  double buf[32];
+  generate_data(buf, 16, 1);
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_compat_headers_declare_reference_types(tmp_path, dim, bf16):
+    """The shim headers define MCTYPE and MCTYPE2 (include/common.h:4-5 of the
+    reference), so reference callers written against them compile."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(CALLER % dim)
+    cmd = ["/opt/rocm/bin/hipcc", "-fsyntax-only", "-std=c++17", f"-I{INC}", str(src)]
+    if bf16:
+        cmd.insert(1, "-DMCFFT_COMPAT_BF16")
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -2,6 +2,8 @@
 the FP64 oracle -- CPU only.  Every LDS and global access of the emulated
 kernels is bounds-checked (an out-of-range index aborts the test process), and
 the MFMA is evaluated from the operand layout the GPU layout test pins."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -342,3 +344,39 @@ def test_column_views_fit_32bit_byte_offsets():
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([1024, 1024, 1024], 1, emu.FMT_F16, emu.FMT_SPLIT)
     assert "pass" in emu.describe([512, 1024, 1024], 1, emu.FMT_F16, emu.FMT_SPLIT)
+
+
+def test_col_pass_byte_offsets_beyond_2gib():
+    """The x-axis pass of an FP16 1024^3 plan addresses its columns through one
+    buffer view of (1023 * 2^20 + 16) * 4 ~ 4.29e9 bytes, so half of every tile's
+    element offsets become byte offsets above 2^31.  Run the last 32 tiles of
+    that pass (columns 2^20 - 512 .. 2^20 - 1 of the y-z plane, every x row) in
+    place on a 4 GiB anonymous mapping, of which only the touched pages are ever
+    backed, and check each column against an FP64 FFT along x."""
+    import mmap
+    nx = 1 << 10
+    plane = 1 << 20                      # ny * nz: the x-axis stride
+    n = nx * plane
+    flags = mmap.MAP_PRIVATE | mmap.MAP_ANONYMOUS | getattr(mmap, "MAP_NORESERVE", 0x4000)
+    mm = mmap.mmap(-1, 4 * n, flags=flags)
+    try:
+        buf = np.frombuffer(mm, dtype=np.uint32)
+        cols = np.arange(plane - 512, plane)
+        idx = np.arange(nx)[:, None] * plane + cols[None, :]
+        assert idx.max() * 4 > (1 << 31) and idx.max() * 4 < (1 << 32)
+        rng = np.random.default_rng(1024)
+        x = rng.standard_normal((nx, 512)) + 1j * rng.standard_normal((nx, 512))
+        buf[idx] = emu.to_storage(x.reshape(-1), emu.FMT_F16).view(np.uint32).reshape(nx, 512)
+        xq = emu.from_storage(buf[idx].reshape(-1), emu.FMT_F16).reshape(nx, 512)
+        desc = emu.describe([nx, 1024, 1024], 1, emu.FMT_F16, emu.FMT_F16, 0)
+        assert "stride(in,out)=(1048576,1048576)" in desc.splitlines()[3]
+        ptr = ctypes.addressof(ctypes.c_char.from_buffer(mm))
+        emu.exec_ptr(ptr, ptr, [nx, 1024, 1024], 1, emu.FMT_F16, emu.FMT_F16, scaling=0, only_pass=2,
+                     blocks=(plane // 16 - 32, 32))
+        y = emu.from_storage(buf[idx].reshape(-1), emu.FMT_F16).reshape(nx, 512)
+        ref = np.fft.fft(xq, axis=0)
+        err = oracle.rel_l2(y.reshape(-1), ref.reshape(-1))
+        assert err <= 1.5e-3, err
+        del buf, y
+    finally:
+        mm.close()
