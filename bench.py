@@ -101,12 +101,14 @@ def main():
     mid = mc.empty(elems, "fp16")
     out = mc.empty(elems, "fp16")
 
+    # the default plan: block floating point (per-transform exponents found on
+    # the GPU); the inverse takes the forward's exponents as its input exponents
     pf = mc.Plan([N], BATCH, "fp16", "fp16")
     stream = torch.cuda.current_stream()
 
     def step(i):
         pf.exec(ins[i % n_rot], mid, mc.FORWARD)
-        pf.exec(mid, out, mc.INVERSE)
+        pf.exec(mid, out, mc.INVERSE, in_exp=pf)
 
     def warm(fn, warmup, min_ms):
         """Untimed: `warmup` calls, then more (in groups of 10) until at least
@@ -148,14 +150,13 @@ def main():
     ms_plain = timed(step, args.steps, args.warmup, step_ms)
 
     # errors (plain): forward output of the first 8 transforms vs FP64, and the round trip
-    pf.exec(ins[0], mid, mc.FORWARD)
-    pf.exec(mid, out, mc.INVERSE)
-    torch.cuda.synchronize()
     import oracle
     k = 8 * N
-    y_f = mc.from_device(mid[:k], "fp16", pf.exponent)
+    pf.exec(ins[0], mid, mc.FORWARD)
+    y_f = pf.result(mid, 8)
+    pf.exec(mid, out, mc.INVERSE, in_exp=pf)
+    y_rt = pf.result(out, 8) / N
     ref_f = oracle.fft(xq[:k], (N,))
-    y_rt = mc.from_device(out[:k], "fp16", 2 * pf.exponent) / N
     plain_err = {"rel_l2_fwd": oracle.rel_l2(y_f, ref_f), "max_abs_fwd": oracle.max_abs(y_f, ref_f),
                  "rel_l2_roundtrip": oracle.rel_l2(y_rt, xq[:k]), "checked_transforms": 8}
 
@@ -213,6 +214,17 @@ def main():
             "peak_tflops": 2500.0, "frac": round(mfma_flops / (dom["ms"] * 1e-3) / 1e12 / 2500.0, 4),
             "kernel": dom["kernel"]}
 
+    # the same step with static BALANCED scaling (one exponent for every
+    # transform): the cost of the data-following scaling is the difference
+    pb = mc.Plan([N], BATCH, "fp16", "fp16", "balanced")
+
+    def bstep(i):
+        pb.exec(ins[i % n_rot], mid, mc.FORWARD)
+        pb.exec(mid, out, mc.INVERSE)
+
+    ms_balanced = timed(bstep, args.steps, args.warmup)
+    pb.destroy()
+
     # split precision leg: forward FP16 -> split, inverse split -> split
     split = None
     if not args.no_split:
@@ -223,14 +235,13 @@ def main():
 
         def sstep(i):
             ps_f.exec(ins[i % n_rot], smid, mc.FORWARD)
-            ps_i.exec(smid, sout, mc.INVERSE)
+            ps_i.exec(smid, sout, mc.INVERSE, in_exp=ps_f)
 
         ms_split = timed(sstep, args.steps, args.warmup)
         ps_f.exec(ins[0], smid, mc.FORWARD)
-        ps_i.exec(smid, sout, mc.INVERSE)
-        torch.cuda.synchronize()
-        ys = mc.from_device(smid[:k], "split", ps_f.exponent)
-        ys_rt = mc.from_device(sout[:k], "split", ps_f.exponent + ps_i.exponent) / N
+        ps_i.exec(smid, sout, mc.INVERSE, in_exp=ps_f)
+        ys = ps_f.result(smid, 8)
+        ys_rt = ps_i.result(sout, 8) / N
         split_bytes = elems * (12 + 16)   # fwd FP16 in + hi/lo out, inv hi/lo in + out
         split = {"value": round(flops_step / (ms_split * 1e-3) / 1e9, 1), "ms_per_step": round(ms_split, 4),
                  "hbm_frac_job": round(split_bytes / (ms_split * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -306,7 +317,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "1D C2C forward+inverse, N=131072 (16^4 x 2), batch 256, interleaved complex FP16, "
                                "re/im iid N(0,1) seed 2",
-                   "n": N, "batch": BATCH, "precision": "plain fp16 (value); split fp16 hi+lo in 'split'"},
+                   "n": N, "batch": BATCH, "precision": "plain fp16 (value); split fp16 hi+lo in 'split'",
+                   "scaling": "auto (block floating point, per-transform exponents; the default plan)"},
+        "static_balanced": {"ms_per_step": round(ms_balanced, 4),
+                            "value": round(flops_step / (ms_balanced * 1e-3) / 1e9, 1)},
         "hbm_frac_job": round(job_bytes / (ms_plain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "gflops_12nlogn": round(value * 12.0 / 5.0, 1),   # the reference's FLOP convention (test_1d.cpp)
         "copy_gbs": round(copy_gbs, 1),

@@ -18,12 +18,21 @@
  * SPLIT->SPLIT.  FP16/BF16 compute on MFMA with FP32 accumulation; SPLIT forms
  * the compensated products hi*hi + 2^-11 (hi*lo + lo*hi) on FP16 MFMA.
  *
- * Scaling: each radix stage multiplies by a power of two chosen by the plan's
- * scaling mode, so the stored result is  true_result * 2^-E  where E is
- * reported by mcfft_get_output_exponent().  NONE keeps E = 0 (unnormalised,
- * FFTW convention, may overflow FP16), BALANCED keeps the RMS of random data
- * roughly constant (E ~ log2(N)/2), FULL divides by N (E = log2 N; never
- * overflows for inputs that fit the format).
+ * Scaling (block floating point): the stored result of transform t is
+ * true_result * 2^-E[t].  AUTO (the default of the mcfft_plan_1d/2d/3d
+ * shorthands) follows the data: the first pass scales each tile of its input
+ * so that its largest |component| lies just below 2^13, every radix-R stage
+ * divides by R, and E[t] is found on the GPU per transform (read it with
+ * mcfft_get_transform_exponents or mcfft_get_exponent_buffer).  No finite input
+ * overflows, and small inputs are scaled up rather than lost to underflow.  The
+ * static modes use one exponent for every transform (mcfft_get_output_exponent):
+ * NONE keeps E = 0 (unnormalised, FFTW convention, may overflow FP16),
+ * BALANCED keeps the RMS of random data roughly constant (E ~ log2(N)/2), FULL
+ * divides by N (E = log2 N; never overflows, but small inputs underflow).  A
+ * statically scaled exec that stores an Inf/NaN, or an AUTO exec whose input
+ * holds one, sets MCFFT_STATUS_NONFINITE (mcfft_get_status_flags); not watched:
+ * the stores of statically scaled 1D plans of exactly 65536 points (their
+ * single-pass kernel has no registers to spare).
  *
  * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D outer sides
  * 16..4096 and innermost side up to 2^16 for FP16/BF16 output, 2^14 for split
@@ -57,7 +66,9 @@ typedef enum mcfftStatus_t {
   MCFFT_HIP_ERROR = 4,       /* a HIP runtime call failed */
   MCFFT_MISALIGNED = 5,      /* buffer not aligned to its element size */
   MCFFT_INVALID_PLAN = 6,
-  MCFFT_CACHE_ERROR = 7      /* tuning cache unreadable or of another version */
+  MCFFT_CACHE_ERROR = 7,     /* tuning cache unreadable or of another version */
+  MCFFT_NONFINITE = 8        /* a result holds Inf/NaN (FP16 overflow under static scaling; the
+                                reference-compatible layer's mcfftGetLastError) */
 } mcfftStatus;
 
 typedef enum mcfftFormat_t {
@@ -71,8 +82,13 @@ typedef enum mcfftDirection_t { MCFFT_FORWARD = -1, MCFFT_INVERSE = 1 } mcfftDir
 typedef enum mcfftScaling_t {
   MCFFT_SCALE_NONE = 0,
   MCFFT_SCALE_BALANCED = 1,
-  MCFFT_SCALE_FULL = 2
+  MCFFT_SCALE_FULL = 2,
+  MCFFT_SCALE_AUTO = 3   /* per-transform exponents chosen from the data (not for ragged plans) */
 } mcfftScaling;
+
+/* bits of mcfft_get_status_flags */
+#define MCFFT_STATUS_WAIT_BOUND 1u  /* ragged single launch: a dependency wait hit its bound */
+#define MCFFT_STATUS_NONFINITE 2u   /* an Inf/NaN was stored (static scaling) or read (AUTO) */
 
 typedef struct mcfftPlan_st* mcfftPlan;
 
@@ -84,7 +100,7 @@ typedef struct mcfftPlan_st* mcfftPlan;
  * Replaces mcfftCreate (source/mcfft_1d_device.cpp:680-790). */
 mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
                             mcfftFormat out_fmt, mcfftScaling scaling);
-/* Shorthands with MCFFT_SCALE_BALANCED. */
+/* Shorthands with MCFFT_SCALE_AUTO. */
 mcfftStatus mcfft_plan_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt);
 mcfftStatus mcfft_plan_2d(mcfftPlan* plan, long long nx, long long ny, long long batch, mcfftFormat in_fmt,
                           mcfftFormat out_fmt);
@@ -99,6 +115,14 @@ mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long
  * must not overlap in time (use one plan per stream).
  * Replaces mcfftExec (source/mcfft_1d_device.cpp:596-678). */
 mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream);
+
+/* mcfft_exec whose input transforms carry their own exponents: in_exp is a
+ * device array of one int per transform (true input = stored * 2^in_exp[t]),
+ * or NULL for all 0; the output exponents then include them.  Chaining plans
+ * this way keeps every result in true units, e.g. an inverse over a forward's
+ * output with in_exp = the forward plan's mcfft_get_exponent_buffer (which may
+ * be the same plan's own buffer).  No reference counterpart. */
+mcfftStatus mcfft_exec_exp(mcfftPlan plan, const void* in, const int* in_exp, void* out, int direction, void* stream);
 
 /* Real-input forward (R2C) and real-output inverse (C2R) 1D transforms; no
  * reference counterpart (the reference has only the forward C2C).
@@ -172,11 +196,24 @@ typedef struct mcfftRaggedItem_t {
 
 mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedItem* items, mcfftFormat in_fmt,
                               mcfftFormat out_fmt, mcfftScaling scaling);
-/* Output exponent of every item (n_items entries; 1 entry for a regular plan). */
+/* Output exponent of every item (n_items entries; 1 entry for a statically
+ * scaled regular plan). */
 mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents);
-/* Ragged plans: nonzero if a queue wait of the last exec hit its bound (its
- * result is then wrong).  Synchronises with the device. */
+/* Status bits (MCFFT_STATUS_*): for a regular plan, MCFFT_STATUS_NONFINITE if
+ * any exec since the last call stored (static scaling) or read (AUTO) an
+ * Inf/NaN -- the bits are cleared by reading them; for a ragged plan,
+ * MCFFT_STATUS_WAIT_BOUND if a queue wait of the last exec hit its bound (its
+ * result is then wrong), plus its batched runs' bits.  Synchronises with the
+ * device. */
 mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags);
+/* Output exponents of the last exec, one per transform (batch entries; a
+ * ragged plan: one per item): true result of transform t = stored * 2^e[t].
+ * Synchronises with the device.  No reference counterpart. */
+mcfftStatus mcfft_get_transform_exponents(mcfftPlan plan, int* exponents);
+/* Device array (batch ints) the plan's execs write their per-transform output
+ * exponents to, in stream order: usable by later work on the same stream, or
+ * as another exec's in_exp.  Regular plans only. */
+mcfftStatus mcfft_get_exponent_buffer(mcfftPlan plan, const int** exponents);
 
 /* Autotuner: times every compiled kernel variant (tile size, waves per
  * workgroup) of each pass of `plan` on the given device buffers, `reps` launches
@@ -223,7 +260,9 @@ mcfftStatus mcfft_exec_passes(mcfftPlan plan, const void* in, void* out, int dir
 mcfftStatus mcfft_exec_timed(mcfftPlan plan, const void* in, void* out, int direction, void* stream,
                              void* const* start_events, void* const* stop_events);
 
-/* E such that true result = stored * 2^E (same for both directions). */
+/* Statically scaled plans: E such that true result = stored * 2^E (same for
+ * both directions).  AUTO plans: MCFFT_UNSUPPORTED (their exponents are per
+ * transform: mcfft_get_transform_exponents). */
 mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent);
 /* Workspace the plan uses (bytes; 0 when none). */
 mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes);

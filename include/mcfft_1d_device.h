@@ -11,8 +11,14 @@
 //   * any power of two 16..2^23 is supported; an unsupported size or a HIP
 //     failure is reported on stderr and by mcfftGetLastError() (the reference
 //     leaves the plan uninitialised).
-//   * transforms are unnormalised (MCFFT_SCALE_NONE), in place, on the null
-//     stream, as in the reference.
+//   * mcfftExec transforms are unnormalised (MCFFT_SCALE_NONE), in place, on
+//     the null stream, as in the reference; a result that overflowed FP16 is
+//     reported by mcfftGetLastError() as MCFFT_NONFINITE (which waits for the
+//     device).
+//   * the utils layer (setup / doit / finalize, FP64 on the host) runs its plan
+//     with block floating point (MCFFT_SCALE_AUTO) and returns true values:
+//     no overflow or underflow whatever the data, and doit(iter) iterates the
+//     transform exactly as the reference would at FP64.
 #ifndef MCFFT_1D_DEVICE_H
 #define MCFFT_1D_DEVICE_H
 

@@ -2,7 +2,8 @@
 // code under a 3D name (include/mcfft_3d_device.h:37-38 takes (nx, ny, n_batch));
 // that signature is kept and plans a 2D transform exactly as the reference
 // does, and a (nx, ny, nz, n_batch) overload plans a real 3D transform
-// ([batch][x][y][z], z contiguous, sides 16..4096).
+// ([batch][x][y][z], z contiguous, sides 16..4096).  See mcfft_1d_device.h
+// for the other differences.
 #ifndef MCFFT_3D_DEVICE_H
 #define MCFFT_3D_DEVICE_H
 

@@ -24,10 +24,11 @@ LIB_PATH = os.path.join(LIB_DIR, "libmcfft.so")
 LIB_PATH = os.environ.get("MCFFT_LIB", LIB_PATH)
 
 FORMATS = {"fp16": 0, "bf16": 1, "split": 2}
-SCALINGS = {"none": 0, "balanced": 1, "full": 2}
+SCALINGS = {"none": 0, "balanced": 1, "full": 2, "auto": 3}
+STATUS_WAIT_BOUND, STATUS_NONFINITE = 1, 2
 FORWARD, INVERSE = -1, 1
 STATUS = {0: "success", 1: "invalid value", 2: "unsupported", 3: "allocation failed", 4: "HIP error",
-          5: "misaligned buffer", 6: "invalid plan", 7: "tuning cache error"}
+          5: "misaligned buffer", 6: "invalid plan", 7: "tuning cache error", 8: "non-finite result (overflow)"}
 
 _lib = None
 
@@ -69,6 +70,9 @@ def lib() -> ctypes.CDLL:
         L.mcfft_plan_2d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, i, i]
         L.mcfft_plan_3d.argtypes = [ctypes.POINTER(vp), ll, ll, ll, ll, i, i]
         L.mcfft_exec.argtypes = [vp, vp, vp, i, vp]
+        L.mcfft_exec_exp.argtypes = [vp, vp, vp, vp, i, vp]
+        L.mcfft_get_transform_exponents.argtypes = [vp, ctypes.POINTER(i)]
+        L.mcfft_get_exponent_buffer.argtypes = [vp, ctypes.POINTER(vp)]
         L.mcfft_exec_passes.argtypes = [vp, vp, vp, i, vp, i, i]
         L.mcfft_exec_timed.argtypes = [vp, vp, vp, i, vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L.mcfft_get_pass_count.argtypes = [vp, ctypes.POINTER(i)]
@@ -92,6 +96,7 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_double), vp]
         L.mcfft_error_norms_noalloc.argtypes = L.mcfft_error_norms.argtypes
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
+                  "mcfft_exec_exp", "mcfft_get_transform_exponents", "mcfft_get_exponent_buffer",
                   "mcfft_exec_passes", "mcfft_exec_timed", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
                   "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_error_norms_noalloc",
                   "mcfft_plan_r2c_1d",
@@ -124,11 +129,24 @@ def _check_tensor(t, nbytes: int, what: str):
         raise ValueError(f"{what}: {have} bytes, the plan needs {nbytes}")
 
 
+def _static_exponent(h, scaling: str):
+    """The plan's one exponent (static scaling), or None (AUTO: per transform)."""
+    if scaling == "auto":
+        return None
+    e = ctypes.c_int()
+    _check(lib().mcfft_get_output_exponent(h, ctypes.byref(e)), "mcfft_get_output_exponent")
+    return e.value
+
+
 class Plan:
-    """A plan for `batch` transforms of shape `dims` (1 to 3 dims, last contiguous)."""
+    """A plan for `batch` transforms of shape `dims` (1 to 3 dims, last contiguous).
+
+    scaling "auto" (default): block floating point following the data, one
+    exponent per transform found on the GPU (`exponents()`, `result()`);
+    "none" / "balanced" / "full": one static exponent (`exponent`)."""
 
     def __init__(self, dims, batch: int, in_fmt: str = "fp16", out_fmt: str | None = None,
-                 scaling: str = "balanced"):
+                 scaling: str = "auto"):
         dims = [int(d) for d in (dims if hasattr(dims, "__len__") else [dims])]
         out_fmt = out_fmt or in_fmt
         self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = dims, int(batch), in_fmt, out_fmt, scaling
@@ -136,10 +154,34 @@ class Plan:
         arr = (ctypes.c_longlong * len(dims))(*dims)
         _check(lib().mcfft_plan_many(ctypes.byref(self._h), len(dims), arr, self.batch, FORMATS[in_fmt],
                                      FORMATS[out_fmt], SCALINGS[scaling]), "mcfft_plan_many")
-        e = ctypes.c_int()
-        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
-        self.exponent = e.value
+        self.exponent = _static_exponent(self._h, scaling)
         self.n = int(np.prod(dims))
+
+    def exponents(self) -> np.ndarray:
+        """Output exponent of every transform of the last exec (true result =
+        stored * 2^e[t]); waits for the device."""
+        e = (ctypes.c_int * self.batch)()
+        _check(lib().mcfft_get_transform_exponents(self._h, e), "mcfft_get_transform_exponents")
+        return np.array(e, dtype=np.int64)
+
+    def exponent_ptr(self) -> int:
+        """Device address of the plan's per-transform exponent array (batch
+        int32), written by its execs in stream order (usable as another exec's
+        in_exp)."""
+        p = ctypes.c_void_p()
+        _check(lib().mcfft_get_exponent_buffer(self._h, ctypes.byref(p)), "mcfft_get_exponent_buffer")
+        return p.value
+
+    def result(self, out, count: int | None = None) -> np.ndarray:
+        """The first `count` transforms (default: all) of device output `out` of
+        the last exec, decoded in true units with their own exponents."""
+        count = self.batch if count is None else int(count)
+        per = self.output_elems_per_transform()
+        y = from_device(out[: count * per], self.out_fmt)
+        return (y.reshape(count, per) * np.exp2(self.exponents()[:count].astype(np.float64))[:, None]).reshape(-1)
+
+    def output_elems_per_transform(self) -> int:
+        return self.n
 
     @property
     def handle(self) -> int:
@@ -155,9 +197,10 @@ class Plan:
         _check(lib().mcfft_plan_describe(self._h, buf, 16384), "mcfft_plan_describe")
         return buf.value.decode()
 
-    def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
-        """mcfft_exec on raw device pointers (no size checks)."""
-        _check(lib().mcfft_exec(self._h, in_ptr, out_ptr, direction, stream), "mcfft_exec")
+    def exec_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0, in_exp: int = 0):
+        """mcfft_exec_exp on raw device pointers (no size checks); in_exp: device
+        address of per-transform input exponents, or 0."""
+        _check(lib().mcfft_exec_exp(self._h, in_ptr, in_exp or None, out_ptr, direction, stream), "mcfft_exec_exp")
 
     def io_bytes(self):
         """(input bytes, output bytes) of the contiguous transform: what exec
@@ -173,16 +216,28 @@ class Plan:
         _check_tensor(inp, ib, "input")
         _check_tensor(out, ob, "output")
 
-    def exec(self, inp, out, direction: int = FORWARD, stream=None):
+    def exec(self, inp, out, direction: int = FORWARD, stream=None, in_exp=None):
         """Run on torch device tensors (storage layout above) on `stream`
-        (default: torch's current stream)."""
+        (default: torch's current stream).  in_exp: the input's per-transform
+        exponents -- a Plan whose last exec produced `inp` (its exponent array),
+        a device int32 tensor of batch entries, or None (all 0)."""
         import torch
         self._check_io(inp, out)
         if stream is None:
             stream = torch.cuda.current_stream(inp.device).cuda_stream
         elif hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
-        self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream)
+        ptr = 0
+        if isinstance(in_exp, Plan):
+            if in_exp.batch != self.batch:
+                raise ValueError("in_exp plan has another batch")
+            ptr = in_exp.exponent_ptr()
+        elif in_exp is not None:
+            _check_tensor(in_exp, 4 * self.batch, "in_exp")
+            if in_exp.dtype != torch.int32:
+                raise ValueError("in_exp must be int32")
+            ptr = in_exp.data_ptr()
+        self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream, ptr)
 
     def tune(self, inp, out, direction: int = FORWARD, reps: int = 5, stream=None):
         """Time every compiled kernel variant of each pass on these buffers, keep
@@ -200,8 +255,10 @@ class Plan:
         return c.value
 
     def status_flags(self) -> int:
-        """Status bits of the last exec's persistent kernel (ragged plans; 0 = fine,
-        always 0 for other plans; waits for the device)."""
+        """Status bits (STATUS_*; 0 = fine; waits for the device): regular plans
+        report an Inf/NaN stored (static scaling) or read (AUTO) by any exec
+        since the last call, and clear it; ragged plans a dependency wait of
+        the last exec that hit its bound."""
         f = ctypes.c_uint()
         _check(lib().mcfft_get_status_flags(self._h, ctypes.byref(f)), "mcfft_get_status_flags")
         return f.value
@@ -249,6 +306,7 @@ class RaggedPlan(Plan):
     `items`: list of (dims, in_offset, out_offset) with element offsets."""
 
     def __init__(self, items, in_fmt: str = "fp16", out_fmt: str | None = None, scaling: str = "balanced"):
+        # ragged items are statically scaled ("auto" is refused)
         out_fmt = out_fmt or in_fmt
         self.in_fmt, self.out_fmt, self.scaling = in_fmt, out_fmt, scaling
         self.items = [(list(d), int(io), int(oo)) for d, io, oo in items]
@@ -286,7 +344,7 @@ class RealPlan(Plan):
     exec with INVERSE).  Real data is stored as complex pairs (x[2m], x[2m+1])."""
 
     def __init__(self, n: int, batch: int, kind: str = "r2c", in_fmt: str = "fp16", out_fmt: str | None = None,
-                 scaling: str = "balanced"):
+                 scaling: str = "auto"):
         out_fmt = out_fmt or in_fmt
         self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = [int(n)], int(batch), in_fmt, out_fmt, scaling
         self.kind = kind
@@ -294,10 +352,12 @@ class RealPlan(Plan):
         fn = lib().mcfft_plan_r2c_1d if kind == "r2c" else lib().mcfft_plan_c2r_1d
         _check(fn(ctypes.byref(self._h), int(n), int(batch), FORMATS[in_fmt], FORMATS[out_fmt], SCALINGS[scaling]),
                f"mcfft_plan_{kind}_1d")
-        e = ctypes.c_int()
-        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
-        self.exponent = e.value
+        self.exponent = _static_exponent(self._h, scaling)
         self.n = int(n)
+
+    def output_elems_per_transform(self) -> int:
+        """complex elements per transform of the output (r2c: n/2 + 1; c2r: n/2 pairs)"""
+        return self.n // 2 + 1 if self.kind == "r2c" else self.n // 2
 
     def exec_io_bytes(self):
         h, c = self.n // 2, (self.n // 2 + 1)
@@ -311,7 +371,7 @@ class LayoutPlan(Plan):
     None for contiguous interleaved) on each side."""
 
     def __init__(self, dims, batch: int, in_layout: Layout | None, out_layout: Layout | None,
-                 in_fmt: str = "fp16", out_fmt: str | None = None, scaling: str = "balanced"):
+                 in_fmt: str = "fp16", out_fmt: str | None = None, scaling: str = "auto"):
         dims = [int(d) for d in (dims if hasattr(dims, "__len__") else [dims])]
         out_fmt = out_fmt or in_fmt
         self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = dims, int(batch), in_fmt, out_fmt, scaling
@@ -322,9 +382,7 @@ class LayoutPlan(Plan):
         _check(lib().mcfft_plan_many_layout(ctypes.byref(self._h), len(dims), arr, self.batch, ref(in_layout),
                                             ref(out_layout), FORMATS[in_fmt], FORMATS[out_fmt], SCALINGS[scaling]),
                "mcfft_plan_many_layout")
-        e = ctypes.c_int()
-        _check(lib().mcfft_get_output_exponent(self._h, ctypes.byref(e)), "mcfft_get_output_exponent")
-        self.exponent = e.value
+        self.exponent = _static_exponent(self._h, scaling)
         self.n = int(np.prod(dims))
 
     def _side_bytes(self, lay, fmt):

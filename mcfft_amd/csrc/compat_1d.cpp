@@ -6,16 +6,17 @@
 
 using namespace mcfft_compat;
 
-void mcfftCreate(mcfftHandle* plan, int n, int n_batch) {
+static void create(mcfftHandle* plan, int n, int n_batch, mcfftScaling scaling) {
   plan->N = n;
   plan->N_batch = n_batch;
   for (int i = 0; i < 9; ++i) plan->radices[i] = 16;
   plan->radices[0] = n;
   plan->n_radices = 1;
   for (int i = 0; i < 3; ++i) plan->mergings[i] = 0;
+  plan->n_mergings = 0;
   plan->impl = nullptr;
   const long long d = n;
-  if (!report(mcfft_plan_many(&plan->impl, 1, &d, n_batch, storage_format(), storage_format(), MCFFT_SCALE_NONE),
+  if (!report(mcfft_plan_many(&plan->impl, 1, &d, n_batch, storage_format(), storage_format(), scaling),
               "mcfftCreate"))
     return;
   int np = 0;
@@ -23,64 +24,41 @@ void mcfftCreate(mcfftHandle* plan, int n, int n_batch) {
   plan->n_mergings = np;
 }
 
-void mcfftExec(mcfftHandle plan, MCTYPE* data) {
-  if (!plan.impl) {
-    report(MCFFT_INVALID_PLAN, "mcfftExec");
-    return;
-  }
-  report(mcfft_exec(plan.impl, data, data, MCFFT_FORWARD, nullptr), "mcfftExec");
-}
+// the reference's contract: unnormalised forward transform in place
+void mcfftCreate(mcfftHandle* plan, int n, int n_batch) { create(plan, n, n_batch, MCFFT_SCALE_NONE); }
+
+void mcfftExec(mcfftHandle plan, MCTYPE* data) { exec_in_place(plan.impl, data, nullptr, "mcfftExec"); }
 
 void mcfftDestroy(mcfftHandle* plan) {
-  if (plan->impl) mcfft_destroy(plan->impl);
+  if (plan->impl) {
+    forget_plan(plan->impl);
+    mcfft_destroy(plan->impl);
+  }
   plan->impl = nullptr;
 }
 
-mcfftStatus mcfftGetLastError(void) { return last_error(); }
+mcfftStatus mcfftGetLastError(void) { return get_last_error(); }
 
 // ---- utils layer: one global plan, as in source/mcfft_1d_utils.cpp:3-6 ----
-static int g_n, g_batch;
-static std::vector<MCTYPE> g_host;
-static MCTYPE* g_dev = nullptr;
+static UtilsState<MCTYPE> g_u;
 static mcfftHandle g_plan;
 
 void setup(double* data, int n, int n_batch) {
-  g_n = n;
-  g_batch = n_batch;
-  mcfftCreate(&g_plan, n, n_batch);
+  mcfftDestroy(&g_plan);
+  create(&g_plan, n, n_batch, MCFFT_SCALE_AUTO);
   std::vector<int> rev(n);
   gen_rev(n, rev.data(), g_plan.radices, g_plan.n_radices);
-  g_host.resize((size_t)2 * n * n_batch);
+  g_u.host.resize((size_t)2 * n * n_batch);
   for (long long j = 0; j < n_batch; ++j)
     for (long long i = 0; i < n; ++i) {
-      g_host[2 * n * j + 2 * i + 0] = (MCTYPE)(float)data[2 * n * j + 2 * rev[i] + 0];
-      g_host[2 * n * j + 2 * i + 1] = (MCTYPE)(float)data[2 * n * j + 2 * rev[i] + 1];
+      g_u.host[2 * n * j + 2 * i + 0] = (MCTYPE)(float)data[2 * n * j + 2 * rev[i] + 0];
+      g_u.host[2 * n * j + 2 * i + 1] = (MCTYPE)(float)data[2 * n * j + 2 * rev[i] + 1];
     }
-  if (g_dev) (void)hipFree(g_dev);
-  g_dev = nullptr;
-  if (hipMalloc(&g_dev, sizeof(MCTYPE) * g_host.size()) != hipSuccess) {
-    report(MCFFT_ALLOC_FAILED, "setup");
-    return;
-  }
-  if (hipMemcpy(g_dev, g_host.data(), sizeof(MCTYPE) * g_host.size(), hipMemcpyHostToDevice) != hipSuccess)
-    report(MCFFT_HIP_ERROR, "setup");
+  upload(g_u, n_batch, n);
 }
 
-void finalize(double* result) {
-  if (!g_dev) {
-    report(MCFFT_INVALID_VALUE, "finalize");
-    return;
-  }
-  if (hipMemcpy(g_host.data(), g_dev, sizeof(MCTYPE) * g_host.size(), hipMemcpyDeviceToHost) != hipSuccess) {
-    report(MCFFT_HIP_ERROR, "finalize");
-    return;
-  }
-  for (size_t i = 0; i < g_host.size(); ++i) result[i] = (double)(float)g_host[i];
-}
+void finalize(double* result) { download(g_u, g_plan.impl, result); }
 
-void doit(int iter) {
-  for (int t = 0; t < iter; ++t) mcfftExec(g_plan, g_dev);
-  if (hipDeviceSynchronize() != hipSuccess) report(MCFFT_HIP_ERROR, "doit");
-}
+void doit(int iter) { run(g_u, g_plan.impl, iter); }
 
 void generate_data(double* data, int n, int n_batch, int seed) { generate(data, n, n_batch, seed); }

@@ -7,7 +7,7 @@
 
 using namespace mcfft_compat;
 
-static void create(mcfftHandle* plan, int rank, int nx, int ny, int nz, int n_batch) {
+static void create(mcfftHandle* plan, int rank, int nx, int ny, int nz, int n_batch, mcfftScaling scaling) {
   plan->Nx = nx;
   plan->Ny = ny;
   plan->Nz = nz;
@@ -20,70 +20,47 @@ static void create(mcfftHandle* plan, int rank, int nx, int ny, int nz, int n_ba
   plan->mergings[0] = plan->mergings[1] = plan->mergings[2] = 0;
   plan->impl = nullptr;
   const long long d[3] = {nx, ny, nz};
-  report(mcfft_plan_many(&plan->impl, rank, d, n_batch, storage_format(), storage_format(), MCFFT_SCALE_NONE),
-         "mcfftCreate");
+  report(mcfft_plan_many(&plan->impl, rank, d, n_batch, storage_format(), storage_format(), scaling), "mcfftCreate");
 }
 
-void mcfftCreate(mcfftHandle* plan, int nx, int ny, int n_batch) { create(plan, 2, nx, ny, 1, n_batch); }
-void mcfftCreate(mcfftHandle* plan, int nx, int ny, int nz, int n_batch) { create(plan, 3, nx, ny, nz, n_batch); }
-
-void mcfftExec(mcfftHandle plan, MCTYPE* data) {
-  if (!plan.impl) {
-    report(MCFFT_INVALID_PLAN, "mcfftExec");
-    return;
-  }
-  report(mcfft_exec(plan.impl, data, data, MCFFT_FORWARD, nullptr), "mcfftExec");
+// the reference's contract: unnormalised forward transform in place
+void mcfftCreate(mcfftHandle* plan, int nx, int ny, int n_batch) {
+  create(plan, 2, nx, ny, 1, n_batch, MCFFT_SCALE_NONE);
 }
+void mcfftCreate(mcfftHandle* plan, int nx, int ny, int nz, int n_batch) {
+  create(plan, 3, nx, ny, nz, n_batch, MCFFT_SCALE_NONE);
+}
+
+void mcfftExec(mcfftHandle plan, MCTYPE* data) { exec_in_place(plan.impl, data, nullptr, "mcfftExec"); }
 
 void mcfftDestroy(mcfftHandle* plan) {
-  if (plan->impl) mcfft_destroy(plan->impl);
+  if (plan->impl) {
+    forget_plan(plan->impl);
+    mcfft_destroy(plan->impl);
+  }
   plan->impl = nullptr;
 }
 
-mcfftStatus mcfftGetLastError(void) { return last_error(); }
+mcfftStatus mcfftGetLastError(void) { return get_last_error(); }
 
-static std::vector<MCTYPE> g_host;
-static MCTYPE* g_dev = nullptr;
+// ---- utils layer: as 2D, plus the (nx, ny, nz, n_batch) overload ----
+static UtilsState<MCTYPE> g_u;
 static mcfftHandle g_plan;
 
-static void setup_any(double* data, long long total) {
-  g_host.resize((size_t)2 * total);
-  for (long long i = 0; i < 2 * total; ++i) g_host[i] = (MCTYPE)(float)data[i];  // identity digit reversal
-  if (g_dev) (void)hipFree(g_dev);
-  g_dev = nullptr;
-  if (hipMalloc(&g_dev, sizeof(MCTYPE) * g_host.size()) != hipSuccess) {
-    report(MCFFT_ALLOC_FAILED, "setup");
-    return;
-  }
-  if (hipMemcpy(g_dev, g_host.data(), sizeof(MCTYPE) * g_host.size(), hipMemcpyHostToDevice) != hipSuccess)
-    report(MCFFT_HIP_ERROR, "setup");
+static void setup_any(double* data, int rank, int nx, int ny, int nz, int n_batch) {
+  mcfftDestroy(&g_plan);
+  create(&g_plan, rank, nx, ny, nz, n_batch, MCFFT_SCALE_AUTO);
+  const long long per = (long long)nx * ny * nz, total = per * n_batch;
+  g_u.host.resize((size_t)2 * total);
+  for (long long i = 0; i < 2 * total; ++i) g_u.host[i] = (MCTYPE)(float)data[i];  // identity digit reversal
+  upload(g_u, n_batch, per);
 }
 
-void setup(double* data, int nx, int ny, int n_batch) {
-  mcfftCreate(&g_plan, nx, ny, n_batch);
-  setup_any(data, (long long)nx * ny * n_batch);
-}
+void setup(double* data, int nx, int ny, int n_batch) { setup_any(data, 2, nx, ny, 1, n_batch); }
+void setup(double* data, int nx, int ny, int nz, int n_batch) { setup_any(data, 3, nx, ny, nz, n_batch); }
 
-void setup(double* data, int nx, int ny, int nz, int n_batch) {
-  mcfftCreate(&g_plan, nx, ny, nz, n_batch);
-  setup_any(data, (long long)nx * ny * nz * n_batch);
-}
+void finalize(double* result) { download(g_u, g_plan.impl, result); }
 
-void finalize(double* result) {
-  if (!g_dev) {
-    report(MCFFT_INVALID_VALUE, "finalize");
-    return;
-  }
-  if (hipMemcpy(g_host.data(), g_dev, sizeof(MCTYPE) * g_host.size(), hipMemcpyDeviceToHost) != hipSuccess) {
-    report(MCFFT_HIP_ERROR, "finalize");
-    return;
-  }
-  for (size_t i = 0; i < g_host.size(); ++i) result[i] = (double)(float)g_host[i];
-}
-
-void doit(int iter) {
-  for (int t = 0; t < iter; ++t) mcfftExec(g_plan, g_dev);
-  if (hipDeviceSynchronize() != hipSuccess) report(MCFFT_HIP_ERROR, "doit");
-}
+void doit(int iter) { run(g_u, g_plan.impl, iter); }
 
 void generate_data(double* data, int n, int n_batch, int seed) { generate(data, n, n_batch, seed); }

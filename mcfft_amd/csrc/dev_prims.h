@@ -37,6 +37,10 @@ MCFFT_DEV int thread_id() { return threadIdx.x; }
 MCFFT_DEV int block_id() { return blockIdx.x; }
 MCFFT_DEV int grid_size() { return gridDim.x; }
 MCFFT_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+MCFFT_DEV long long uniform64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
 MCFFT_DEV const void* uniform_ptr(const void* p) {
   const unsigned long long v = (unsigned long long)p;
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
@@ -63,6 +67,8 @@ MCFFT_DEV int opaque(int v) {
   return v;
 }
 MCFFT_DEV void block_barrier() { __syncthreads(); }
+// the compiler may not move memory accesses across this point
+MCFFT_DEV void mem_order_point() { asm volatile("" ::: "memory"); }
 
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
 template <class T> MCFFT_DEV void lds_st(T* p, int i, T v) { p[i] = v; }
@@ -104,6 +110,31 @@ MCFFT_DEV unsigned atomic_load_dev(const unsigned* p) {
 }
 MCFFT_DEV void atomic_or_dev(unsigned* p, unsigned v) {
   (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MCFFT_DEV void atomic_max_dev64(unsigned long long* p, unsigned long long v) {
+  (void)__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MCFFT_DEV void atomic_add_dev64(unsigned long long* p, unsigned long long v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// maximum of v over the 64 lanes of the wave, as a wave-uniform value (every
+// lane takes part): xor butterflies inside each 32-lane half with ds_swizzle
+// (the pattern is an immediate, so no lane-address registers are needed, unlike
+// ds_bpermute), then the two halves' results combined in scalar registers
+MCFFT_DEV unsigned wave_max(unsigned v) {
+#define MCFFT_SWZ_MAX(pattern)                                                  \
+  {                                                                             \
+    const unsigned w = (unsigned)__builtin_amdgcn_ds_swizzle((int)v, pattern);  \
+    v = w > v ? w : v;                                                          \
+  }
+  MCFFT_SWZ_MAX(0x041f)  // lane ^ 1
+  MCFFT_SWZ_MAX(0x081f)  // lane ^ 2
+  MCFFT_SWZ_MAX(0x101f)  // lane ^ 4
+  MCFFT_SWZ_MAX(0x201f)  // lane ^ 8
+  MCFFT_SWZ_MAX(0x401f)  // lane ^ 16
+#undef MCFFT_SWZ_MAX
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0), b = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+  return a > b ? a : b;
 }
 // device-scope release / acquire, one side each: a producer's release (writes back
 // its XCD's dirty L2 lines) before it counts a tile, a consumer's acquire (drops

@@ -27,7 +27,8 @@ struct BigCfg {
   static_assert(NW_ == 8 || NW_ == 16, "8 or 16 waves");
   static constexpr int HALF = 32768;
   static constexpr int LDS_ELEMS = HALF + HALF / 16;
-  static constexpr int LDS_BYTES = LDS_ELEMS * 4;
+  static constexpr int RED_OFF = LDS_ELEMS * 4;  // then one word per wave for block reductions
+  static constexpr int LDS_BYTES = RED_OFF + 64;
   // LDS slot of position m inside its round: drop bit 13, pad every 16 elements
   //   lidx(m) = c + (c >> 4),  c = (m & 0x1fff) | ((m >> 14) << 13).
   // Column j of MFMA tile k is low + 32 NW k (low = 32 wave + lane%32 < 32 NW).
@@ -64,9 +65,11 @@ MCFFT_DEV GBuf big_view(const void* p, long long sg, long long st, int tpg, int 
 // MFMA tile k and, if next >= 0, immediately loads tile k of transform `next`
 // into the registers it just freed, so the next transform's loads overlap this
 // one's stores.
-template <int CM, int NW>
+// AUTO: the kernel serves MCFFT_SCALE_AUTO plans (first-pass normalisation and
+// output exponents); otherwise statically scaled plans (output exponents).
+template <int CM, int NW, bool AUTO>
 MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A,
-                            typename Traits<CM>::T (&x)[BigCfg<NW>::TPW][8]) {
+                            typename Traits<CM>::T (&x)[BigCfg<NW>::TPW][8], SlotAcc& sacc) {
   static_assert(CM == CM_F16 || CM == CM_BF16, "4-byte elements only");
   typedef Traits<CM> Tr;
   typedef typename Tr::T El;
@@ -85,6 +88,22 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
 
   El tmp[B::NLATE][4];
   constexpr int T0[4] = {0, 1, 4, 5}, T1[4] = {2, 3, 6, 7};
+
+  // block floating point (bfp.h): an AUTO first pass normalises the transform
+  // from its largest input |component| (stage 0's scale carries 2^n)
+  const int bf = a.bfp_flags;
+  int e_out = a.bfp_e_base;
+  float nsc = 1.0f;
+  if (AUTO && (bf & BFP_FIRST)) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < B::TPW; ++k)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc = pk_absmax(acc, x[k][t]);
+    const int grp = tile / a.tiles_per_group;
+    nsc = bfp_first_tile(a, acc, CM, (unsigned*)((unsigned char*)lds + B::RED_OFF), NW, tile,
+                         a.bfp_gpt > 0 ? grp / a.bfp_gpt : 0, e_out, sacc);
+  }
 
   For<0, 4>::run([&](auto pc) __attribute__((always_inline)) {
     constexpr int P = decltype(pc)::value;
@@ -112,7 +131,7 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
           const int j = low + 32 * NW * k;
           const int mo0 = (int)(((unsigned)j / NS) * NS * 16 + ((unsigned)j % NS)) + (4 * h + 8 * g) * NS;
           const int r2 = mo0 >> 12;
-          const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : a.st_scale[P];
+          const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : P == 0 ? a.st_scale[0] * nsc : a.st_scale[P];
           const f32x2 d = twiddle_rev(r2 * NS, INV_M, a.inverse != 0, 1.0f);
           tw[0] = twiddle_rev(r2 * (mo0 & (16 * NS - 1)), INV_M, a.inverse != 0, sc);
 #pragma unroll
@@ -127,7 +146,8 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
           if constexpr (P < 3) {
             x[k][s] = Tr::pack(cmul(v, tw[u]));
           } else {
-            buf_st32s(gout, vout, (k * 32 * NW + B::rrc(s) * B::LQ) * 4, Tr::pack(v));
+            const El e = Tr::pack(v);
+            buf_st32s(gout, vout, (k * 32 * NW + B::rrc(s) * B::LQ) * 4, e);
           }
         }
       }
@@ -177,9 +197,16 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
       block_barrier();
     }
   });
+  if (bf & BFP_LAST) {
+    const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
+    // no Inf/NaN watch over the stores: every form tried (a fold of the packed
+    // values, a maximum or compares of the FP32 results) made the static kernel
+    // spill hundreds of bytes per lane; AUTO plans cannot overflow
+    bfp_last_tile(a, 1, til, grp, til, a.bfp_gpt > 0 ? grp / a.bfp_gpt : 0, e_out, false);
+  }
 }
 
-template <int CM, int NW>
+template <int CM, int NW, bool AUTO>
 MCFFT_DEV void fft_big_body(const PassArgs& a) {
   typedef BigCfg<NW> B;
   const AOps A = load_aops(a.aops, lane_id());
@@ -195,10 +222,13 @@ MCFFT_DEV void fft_big_body(const PassArgs& a) {
 #pragma unroll
       for (int t = 0; t < 8; ++t) x[k][t] = buf_ld32s(gin, vin, k * 128 * NW + t * 16384);
   }
+  SlotAcc sacc;
+  if (AUTO && (a.bfp_flags & BFP_FIRST)) sacc.epoch = (unsigned long long)uniform64(g_ld(&a.bfp_state->epoch, 0));
   for (; tile < a.n_tiles; tile += g) {
     const int next = tile + g < a.n_tiles ? tile + g : -1;
-    fft_big_tile<CM, NW>(a, tile, next, A, x);
+    fft_big_tile<CM, NW, AUTO>(a, tile, next, A, x, sacc);
   }
+  if constexpr (AUTO) slot_flush(sacc, a.bfp_slots);
 }
 
 }  // namespace mcfft

@@ -34,6 +34,7 @@
 // natural-order input, table twiddles generated in FP64, explicit MFMA lane
 // layout, one kernel template for rows, columns and four-step passes.
 #pragma once
+#include "bfp.h"
 #include "dev_prims.h"
 
 namespace mcfft {
@@ -74,6 +75,21 @@ struct PassArgs {
   long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
   int stage_tw_entries, fs_tw_entries;
   float st_scale[6];          // 2^-shift of stage P (twiddles computed in the kernel carry it)
+  // block floating point and exec bookkeeping (bfp.h); zero flags: static
+  // scaling, nothing recorded
+  int bfp_flags;
+  int bfp_quantum;            // AUTO first pass: step of the normalisation exponent
+  int bfp_e_base;             // exponent the pass's stores add (first pass: its stage shifts; later
+                              // passes: the shifts of passes 1..this; static last pass: the plan's)
+  int bfp_tr0;                // global index of the launch's first transform
+  int bfp_gpt;                // groups of this pass per transform
+  int bfp_nt;                 // transforms of the launch
+  int bfp_q0_lg, bfp_tpg0;    // first pass: log2 of its tile width, tiles per group
+  ExecState* bfp_state;
+  unsigned long long* bfp_slots;  // [2t]: max e, [2t+1]: min e of t's first-pass tiles (see bfp.h)
+  signed char* bfp_tile_e;    // exponents of the launch's first-pass tiles
+  int* bfp_out_exp;           // per transform
+  const int* bfp_in_exp;      // per transform (caller's input exponents), or null
 };
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:
@@ -97,6 +113,10 @@ MCFFT_DEV uint32_t pk_f16(f32x2 v) { return __builtin_bit_cast(uint32_t, __built
 MCFFT_DEV uint32_t pk_bf16(f32x2 v) { return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2)); }
 MCFFT_DEV f32x2 up_f16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(f16x2, e), f32x2); }
 MCFFT_DEV f32x2 up_bf16(uint32_t e) { return __builtin_convertvector(__builtin_bit_cast(bf16x2, e), f32x2); }
+// product of two packed binary16 pairs (v_pk_mul_f16, RNE)
+MCFFT_DEV uint32_t pk_mul_f16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(f16x2, a) * __builtin_bit_cast(f16x2, b));
+}
 
 // Byte offset of element i (esz bytes each) inside a buffer view.  Views span up
 // to 2^32 bytes, so the product is formed in unsigned 32-bit arithmetic (element
@@ -293,7 +313,7 @@ template <int E> struct For<E, E> {
 template <int S, int T, int Q, int NW, int CM, int KIND>
 struct PassCfg {
   static constexpr int L = ipow16(S) * T;
-  static constexpr int Q_ = Q, NW_ = NW;
+  static constexpr int Q_ = Q, NW_ = NW, CM_ = CM;
   static constexpr int NT = NW * 64;
   static constexpr int NSTAGE = S + (T > 1 ? 1 : 0);
   static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF);
@@ -302,7 +322,9 @@ struct PassCfg {
   static constexpr int ROW = L + L / 16 + 1;  // padded LDS row (contiguous-row layout)
   static constexpr int LDS_ELEMS = QFAST ? Q * L : Q * ROW;
   typedef typename Traits<CM>::T El;
-  static constexpr int LDS_BYTES = LDS_ELEMS * (int)sizeof(El);
+  // the tile's data, then one word per wave for block-wide reductions
+  static constexpr int RED_OFF = LDS_ELEMS * (int)sizeof(El);
+  static constexpr int LDS_BYTES = RED_OFF + 64;
   static constexpr int NCOLT = Q * L / 512;          // 32-column MFMA tiles of the tile
   static constexpr int TPW = (NCOLT + NW - 1) / NW;  // MFMA tiles per wave
   // L = 512: the radix-2 tail is done right after the last radix-16 stage
@@ -359,10 +381,87 @@ MCFFT_DEV void load_stage0(const PassArgs& a, GBuf gin, int wave, int lane, type
   }
 }
 
+// AUTO first pass: `acc` holds this thread's largest |component| of the tile's
+// input (packed, see pk_absmax).  Returns the tile's normalisation 2^n, sets
+// e_out (the stored output's exponent) and, unless the pass is also the last,
+// records the tile's exponent (byte) and folds it into its transform's slots.
+// An all-zero tile keeps 2^0 and records no exponent; an Inf/NaN input sets the
+// status bit and is treated like it.  Ends after a barrier (block_max).
+MCFFT_DEV float bfp_first_tile(const PassArgs& a, uint32_t acc, int cm, unsigned* red, int nw, int tile, int t_loc,
+                               int& e_out, SlotAcc& sacc) {
+  const int tid = thread_id();
+  const uint32_t mb = block_max(pk_fold(acc), red, nw, tid >> 6, tid & 63);
+  const bool nonfinite = bits_nonfinite(mb, cm), zero = nonfinite || mb == 0;
+  if (nonfinite && tid == 0) atomic_or_dev(&a.bfp_state->status, kStatusNonFinite);
+  const int n = zero ? 0 : bfp_norm(mb, cm, a.bfp_quantum);
+  e_out = a.bfp_e_base - n;
+  if (!(a.bfp_flags & BFP_LAST)) {
+    if (tid == 0) g_st(a.bfp_tile_e, (long long)tile, (signed char)(zero ? kBfpZeroTile : e_out));
+    slot_note(sacc, a.bfp_slots, (long long)a.bfp_tr0 + t_loc, e_out, zero);
+  }
+  return exp2i(n);
+}
+
+// Last pass, end of a tile: the per-transform output exponents (a single-pass
+// tile's instances are transforms, q < Q; otherwise the first tile of the
+// transform's first group writes it) and, for a statically scaled plan, the
+// status bit when one of this thread's stores was Inf/NaN.
+MCFFT_DEV void bfp_last_tile(const PassArgs& a, int Q, int inst0, int grp, int til, int t_loc, int e_out,
+                             bool stored_nonfinite) {
+  const int tid = thread_id();
+  const bool single = (a.bfp_flags & BFP_SINGLE) != 0;
+  if (single ? (tid < Q && inst0 + tid < a.n_inst) : (tid == 0 && til == 0 && grp == t_loc * a.bfp_gpt)) {
+    const long long t = (long long)a.bfp_tr0 + (single ? inst0 + tid : t_loc);
+    g_st(a.bfp_out_exp, t, e_out + (a.bfp_in_exp ? g_ld(a.bfp_in_exp, t) : 0));
+  }
+  if ((a.bfp_flags & BFP_CHECK) && stored_nonfinite) atomic_or_dev(&a.bfp_state->status, kStatusNonFinite);
+}
+
+// AUTO scaling, pass after the first, when the first pass's tiles of this
+// transform chose different exponents: bring every stage-0 input to the common
+// exponent E (the largest), multiplying by 2^(e_tile - E) <= 1.  Element (q, m) of
+// group `grp` is instance i0 = (grp % gpt) * L + m of first-pass group grp / gpt
+// (2D and four-step 1D: gpt = 1, i0 = m; 3D y-axis pass: gpt = NX), made by
+// first-pass tile (grp / gpt) * tpg0 + (i0 >> q0_lg).
+template <class C, class Tr>
+MCFFT_DEV void rescale_stage0(const PassArgs& a, int wave, int lane, int grp, int E, typename C::El (*x)[8]) {
+  constexpr int L = C::L;
+  const int h = lane >> 5;
+  const int g0 = grp / a.bfp_gpt, ibase = (grp - g0 * a.bfp_gpt) * L;
+  const long long tbase = (long long)g0 * a.bfp_tpg0;
+  for (int k = 0; k < C::TPW; ++k) {
+    const int ti = wave + k * C::NW_;
+    if (ti < C::NCOLT) {
+      int q, j;
+      C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+      for (int t = 0; t < 8; ++t) {
+        const int m = j + (8 * h + t) * (L / 16);
+        const int e = (int)g_ld(a.bfp_tile_e, tbase + ((ibase + m) >> a.bfp_q0_lg));
+        const int d = e - E;
+        if constexpr (C::CM_ == CM_F16) {
+          // 2^d as binary16 (normal down to 2^-14, subnormal to 2^-24), both
+          // halves; one packed FP16 multiply per pair, rounded as a repack of
+          // the FP32 product would be
+          const uint32_t f16 = (e == kBfpZeroTile || d < -24) ? 0u
+                               : d >= -14 ? (uint32_t)(15 + d) << 10 : 1u << (24 + d);
+          x[k][t] = pk_mul_f16(x[k][t], f16 | (f16 << 16));
+        } else {
+          // BF16; split through FP32, so that bits a subnormal hi part loses move
+          // into the lo part
+          const float f = (e == kBfpZeroTile || d < -126) ? 0.0f : exp2i(d);
+          x[k][t] = Tr::pack(Tr::unpack(x[k][t]) * f);
+        }
+      }
+    }
+  }
+}
+
 // One tile of a pass: stages on the tile's Q instances, ending with a barrier
-// (LDS is reused by the workgroup's next tile).
-template <int S, int T, int Q, int NW, int CM, int KIND>
-MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
+// (LDS is reused by the workgroup's next tile).  BFP: the tile takes part in the
+// block floating point and exec bookkeeping of bfp.h (off in the ragged kernel,
+// whose passes are statically scaled).
+template <int S, int T, int Q, int NW, int CM, int KIND, bool BFP = true>
+MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sacc = nullptr) {
   typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
@@ -382,7 +481,8 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
   if constexpr (FOURSTEP) gfs = make_gbuf(a.fs_tw, (unsigned)a.fs_tw_entries * 8u);
 
   {
-    const int grp = tile / a.tiles_per_group, til = tile - grp * a.tiles_per_group;
+    // wave-uniform (scalar registers): kept to the end of the tile by the bookkeeping
+    const int grp = uniform(tile / a.tiles_per_group), til = uniform(tile - grp * a.tiles_per_group);
     const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
     const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
     const int inst0 = til * Q;
@@ -398,15 +498,36 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
     const GBuf gin = clamp_view(a.in, ib, in_end, esz_in);
     const GBuf gout = clamp_view(a.out, ob, out_end, esz_out);
 
+    // ---- block floating point (bfp.h) ----
+    // Row and four-step column passes are always a plan's first pass; column and
+    // transposed-row passes never are (so only they can need reconciling).
+    constexpr bool FIRST_KIND = (KIND == KIND_ROW || KIND == KIND_COLF);
+    const int bf = BFP ? a.bfp_flags : 0;
+    const int t_loc = (BFP && a.bfp_gpt > 0) ? uniform(grp / a.bfp_gpt) : 0;  // transform of the tile in the launch
+    float nsc = 1.0f;        // AUTO first pass: 2^n applied to the stage-0 results
+    int e_out = BFP ? a.bfp_e_base : 0;  // exponent of the stored output (true = stored * 2^e_out)
+    bool rescale = false;    // AUTO reconcile pass: the transform's first-pass tiles disagree
+    uint32_t watch = ~0u;    // static last pass: Inf/NaN watch over this thread's stores (bfp.h)
+    if (BFP && !FIRST_KIND && (bf & BFP_AUTO)) {
+      int emax, emin;
+      slot_read(a.bfp_slots, (long long)a.bfp_tr0 + t_loc, emax, emin);
+      e_out = uniform(e_out + emax);
+      rescale = (bf & BFP_RECONCILE) && uniform(emin) != uniform(emax);
+    }
+
     // Scale factors.  With computed twiddles the last stage's power-of-two scale
     // rides on the twiddle the stage before it writes (the stages are linear), so
     // the outputs need no extra multiply; twiddle tables carry each stage's own.
+    // An AUTO first pass's normalisation nsc multiplies stage 0's results.
     constexpr bool TABLE_TW = (CM == CM_SPLIT) && kSplitTableTwiddles;
     constexpr bool FOLD = !TABLE_TW && C::NSTAGE >= 2;
     const float last_scale = FOLD ? 1.0f : a.out_scale;
     auto stage_scale = [&](int P) -> float {  // scale carried by the twiddle stage P writes
-      return (FOLD && P == C::NSTAGE - 2) ? a.st_scale[P] * a.st_scale[P + 1] : a.st_scale[P];
+      const float s = (FOLD && P == C::NSTAGE - 2) ? a.st_scale[P] * a.st_scale[P + 1] : a.st_scale[P];
+      return P == 0 ? s * nsc : s;
     };
+    // scale of a single-stage pass's outputs (stage 0 is its last)
+    auto single_scale = [&]() -> float { return C::NSTAGE == 1 ? a.out_scale * nsc : a.out_scale; };
 
     // output post-processing of the pass's last stage: four-step twiddle or scale
     // four-step twiddle W_N^(m * column), N = L * fs_cols (m < L, column < fs_cols)
@@ -424,12 +545,15 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       } else if constexpr (FOLD) {
         return v;
       } else {
-        return v * a.out_scale;
+        return v * single_scale();
       }
     };
-    auto store_out = [&](int q, int m, El e) {
-      Tr::store(gout, C::out_off(q, m, sout), e);
+    const bool check = BFP && (bf & BFP_CHECK);
+    auto emit = [&](int off, El e) {
+      if (check) watch = nonfinite_fold(watch, e, CM);
+      Tr::store(gout, off, e);
     };
+    auto store_out = [&](int q, int m, El e) { emit(C::out_off(q, m, sout), e); };
 
     // ---------------- radix-16 MFMA stages ----------------
     For<0, S>::run([&](auto pc) {
@@ -461,6 +585,17 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       El x[TPW][8];
       if constexpr (P == 0) {
         load_stage0<C, Tr>(a, gin, wave, lane, x);
+        if (BFP && FIRST_KIND && (bf & BFP_FIRST)) {
+          // AUTO first pass: the tile's largest |component| sets its normalisation
+          uint32_t acc = 0;
+          for (int k = 0; k < TPW; ++k)
+            if (wave + k * NW < NCOLT)
+              for (int t = 0; t < 8; ++t) acc = pk_absmax(acc, x[k][t]);
+          nsc = bfp_first_tile(a, acc, CM, (unsigned*)((unsigned char*)lds + C::RED_OFF), NW, tile, t_loc, e_out,
+                               *sacc);
+        } else if (BFP && !FIRST_KIND && rescale) {
+          rescale_stage0<C, Tr>(a, wave, lane, grp, e_out - a.bfp_e_base, x);
+        }
       } else {
         for (int k = 0; k < TPW; ++k) {
           const int ti = wave + k * NW;
@@ -559,6 +694,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
               if constexpr (TABLE_TW) {
                 // W^E: table entry rr = 4h + 8g of row j (FP64-generated, scale folded in)
                 tw[0] = __builtin_bit_cast(f32x2, buf_ld64(gtw, (P * L + j * 16 + 4 * h + 8 * g) * 8));
+                if constexpr (P == 0) tw[0] = tw[0] * nsc;
               } else {
                 tw[0] = twiddle_rev(r2 * (mo0 & (NSN - 1)), INV_M, inv, stage_scale(P));
               }
@@ -604,9 +740,9 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
                     if (u > 0) fw = cmul(fw, fs_d);
                     v = cmul(v, fw);
                   } else if constexpr (!FOLD) {
-                    v = v * a.out_scale;
+                    v = v * single_scale();
                   }
-                  Tr::store(gout, o0 + u * ostep, Tr::pack(v));
+                  emit(o0 + u * ostep, Tr::pack(v));
                 }
               }
             }
@@ -650,8 +786,13 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A) {
       const int q = (int)((unsigned)tid % Q), m0 = (int)((unsigned)tid / Q);
       const int vo = C::out_off(q, m0, sout);
       for (int i = 0; i < Q * L / C::NT; ++i)
-        Tr::store_s(gout, vo, i * MSTEP * sout, lds_ld(lds, C::lidx(q, m0 + i * MSTEP)));
+      {
+        const El e = lds_ld(lds, C::lidx(q, m0 + i * MSTEP));
+        if (check) watch = nonfinite_fold(watch, e, CM);
+        Tr::store_s(gout, vo, i * MSTEP * sout, e);
+      }
     }
+    if (BFP && (bf & BFP_LAST)) bfp_last_tile(a, Q, inst0, grp, til, t_loc, e_out, nonfinite_seen(watch));
     block_barrier();  // LDS is reused by the next tile
   }
 }
@@ -671,7 +812,13 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   if constexpr (XCD_GROUP) {
     if (b < (g & ~31)) b = (b & ~31) + 4 * (b & 7) + ((b >> 3) & 3);
   }
-  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A);
+  SlotAcc sacc;
+  if (a.bfp_flags & BFP_FIRST) sacc.epoch = (unsigned long long)uniform64(g_ld(&a.bfp_state->epoch, 0));
+  // the last pass of an AUTO multi-pass exec advances the epoch (nothing in it
+  // reads the epoch; the next first pass is a later launch)
+  if ((a.bfp_flags & BFP_EPOCH) && block_id() == 0 && thread_id() == 0) atomic_add_dev64(&a.bfp_state->epoch, 1);
+  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A, &sacc);
+  slot_flush(sacc, a.bfp_slots);
 }
 
 }  // namespace mcfft

@@ -132,6 +132,7 @@ struct KernelInfo {
   const void* fn;   // __global__ entry (host stub); launched with hipLaunchKernel
   int S, T, Q, nw, cm, kind;
   int lds_bytes;
+  int mode = 0;     // scaling it serves: 0 any, 1 static (NONE/BALANCED/FULL) only, 2 AUTO only
 };
 // tables defined in each kernel translation unit
 extern const KernelInfo kRowKernels[];

@@ -23,6 +23,7 @@
 #include "real_pass.h"
 #include "layout_pass.h"
 #include "ragged_plan.h"
+#include "exec_args.h"
 
 using namespace mcfft;
 
@@ -36,6 +37,13 @@ struct mcfftPlan_st {
   std::vector<int> grid_cap;              // resident workgroups per pass (occupancy x CUs)
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // exec bookkeeping (bfp.h): status word and last-launch counter, per-transform
+  // output exponents, and for AUTO multi-pass plans the exponent slots of every
+  // transform and the first pass's tile exponents of one chunk
+  ExecState* state = nullptr;
+  int* out_exp = nullptr;
+  unsigned long long* slots = nullptr;
+  signed char* tile_e = nullptr;
   // ragged plans
   bool ragged = false;
   RaggedPlanCore rcore;
@@ -285,7 +293,7 @@ void apply_cached(mcfftPlan p) {
   if (names.size() != p->core.passes.size()) return;
   for (size_t i = 0; i < names.size(); ++i) {
     PassPlan& pp = p->core.passes[i];
-    for (const KernelInfo* k : pass_candidates(pp, p->core.cm))
+    for (const KernelInfo* k : pass_candidates(pp, p->core.cm, p->core.scaling))
       if (names[i] == k->name) {
         PassPlan trial = pp;
         if (set_pass_kernel(trial, k)) pp = trial;
@@ -300,7 +308,7 @@ void apply_forced(mcfftPlan p) {
   const char* env = getenv("MCFFT_FORCE_KERNEL");
   if (!env || !*env) return;
   for (PassPlan& pp : p->core.passes)
-    for (const KernelInfo* k : pass_candidates(pp, p->core.cm))
+    for (const KernelInfo* k : pass_candidates(pp, p->core.cm, p->core.scaling))
       if (std::string(env) == k->name) {
         PassPlan trial = pp;
         if (set_pass_kernel(trial, k)) pp = trial;
@@ -315,6 +323,10 @@ void free_plan(mcfftPlan p) {
     if (p->aops[d]) (void)hipFree(p->aops[d]);
   }
   if (p->ws) (void)hipFree(p->ws);
+  if (p->state) (void)hipFree(p->state);
+  if (p->out_exp) (void)hipFree(p->out_exp);
+  if (p->slots) (void)hipFree(p->slots);
+  if (p->tile_e) (void)hipFree(p->tile_e);
   for (int d = 0; d < 2; ++d)
     if (p->rpasses[d]) (void)hipFree(p->rpasses[d]);
   if (p->rtasks) (void)hipFree(p->rtasks);
@@ -353,6 +365,26 @@ mcfftStatus make_tables(mcfftPlan p, const std::vector<TwSpec>& tw, int cm) {
   if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;
   if (dspec) (void)hipFree(dspec);
   return rs;
+}
+
+// Exec bookkeeping buffers of a regular plan (bfp.h), all zeroed: the status
+// word, per-transform output exponents, and for AUTO multi-pass plans two
+// exponent slots per transform plus one byte per first-pass instance of a chunk
+// (at least as many as the first pass's tiles, whatever tile width it runs).
+mcfftStatus alloc_exec_state(mcfftPlan p) {
+  const PlanCore& c = p->core;
+  const size_t n_slots = bfp_slot_count(c), tile_bytes = bfp_tile_bytes(c);
+  if (hipMalloc(&p->state, sizeof(ExecState)) != hipSuccess ||
+      hipMalloc(&p->out_exp, sizeof(int) * (size_t)c.batch) != hipSuccess ||
+      (n_slots && hipMalloc(&p->slots, sizeof(unsigned long long) * n_slots) != hipSuccess) ||
+      (tile_bytes && hipMalloc(&p->tile_e, tile_bytes) != hipSuccess))
+    return MCFFT_ALLOC_FAILED;
+  if (hipMemset(p->state, 0, sizeof(ExecState)) != hipSuccess ||
+      hipMemset(p->out_exp, 0, sizeof(int) * (size_t)c.batch) != hipSuccess ||
+      (n_slots && hipMemset(p->slots, 0, sizeof(unsigned long long) * n_slots) != hipSuccess) ||
+      hipDeviceSynchronize() != hipSuccess)
+    return MCFFT_HIP_ERROR;
+  return MCFFT_SUCCESS;
 }
 
 }  // namespace
@@ -404,6 +436,7 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
     p->ws_bytes = (size_t)p->core.ws_elems * (size_t)elem_bytes(ws_fmt);
     if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
   }
+  if (rs == MCFFT_SUCCESS) rs = alloc_exec_state(p);
   if (rs != MCFFT_SUCCESS) {
     free_plan(p);
     return rs;
@@ -538,17 +571,17 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
 }
 
 mcfftStatus mcfft_plan_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt) {
-  return mcfft_plan_many(plan, 1, &n, batch, in_fmt, out_fmt, MCFFT_SCALE_BALANCED);
+  return mcfft_plan_many(plan, 1, &n, batch, in_fmt, out_fmt, MCFFT_SCALE_AUTO);
 }
 mcfftStatus mcfft_plan_2d(mcfftPlan* plan, long long nx, long long ny, long long batch, mcfftFormat in_fmt,
                           mcfftFormat out_fmt) {
   const long long d[2] = {nx, ny};
-  return mcfft_plan_many(plan, 2, d, batch, in_fmt, out_fmt, MCFFT_SCALE_BALANCED);
+  return mcfft_plan_many(plan, 2, d, batch, in_fmt, out_fmt, MCFFT_SCALE_AUTO);
 }
 mcfftStatus mcfft_plan_3d(mcfftPlan* plan, long long nx, long long ny, long long nz, long long batch,
                           mcfftFormat in_fmt, mcfftFormat out_fmt) {
   const long long d[3] = {nx, ny, nz};
-  return mcfft_plan_many(plan, 3, d, batch, in_fmt, out_fmt, MCFFT_SCALE_BALANCED);
+  return mcfft_plan_many(plan, 3, d, batch, in_fmt, out_fmt, MCFFT_SCALE_AUTO);
 }
 
 mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedItem* items, mcfftFormat in_fmt,
@@ -558,6 +591,8 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
   const int cm = compute_mode((int)in_fmt, (int)out_fmt);
   if (cm < 0) return (int)in_fmt < 0 || (int)in_fmt > 2 || (int)out_fmt < 0 || (int)out_fmt > 2 ? MCFFT_INVALID_VALUE
                                                                                              : MCFFT_UNSUPPORTED;
+  if ((int)scaling < 0 || (int)scaling > 3) return MCFFT_INVALID_VALUE;
+  if (scaling == MCFFT_SCALE_AUTO) return MCFFT_UNSUPPORTED;  // ragged items are statically scaled
   mcfftPlan p = new (std::nothrow) mcfftPlan_st();
   if (!p) return MCFFT_ALLOC_FAILED;
   p->ragged = true;
@@ -759,11 +794,13 @@ static mcfftStatus exec_ragged(mcfftPlan plan, const void* in, void* out, int di
 }
 
 static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
-                                int count, void* const* start_events = nullptr, void* const* stop_events = nullptr);
+                                int count, void* const* start_events = nullptr, void* const* stop_events = nullptr,
+                                const int* in_exp = nullptr);
 
 // R2C: complex FFT of length H into the plan's buffer, then the post-pass;
 // C2R: the pre-pass into the buffer, then the inverse complex FFT.
-static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int direction, hipStream_t s) {
+static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int direction, hipStream_t s,
+                             const int* in_exp) {
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if ((plan->real_kind == 1) != (direction == MCFFT_FORWARD)) return MCFFT_INVALID_VALUE;
   const int cm = plan->core.cm;
@@ -777,7 +814,7 @@ static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int dire
   const unsigned grid = (unsigned)(need < plan->real_grid ? need : plan->real_grid);
   void* args[] = {&ra};
   if (plan->real_kind == 1) {
-    mcfftStatus st = exec_complex(plan, in, plan->real_tmp, direction, s, 0, np);
+    mcfftStatus st = exec_complex(plan, in, plan->real_tmp, direction, s, 0, np, nullptr, nullptr, in_exp);
     if (st != MCFFT_SUCCESS) return st;
     ra.in = plan->real_tmp;
     ra.out = out;
@@ -789,13 +826,13 @@ static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int dire
   ra.out = plan->real_tmp;
   ra.in_fmt = plan->real_in_fmt;
   if (hipLaunchKernel(kC2RPre[cm], dim3(grid), dim3(1024), args, 0, s) != hipSuccess) return MCFFT_HIP_ERROR;
-  return exec_complex(plan, plan->real_tmp, out, direction, s, 0, np);
+  return exec_complex(plan, plan->real_tmp, out, direction, s, 0, np, nullptr, nullptr, in_exp);
 }
 
 // Gather a non-contiguous input into the staging buffer, run the transform,
 // scatter the result to a non-contiguous output.
 static mcfftStatus exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
-                               int direction, hipStream_t s) {
+                               int direction, hipStream_t s, const int* in_exp = nullptr) {
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
   if (!in_re || !out_re) return MCFFT_INVALID_VALUE;
   if ((in_im != nullptr) != (plan->lay_planar[0] != 0) || (out_im != nullptr) != (plan->lay_planar[1] != 0))
@@ -830,7 +867,7 @@ static mcfftStatus exec_layout(mcfftPlan plan, const void* in_re, const void* in
     src = plan->lay_buf[0];
   }
   if (!plan->lay_dense[1]) dst = plan->lay_buf[1];
-  const mcfftStatus st = exec_complex(plan, src, dst, direction, s, 0, np);
+  const mcfftStatus st = exec_complex(plan, src, dst, direction, s, 0, np, nullptr, nullptr, in_exp);
   if (st != MCFFT_SUCCESS || plan->lay_dense[1]) return st;
   LayoutArgs la{};
   la.src[0] = plan->lay_buf[1];
@@ -845,16 +882,22 @@ mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_
   return exec_layout(plan, in_re, in_im, out_re, out_im, direction, (hipStream_t)stream);
 }
 
-mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
+mcfftStatus mcfft_exec_exp(mcfftPlan plan, const void* in, const int* in_exp, void* out, int direction,
+                           void* stream) {
   if (!plan) return MCFFT_INVALID_PLAN;
-  if (plan->layout) return exec_layout(plan, in, nullptr, out, nullptr, direction, (hipStream_t)stream);
+  if (plan->layout) return exec_layout(plan, in, nullptr, out, nullptr, direction, (hipStream_t)stream, in_exp);
   if (plan->ragged) {
+    if (in_exp) return MCFFT_UNSUPPORTED;
     if (!in || !out) return MCFFT_INVALID_VALUE;
     if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
     return exec_ragged(plan, in, out, direction, (hipStream_t)stream);
   }
-  if (plan->real_kind) return exec_real(plan, in, out, direction, (hipStream_t)stream);
-  return exec_complex(plan, in, out, direction, stream, 0, (int)plan->core.passes.size());
+  if (plan->real_kind) return exec_real(plan, in, out, direction, (hipStream_t)stream, in_exp);
+  return exec_complex(plan, in, out, direction, stream, 0, (int)plan->core.passes.size(), nullptr, nullptr, in_exp);
+}
+
+mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction, void* stream) {
+  return mcfft_exec_exp(plan, in, nullptr, out, direction, stream);
 }
 
 mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
@@ -874,7 +917,7 @@ mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direc
     PassPlan best = plan->core.passes[pi];
     int best_cap = plan->grid_cap[pi];
     float best_ms = 1e30f;
-    for (const KernelInfo* k : pass_candidates(plan->core.passes[pi], plan->core.cm)) {
+    for (const KernelInfo* k : pass_candidates(plan->core.passes[pi], plan->core.cm, plan->core.scaling)) {
       PassPlan trial = plan->core.passes[pi];
       if (!set_pass_kernel(trial, k)) continue;
       int per_cu = 0;
@@ -971,14 +1014,45 @@ mcfftStatus mcfft_get_item_exponents(mcfftPlan plan, int* exponents) {
   return MCFFT_SUCCESS;
 }
 
+// read and clear the sticky status word of a regular plan (device synchronised)
+static mcfftStatus take_status(mcfftPlan plan, unsigned* flags) {
+  if (!plan->state) return MCFFT_SUCCESS;
+  ExecState st;
+  if (hipMemcpy(&st, plan->state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return MCFFT_HIP_ERROR;
+  *flags |= st.status;
+  if (st.status && hipMemset(&plan->state->status, 0, sizeof(unsigned)) != hipSuccess) return MCFFT_HIP_ERROR;
+  return MCFFT_SUCCESS;
+}
+
 mcfftStatus mcfft_get_status_flags(mcfftPlan plan, unsigned* flags) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!flags) return MCFFT_INVALID_VALUE;
   *flags = 0;
-  if (!plan->ragged) return MCFFT_SUCCESS;
-  if (hipDeviceSynchronize() != hipSuccess ||
-      (plan->rcounters && hipMemcpy(flags, plan->rcounters + 4, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess))
+  if (hipDeviceSynchronize() != hipSuccess) return MCFFT_HIP_ERROR;
+  if (!plan->ragged) return take_status(plan, flags);
+  if (plan->rcounters && hipMemcpy(flags, plan->rcounters + 4, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
     return MCFFT_HIP_ERROR;
+  for (const auto& b : plan->bulk) {
+    const mcfftStatus st = take_status(b.plan, flags);
+    if (st != MCFFT_SUCCESS) return st;
+  }
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_get_transform_exponents(mcfftPlan plan, int* exponents) {
+  if (!plan) return MCFFT_INVALID_PLAN;
+  if (!exponents) return MCFFT_INVALID_VALUE;
+  if (plan->ragged) return mcfft_get_item_exponents(plan, exponents);
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(exponents, plan->out_exp, sizeof(int) * (size_t)plan->core.batch, hipMemcpyDeviceToHost) != hipSuccess)
+    return MCFFT_HIP_ERROR;
+  return MCFFT_SUCCESS;
+}
+
+mcfftStatus mcfft_get_exponent_buffer(mcfftPlan plan, const int** exponents) {
+  if (!plan || plan->ragged) return MCFFT_INVALID_PLAN;
+  if (!exponents) return MCFFT_INVALID_VALUE;
+  *exponents = plan->out_exp;
   return MCFFT_SUCCESS;
 }
 
@@ -1008,7 +1082,7 @@ mcfftStatus mcfft_exec_timed(mcfftPlan plan, const void* in, void* out, int dire
 // launched with hipExtLaunchKernel, which records the events at the start and
 // the end of that kernel's execution.
 static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int first,
-                                int count, void* const* start_events, void* const* stop_events) {
+                                int count, void* const* start_events, void* const* stop_events, const int* in_exp) {
   if (first < 0 || count < 0 || first + count > (int)plan->core.passes.size()) return MCFFT_INVALID_VALUE;
   if (!in || !out) return MCFFT_INVALID_VALUE;
   if (direction != MCFFT_FORWARD && direction != MCFFT_INVERSE) return MCFFT_INVALID_VALUE;
@@ -1059,6 +1133,8 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
         a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);
         a.fs_cols = pp.fs_cols;
       }
+      fill_bfp(a, c, pi, t0, nt, plan->real_kind == 2, in_exp,
+               BfpBuffers{plan->state, plan->slots, plan->tile_e, plan->out_exp});
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * (long long)pp.tiles_per_group;
       if (tiles <= 0 || tiles > 0x7fffffffLL) return MCFFT_UNSUPPORTED;
@@ -1082,6 +1158,7 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
 mcfftStatus mcfft_get_output_exponent(mcfftPlan plan, int* exponent) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!exponent) return MCFFT_INVALID_VALUE;
+  if (!plan->ragged && plan->core.scaling == SCALE_AUTO) return MCFFT_UNSUPPORTED;  // per transform, found on the device
   *exponent = plan->core.out_exponent + (plan->real_kind == 2 ? 1 : 0);
   return MCFFT_SUCCESS;
 }
@@ -1134,6 +1211,7 @@ const char* mcfft_status_string(mcfftStatus s) {
     case MCFFT_MISALIGNED: return "misaligned buffer";
     case MCFFT_INVALID_PLAN: return "invalid plan";
     case MCFFT_CACHE_ERROR: return "tuning cache error";
+    case MCFFT_NONFINITE: return "non-finite result (overflow)";
   }
   return "unknown status";
 }

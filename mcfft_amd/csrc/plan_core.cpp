@@ -54,10 +54,15 @@ static bool col_view_fits(long long L, long long stride, long long Q, int cm) {
 // `ninst` > 0 (column and transposed-row kinds): the pass has that many
 // instances per group, which the tile's Q must divide, so a narrow outer axis
 // falls back to a variant with a smaller tile.
-static const KernelInfo* find_kernel(int S, int T, int cm, int kind, long long ninst = 0) {
+// a kernel built for the other scaling family (KernelInfo::mode) cannot serve the plan
+bool kernel_serves(const KernelInfo& k, int scaling) {
+  return k.mode == 0 || (k.mode == 2) == (scaling == SCALE_AUTO);
+}
+
+static const KernelInfo* find_kernel(int S, int T, int cm, int kind, int scaling, long long ninst = 0) {
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
-  auto divides = [&](const KernelInfo& k) { return ninst <= 0 || ninst % k.Q == 0; };
+  auto divides = [&](const KernelInfo& k) { return (ninst <= 0 || ninst % k.Q == 0) && kernel_serves(k, scaling); };
   for (const PreferredShape& p : kPreferred) {
     if (p.kind != kind || p.S != S || p.T != T || p.cm != cm) continue;
     for (int i = 0; i < counts[kind]; ++i) {
@@ -74,7 +79,7 @@ static const KernelInfo* find_kernel(int S, int T, int cm, int kind, long long n
 
 static int stage_shift(int radix, int scaling) {
   int lg = ilog2(radix);
-  if (scaling == SCALE_FULL) return lg;
+  if (scaling == SCALE_FULL || scaling == SCALE_AUTO) return lg;
   if (scaling == SCALE_BALANCED) return lg / 2;
   return 0;
 }
@@ -119,7 +124,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   p = PlanCore();
   if (rank < 1 || rank > 3 || batch < 1 || !dims) return ST_INVALID_VALUE;
   if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2) return ST_INVALID_VALUE;
-  if (scaling < 0 || scaling > 2) return ST_INVALID_VALUE;
+  if (scaling < 0 || scaling > 3) return ST_INVALID_VALUE;
   const int cm = compute_mode(in_fmt, out_fmt);
   if (cm < 0) return ST_UNSUPPORTED;
   p.rank = rank;
@@ -143,7 +148,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt, bool per_transform) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, KIND_ROW);
+    const KernelInfo* k = find_kernel(S, T, cm, KIND_ROW, scaling);
     if (!k) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
@@ -170,7 +175,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
                      int dst, int src_fmt) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, KIND_COL, ncols);
+    const KernelInfo* k = find_kernel(S, T, cm, KIND_COL, scaling, ncols);
     if (!k || ncols % k->Q != 0 || !col_view_fits(L, stride, k->Q, cm)) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;
@@ -197,7 +202,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   if (rank == 1) {
     const long long N = p.dims[0];
     int S0, T0;
-    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW)) {  // single pass: 2^15, 2^16 in FP16/BF16
+    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW, scaling)) {  // single pass: 2^15, 2^16 in FP16/BF16
       p.chunk = batch;
       p.n_chunks = 1;
       st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);
@@ -212,8 +217,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
       int S1, T1, S2, T2;
       if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
-      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF, N2);
-      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT, N1);
+      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF, scaling, N2);
+      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT, scaling, N1);
       if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
       long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (256LL << 20);
       long long per = N * elem_bytes(ws_fmt);
@@ -297,14 +302,14 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   return ST_OK;
 }
 
-std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm) {
+std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm, int scaling) {
   std::vector<const KernelInfo*> out;
   const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
   const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
   if (!pp.k) return out;
   for (int i = 0; i < counts[pp.kind]; ++i) {
     const KernelInfo& k = lists[pp.kind][i];
-    if (k.S == pp.k->S && k.T == pp.k->T && k.cm == cm) out.push_back(&k);
+    if (k.S == pp.k->S && k.T == pp.k->T && k.cm == cm && kernel_serves(k, scaling)) out.push_back(&k);
   }
   return out;
 }

@@ -17,7 +17,14 @@
 namespace mcfft {
 
 enum Buf { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
-enum Scaling { SCALE_NONE = 0, SCALE_BALANCED = 1, SCALE_FULL = 2 };
+// SCALE_AUTO: block floating point following the data (bfp.h); its stages
+// divide by their radix as SCALE_FULL's do, and each transform's exponent is
+// found on the device
+enum Scaling { SCALE_NONE = 0, SCALE_BALANCED = 1, SCALE_FULL = 2, SCALE_AUTO = 3 };
+// first-pass normalisation step of multi-pass AUTO plans: tiles whose input
+// maxima lie within one step of 2^4 agree on their exponent, so the pass after
+// the first has nothing to reconcile for such data
+constexpr int kBfpQuantumMultiPass = 4;
 
 // one twiddle-table entry: exp(sign * 2 pi i * e / M) * scale (sign = -1 forward)
 struct TwSpec {
@@ -52,7 +59,7 @@ struct PlanCore {
   long long n = 0;              // elements per transform
   int in_fmt = 0, out_fmt = 0, cm = 0;
   int scaling = SCALE_BALANCED;
-  int out_exponent = 0;         // true value = stored * 2^out_exponent
+  int out_exponent = 0;         // true value = stored * 2^out_exponent (AUTO: the stage shifts only)
   long long chunk = 0;          // transforms per chunk
   long long n_chunks = 0;
   long long ws_elems = 0;       // workspace elements in the compute format
@@ -76,7 +83,9 @@ enum {
 int elem_bytes(int fmt);
 // every compiled kernel that can run pass `pp` (same kind, length and compute
 // mode; the first is the planner's default); used by the tuner
-std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm);
+std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm, int scaling);
+// whether kernel k may run a pass of a plan with this scaling mode
+bool kernel_serves(const KernelInfo& k, int scaling);
 // bind kernel `k` to pass `pp`, recomputing the tile-size dependent fields;
 // false if its tile does not divide the pass
 bool set_pass_kernel(PassPlan& pp, const KernelInfo* k);

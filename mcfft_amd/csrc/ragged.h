@@ -116,7 +116,7 @@ MCFFT_DEV void ragged_tile(const PassArgs* gpa, const void* in, void* out, int t
   PassArgs a = *(const PassArgs*)uniform_ptr(gpa);
   a.in = uniform_ptr(in);
   a.out = (void*)uniform_ptr(out);
-  fft_tile<S, T, Q, NW, CM, KIND>(a, uniform(tile), A);
+  fft_tile<S, T, Q, NW, CM, KIND, false>(a, uniform(tile), A);  // statically scaled
 }
 
 template <int CM>

@@ -155,7 +155,8 @@ int build_shape(RaggedPlanCore& r, int rank, const long long* dims, long long ro
 int ragged_build(RaggedPlanCore& r, int n_items, const RaggedItemSpec* items, int in_fmt, int out_fmt, int scaling) {
   r = RaggedPlanCore();
   if (n_items < 1 || !items) return ST_INVALID_VALUE;
-  if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2 || scaling < 0 || scaling > 2) return ST_INVALID_VALUE;
+  if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2 || scaling < 0 || scaling > 3) return ST_INVALID_VALUE;
+  if (scaling == SCALE_AUTO) return ST_UNSUPPORTED;  // ragged items are statically scaled
   const int cm = compute_mode(in_fmt, out_fmt);
   if (cm < 0) return ST_UNSUPPORTED;
   r.cm = cm;

@@ -67,8 +67,61 @@ def from_storage(v: np.ndarray, fmt: int) -> np.ndarray:
     return (q[:, 0] + q[:, 2] / 2048.0) + 1j * (q[:, 1] + q[:, 3] / 2048.0)
 
 
+SCALE_NONE, SCALE_BALANCED, SCALE_FULL, SCALE_AUTO = 0, 1, 2, 3
+
+
+def transform_exponents(batch: int) -> np.ndarray:
+    """Per-transform output exponents the last emulated exec wrote (the
+    runtime's mcfft_get_transform_exponents)."""
+    L = lib()
+    L.emu_transform_exponents.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    e = np.zeros(batch, dtype=np.int32)
+    if L.emu_transform_exponents(e.ctypes.data, batch) != 0:
+        raise RuntimeError("no exponents for this batch")
+    return e
+
+
+def status() -> int:
+    """Status word of the last emulated exec (MCFFT_STATUS_* bits)."""
+    L = lib()
+    L.emu_status.restype = ctypes.c_uint
+    return int(L.emu_status())
+
+
+_input_exponents = None
+
+
+def set_input_exponents(e):
+    """Per-transform input exponents for the following execs (None: none)."""
+    global _input_exponents
+    L = lib()
+    L.emu_set_input_exponents.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    if e is None:
+        _input_exponents = None
+        L.emu_set_input_exponents(None, 0)
+        return
+    a = np.ascontiguousarray(e, dtype=np.int32)
+    _input_exponents = a.copy()
+    L.emu_set_input_exponents(a.ctypes.data, a.size)
+
+
+def _check_static(te, e, scaling):
+    """static scaling: every transform's exponent is the plan's (plus its input exponent)"""
+    if scaling != SCALE_AUTO:
+        want = e + (_input_exponents if _input_exponents is not None else 0)
+        assert (te == want).all(), (te, e)
+
+
+def _scaled(y, batch):
+    """stored values (batch rows) times 2^e of their transform"""
+    e = transform_exponents(batch)
+    return (np.asarray(y).reshape(batch, -1) * np.exp2(e.astype(np.float64))[:, None]).reshape(-1), e
+
+
 def exec_fft(x, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, ws_limit=0):
-    """Run the emulated plan; returns (complex128 output * 2^exponent, exponent)."""
+    """Run the emulated plan; returns (complex128 output in true units, the
+    per-transform output exponents).  Every plan writes one exponent per
+    transform (the same for all of them under static scaling)."""
     src = to_storage(x, in_fmt)
     n = int(np.prod(dims)) * batch
     out = np.zeros(n * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
@@ -78,7 +131,9 @@ def exec_fft(x, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, ws_limit
                         ws_limit, ctypes.byref(e))
     if rc != 0:
         raise RuntimeError(f"emu_exec status {rc}")
-    return from_storage(out, out_fmt) * (2.0 ** e.value), e.value
+    y, te = _scaled(from_storage(out, out_fmt), batch)
+    _check_static(te, e.value, scaling)
+    return y, te
 
 
 def exec_ptr(in_ptr, out_ptr, dims, batch, in_fmt, out_fmt, scaling=1, inverse=False, only_pass=None,
@@ -209,7 +264,8 @@ def real_exec(kind: str, data, n: int, batch: int, in_fmt: int, out_fmt: int, sc
                          out.ctypes.data, ctypes.byref(e))
     if rc != 0:
         raise RuntimeError(f"emu_real_exec status {rc}")
-    y = from_storage(out, out_fmt) * (2.0 ** e.value)
+    y, te = _scaled(from_storage(out, out_fmt), batch)
+    _check_static(te, e.value, scaling)
     if kind == "r2c":
         return y.reshape(batch, h + 1)
     z = y.reshape(batch, h)
@@ -307,5 +363,6 @@ def layout_exec(x, dims, batch, in_layout, out_layout, in_fmt, out_fmt, scaling=
                            1 if inverse else 0, ptr(in_re), ptr(in_im), ptr(out_re), ptr(out_im), ctypes.byref(e))
     if rc != 0:
         raise RuntimeError(f"emu_layout_exec status {rc}")
-    y = layout_decode(out_re, out_im, out_fmt, opl, opos) * (2.0 ** e.value)
+    y, te = _scaled(layout_decode(out_re, out_im, out_fmt, opl, opos), batch)
+    _check_static(te, e.value, scaling)
     return y.reshape(batch, -1), (out_re, out_im), opos

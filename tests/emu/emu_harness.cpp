@@ -16,6 +16,7 @@
 #include "ragged_plan.h"
 #include "real_pass.h"
 #include "layout_pass.h"
+#include "exec_args.h"
 
 using namespace mcfft;
 
@@ -72,6 +73,19 @@ static std::string g_kernel_override, g_last_kernels;
 extern "C" void emu_set_kernel_override(const char* name) { g_kernel_override = name ? name : ""; }
 extern "C" const char* emu_last_kernels() { return g_last_kernels.c_str(); }
 
+// Per-transform output exponents and status word of the last emu_exec, and
+// input exponents for the next ones (empty: none) -- the runtime's
+// mcfft_get_transform_exponents / mcfft_get_status_flags / mcfft_exec_exp.
+static std::vector<int> g_texps, g_in_exps;
+static unsigned g_status = 0;
+extern "C" int emu_transform_exponents(int* out, long long n) {
+  if ((size_t)n != g_texps.size()) return -1;
+  for (long long i = 0; i < n; ++i) out[i] = g_texps[(size_t)i];
+  return 0;
+}
+extern "C" unsigned emu_status() { return g_status; }
+extern "C" void emu_set_input_exponents(const int* e, long long n) { g_in_exps.assign(e, e + (e ? n : 0)); }
+
 // Restrict emu_exec to pass `pass` of the plan, launched with one workgroup per
 // tile, of which only workgroups [blk_first, blk_first + blk_count) run: one
 // part of a plan too large to run whole on the CPU.  pass < 0 lifts it.
@@ -90,7 +104,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   if (st) return st;
   if (!g_kernel_override.empty())
     for (PassPlan& pp : p.passes)
-      for (const KernelInfo* k : pass_candidates(pp, p.cm))
+      for (const KernelInfo* k : pass_candidates(pp, p.cm, p.scaling))
         if (g_kernel_override == k->name) set_pass_kernel(pp, k);
   g_last_kernels.clear();
   for (const PassPlan& pp : p.passes) g_last_kernels += std::string(pp.k->name) + " ";
@@ -107,12 +121,35 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   std::vector<unsigned char> ws((size_t)(p.ws_elems * elem_bytes(ws_fmt)) + 16);
   const size_t in_bytes = (size_t)(p.n * batch * elem_bytes(in_fmt));
   const size_t out_bytes = (size_t)(p.n * batch * elem_bytes(out_fmt));
+  // bookkeeping buffers, as the runtime allocates them (all zero)
+  ExecState state{0ull, 0u, 0u};
+  std::vector<int> out_exp((size_t)batch, 0);
+  // slots persist across emulated execs of the same plan shape, as they do in a plan
+  static std::vector<unsigned long long> slots;
+  static unsigned long long epoch = 0;
+  static std::string slot_key;
+  const std::string key = plan_signature(p);
+  if (key != slot_key || slots.size() != bfp_slot_count(p) + 1) {
+    slots.assign(bfp_slot_count(p) + 1, 0ull);
+    epoch = 0;
+    slot_key = key;
+  }
+  state.epoch = epoch;
+  std::vector<signed char> tile_e(bfp_tile_bytes(p) + 1, 0);
+  if (!g_in_exps.empty() && (long long)g_in_exps.size() != batch) return ST_INVALID_VALUE;
+  const int* in_exp = g_in_exps.empty() ? nullptr : g_in_exps.data();
+  const BfpBuffers bufs{&state, slots.data(), tile_e.data(), out_exp.data()};
   emu::clear_ranges();
   emu::register_range(in, in_bytes);
   emu::register_range(out, out_bytes);
   emu::register_range(ws.data(), (size_t)(p.ws_elems * elem_bytes(ws_fmt)));
   emu::register_range(tw.data(), p.tw.size() * sizeof(f32x2));
   emu::register_range(aops.data(), aops.size() * sizeof(u32x4));
+  emu::register_range(&state, sizeof(state));
+  emu::register_range(out_exp.data(), out_exp.size() * sizeof(int));
+  emu::register_range(slots.data(), bfp_slot_count(p) * sizeof(unsigned long long));
+  emu::register_range(tile_e.data(), bfp_tile_bytes(p));
+  if (in_exp) emu::register_range(in_exp, g_in_exps.size() * sizeof(int));
   for (long long c = 0; c < p.n_chunks; ++c) {
     const long long t0 = c * p.chunk;
     const long long nt = (t0 + p.chunk <= batch) ? p.chunk : batch - t0;
@@ -150,6 +187,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
         a.fs_tw_entries = (int)((long long)p.tw.size() - pp.fs_tw_off);
         a.fs_cols = pp.fs_cols;
       }
+      fill_bfp(a, p, (int)pi, t0, nt, false, in_exp, bufs);
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * pp.tiles_per_group;
       a.n_tiles = (int)tiles;
@@ -164,6 +202,16 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       emu::launch((int)grid, pp.k->nw * 64, (size_t)pp.k->lds_bytes, [&] { fn(a); });
     }
   }
+  g_texps = out_exp;
+  g_status = state.status;
+  // an AUTO multi-pass exec advances the epoch once per chunk, and every slot
+  // of the batch carries a tag of this exec
+  if (bfp_uses_slots(p) && g_only_pass < 0) {
+    if (state.epoch != epoch + (unsigned long long)p.n_chunks) return -100;
+    for (size_t i = 0; i < bfp_slot_count(p); ++i)
+      if ((slots[i] >> 16) < epoch || (slots[i] >> 16) >= state.epoch) return -101;
+  }
+  epoch = state.epoch;
   return 0;
 }
 
@@ -375,6 +423,7 @@ extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt,
     int st = emu_exec(1, &h, batch, tmp_fmt, out_fmt, scaling, 1, tmp.data(), out, 0, &e);
     if (st) return st;
     e += 1;
+    for (int& v : g_texps) v += 1;  // the pre-pass halves its input
   }
   if (out_exponent) *out_exponent = e;
   return 0;

@@ -119,6 +119,7 @@ inline int thread_id() { return emu::tl_tid; }
 inline int block_id() { return emu::tl_blk->bid; }
 inline int grid_size() { return emu::tl_blk->nblocks; }
 inline int uniform(int v) { return v; }
+inline long long uniform64(unsigned long long v) { return (long long)v; }
 inline const void* uniform_ptr(const void* p) { return p; }
 inline int opaque(int v) { return v; }
 inline float lane_xor(float v, int mask);
@@ -141,6 +142,7 @@ inline f32x2 cis_rev(float x) {
   return r;
 }
 inline void block_barrier() { emu::tl_blk->bar->arrive_and_wait(); }
+inline void mem_order_point() {}
 
 template <class T> inline T lds_ld(const T* p, int i) {
   const char* base = (const char*)emu::lds_base();
@@ -198,6 +200,28 @@ inline void buf_st64s(GBuf b, int voff, int soff, u32x2 v) { buf_st<u32x2>(b, ua
 inline unsigned atomic_add_dev(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomic_load_dev(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 inline void atomic_or_dev(unsigned* p, unsigned v) { (void)__atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline void atomic_max_dev64(unsigned long long* p, unsigned long long v) {
+  unsigned long long cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+}
+inline void atomic_add_dev64(unsigned long long* p, unsigned long long v) { (void)__atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+// maximum of v over the 64 lanes of the wave (all 64 lanes take part): xor
+// butterflies inside each 32-lane half, then lanes 0 and 32 combined
+inline unsigned wave_max(unsigned v) {
+  for (int o = 1; o <= 16; o <<= 1) {
+    const unsigned w = __builtin_bit_cast(unsigned, lane_xor(__builtin_bit_cast(float, v), o));
+    v = w > v ? w : v;
+  }
+  emu::Block* b = emu::tl_blk;
+  const int t = emu::tl_tid, w = t / 64, l = t % 64;
+  b->xs[(size_t)w * 64 + l] = __builtin_bit_cast(float, v);
+  b->wbar[w]->arrive_and_wait();
+  const unsigned lo = __builtin_bit_cast(unsigned, b->xs[(size_t)w * 64]);
+  const unsigned hi = __builtin_bit_cast(unsigned, b->xs[(size_t)w * 64 + 32]);
+  b->wbar[w]->arrive_and_wait();
+  return lo > hi ? lo : hi;
+}
 inline void release_dev() { __atomic_thread_fence(__ATOMIC_RELEASE); }
 inline void acquire_dev() { __atomic_thread_fence(__ATOMIC_ACQUIRE); }
 inline void nap() { std::this_thread::yield(); }

@@ -380,3 +380,174 @@ def test_col_pass_byte_offsets_beyond_2gib():
         del buf, y
     finally:
         mm.close()
+
+
+# ---------------- block floating point (MCFFT_SCALE_AUTO) ----------------
+AUTO = emu.SCALE_AUTO
+
+
+def sinusoid_input(dims, batch, seed, amplitude=100.0):
+    """re/im iid U[-1,1) plus amplitude * exp(2 pi i (kx x / nx + ky y / ny)) at a
+    random (kx, ky) per transform: the 2D benchmark input's shape (its spectrum
+    peaks at amplitude * nx * ny)."""
+    rng = np.random.default_rng(seed)
+    nx, ny = dims
+    xs, ys = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    out = []
+    for _ in range(batch):
+        kx, ky = rng.integers(0, nx), rng.integers(0, ny)
+        u = rng.uniform(-1, 1, (nx, ny)) + 1j * rng.uniform(-1, 1, (nx, ny))
+        out.append(u + amplitude * np.exp(2j * np.pi * (kx * xs / nx + ky * ys / ny)))
+    return np.concatenate([o.reshape(-1) for o in out])
+
+
+def small_uniform_input(n, seed):
+    """re/im iid U[0, 1e-4): the magnitude of the reference's generate_data
+    (source/mcfft_1d_utils.cpp:88-100), partly FP16 subnormals."""
+    rng = np.random.default_rng(seed)
+    return 1e-4 * (rng.uniform(0, 1, n) + 1j * rng.uniform(0, 1, n))
+
+
+def run_auto(x, dims, batch, fi, fo, scaling=AUTO):
+    xq = emu.from_storage(emu.to_storage(x, fi), fi)
+    y, e = emu.exec_fft(x, list(dims), batch, fi, fo, scaling=scaling)
+    return y, e, oracle.fft(xq, tuple(dims))
+
+
+@pytest.mark.parametrize("dims,batch", [([256], 8), ([2048], 3), ([131072], 1), ([64, 128], 2), ([32, 32, 32], 1),
+                                        ([16, 65536], 1)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_auto_scaling_parity(dims, batch, fi, fo):
+    if fo == emu.FMT_SPLIT and dims[-1] > 16384:
+        pytest.skip("split innermost sides stop at 2^14")
+    check(dims, batch, fi, fo, scaling=AUTO)
+
+
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_auto_scaling_amplitude_100_sinusoid(fi, fo):
+    """512 x 512: the peak 100 * 2^18 is far above FP16's 65504.  AUTO keeps the
+    FP16/split result within tolerance with no status bit; BALANCED's static
+    shifts (2^-8 over both axes) overflow FP16 and say so in the status word."""
+    dims, batch = (512, 512), 2
+    x = sinusoid_input(dims, batch, seed=3)
+    y, e, ref = run_auto(x, dims, batch, fi, fo)
+    err = oracle.rel_l2(y, ref)
+    assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], err
+    assert emu.status() == 0
+    yb, _, _ = run_auto(x, dims, batch, fi, fo, scaling=emu.SCALE_BALANCED)
+    if fi == emu.FMT_BF16:
+        assert emu.status() == 0 and np.isfinite(yb).all()   # BF16's range holds it
+    else:
+        assert emu.status() == 2 and not np.isfinite(yb).all()
+
+
+@pytest.mark.parametrize("dims,batch", [([4096], 2), ([256, 256], 1)])
+def test_auto_scaling_small_inputs(dims, batch):
+    """Inputs of generate_data's magnitude (~1e-4): AUTO scales them up and stays
+    within the FP16 tolerance; FULL (divide by N) drops most of the spectrum
+    below FP16's subnormal range."""
+    n = int(np.prod(dims))
+    x = small_uniform_input(n * batch, seed=42)
+    y, e, ref = run_auto(x, dims, batch, emu.FMT_F16, emu.FMT_F16)
+    assert oracle.rel_l2(y, ref) <= TOL[emu.FMT_F16]
+    assert (e < 0).all()   # scaled up: stored values are larger than the true ones
+    yf, _, _ = run_auto(x, dims, batch, emu.FMT_F16, emu.FMT_F16, scaling=emu.SCALE_FULL)
+    assert oracle.rel_l2(yf, ref) > 0.05
+
+
+@pytest.mark.parametrize("case", ["2d", "3d", "fourstep"])
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_auto_scaling_reconcile(case, fi, fo):
+    """First-pass tiles of one transform with input maxima 2^4..2^12 apart pick
+    different exponents; the pass after the first brings them to a common one
+    element by element.  Parity against the oracle."""
+    rng = np.random.default_rng(17)
+    if case == "2d":
+        dims, batch = (64, 256), 2
+        scale = np.exp2(4.0 * (np.arange(64) % 4))[:, None] * np.ones((1, 256))        # per row
+    elif case == "3d":
+        dims, batch = (16, 32, 64), 1
+        scale = np.exp2(4.0 * (np.arange(32) % 3))[None, :, None] * np.ones((16, 32, 64))  # per (x, y) row
+    else:
+        dims, batch = (131072,), 1
+        scale = np.exp2(4.0 * ((np.arange(131072) % 512) // 32 % 3))              # per four-step column group
+    n = int(np.prod(dims))
+    x = (rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)) * np.tile(scale.reshape(-1), batch)
+    y, e, ref = run_auto(x, dims, batch, fi, fo)
+    assert oracle.rel_l2(y, ref) <= TOL[fo if fo == emu.FMT_SPLIT else fi]
+
+
+def test_auto_scaling_zero_transforms():
+    """An all-zero transform comes out as zeros; one whose rows are half zero,
+    and a normal one next to it, keep their tolerance."""
+    dims, batch = (64, 256), 3
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((batch, 64, 256)) + 1j * rng.standard_normal((batch, 64, 256))
+    x[0] = 0
+    x[1, ::2] = 0
+    y, e, ref = run_auto(x.reshape(-1), dims, batch, emu.FMT_F16, emu.FMT_F16)
+    y = y.reshape(batch, -1)
+    ref = ref.reshape(batch, -1)
+    assert (y[0] == 0).all()
+    for b in (1, 2):
+        assert oracle.rel_l2(y[b], ref[b]) <= TOL[emu.FMT_F16]
+
+
+@pytest.mark.parametrize("dims", [[256], [64, 128]])
+def test_auto_scaling_nonfinite_input(dims):
+    """An Inf in one transform's input sets the status bit; the other
+    transforms are unaffected."""
+    batch = 4
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
+    x[2 * n + 5] = np.inf
+    y, e = emu.exec_fft(x, dims, batch, emu.FMT_F16, emu.FMT_F16, scaling=AUTO)
+    assert emu.status() == 2
+    xq = emu.from_storage(emu.to_storage(x, emu.FMT_F16), emu.FMT_F16)
+    y = y.reshape(batch, n)
+    for b in (0, 1, 3):
+        assert oracle.rel_l2(y[b], oracle.fft(xq[b * n:(b + 1) * n], tuple(dims))) <= TOL[emu.FMT_F16]
+    x[2 * n + 5] = 0
+    emu.exec_fft(x, dims, batch, emu.FMT_F16, emu.FMT_F16, scaling=AUTO)
+    assert emu.status() == 0
+
+
+@pytest.mark.parametrize("scaling", [AUTO, emu.SCALE_BALANCED])
+@pytest.mark.parametrize("dims", [[1024], [131072], [32, 64]])
+def test_input_exponents_add_to_output_exponents(scaling, dims):
+    """Input exponents (true input = stored * 2^in[t]) add to the output
+    exponents transform by transform, so the decoded results scale by 2^in[t]."""
+    batch = 4 if dims != [131072] else 2
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
+    y0, e0 = emu.exec_fft(x, dims, batch, emu.FMT_F16, emu.FMT_F16, scaling=scaling)
+    ins = np.array([3, -2, 0, 7][:batch], dtype=np.int32)
+    emu.set_input_exponents(ins)
+    try:
+        y1, e1 = emu.exec_fft(x, dims, batch, emu.FMT_F16, emu.FMT_F16, scaling=scaling)
+    finally:
+        emu.set_input_exponents(None)
+    assert (e1 == e0 + ins).all()
+    assert np.array_equal(y1.reshape(batch, -1), y0.reshape(batch, -1) * np.exp2(ins.astype(float))[:, None])
+
+
+@pytest.mark.parametrize("kernel", ["mcfft_pass_bigauto16_0", "mcfft_pass_bigauto_0"])
+def test_auto_scaling_single_pass_65536(kernel):
+    """The AUTO form of the single-pass 65536 kernel (normalises and writes the
+    exponent of each transform itself), both wave counts, with transforms of
+    very different magnitude side by side."""
+    emu.set_kernel_override(kernel)
+    try:
+        n, batch = 65536, 3
+        rng = np.random.default_rng(12)
+        x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))) * np.array([1e-4, 1.0, 300.0])[:, None]
+        y, e, ref = run_auto(x.reshape(-1), [n], batch, emu.FMT_F16, emu.FMT_F16)
+        assert kernel in emu.last_kernels()
+        y, ref = y.reshape(batch, n), ref.reshape(batch, n)
+        for b in range(batch):
+            assert oracle.rel_l2(y[b], ref[b]) <= TOL[emu.FMT_F16]
+        assert e[0] < e[1] < e[2]
+    finally:
+        emu.set_kernel_override(None)

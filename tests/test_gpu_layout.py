@@ -61,7 +61,9 @@ def run_layout(torch, x, dims, batch, li, lo, fi, fo, direction):
     p.exec_layout(d[0], d[1], d[2], d[3], direction)
     torch.cuda.synchronize()
     ore, oim = _from_dev(d[2]), _from_dev(d[3])
-    y = emu.layout_decode(ore, oim, FMT_ID[fo], lo[2], opos) * 2.0 ** p.exponent
+    e = p.exponents()   # per transform (AUTO by default)
+    y = (emu.layout_decode(ore, oim, FMT_ID[fo], lo[2], opos).reshape(batch, -1) *
+         np.exp2(e.astype(np.float64))[:, None]).reshape(-1)
     p.destroy()
     return y, (ore, oim), opos
 
@@ -82,7 +84,7 @@ def test_layout_exec(torch_cuda, case, direction):
     d_out = mc.empty(n * batch, fo)
     p.exec(d_in, d_out, direction)
     torch.cuda.synchronize()
-    y0 = mc.from_device(d_out, fo, p.exponent)
+    y0 = p.result(d_out)
     p.destroy()
     assert np.array_equal(y, y0), case
     xq = mc.decode(mc.encode(x, fi), fi)

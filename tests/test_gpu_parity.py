@@ -28,7 +28,7 @@ def _quantised(x, in_fmt):
     return mc.decode(mc.encode(x, in_fmt), in_fmt)
 
 
-def run(dims, batch, in_fmt, out_fmt, x, direction=mc.FORWARD, scaling="balanced", inplace=False):
+def run(dims, batch, in_fmt, out_fmt, x, direction=mc.FORWARD, scaling="auto", inplace=False):
     import torch
     p = mc.Plan(dims, batch, in_fmt, out_fmt, scaling)
     din = mc.to_device(x, in_fmt)
@@ -39,7 +39,7 @@ def run(dims, batch, in_fmt, out_fmt, x, direction=mc.FORWARD, scaling="balanced
         dout = mc.empty(x.size, out_fmt)
     p.exec(din, dout, direction)
     torch.cuda.synchronize()
-    y = mc.from_device(dout, out_fmt, p.exponent)
+    y = p.result(dout)
     p.destroy()
     return y
 
@@ -111,23 +111,26 @@ def test_plain_no_worse_than_reference_fp16(torch_cuda, n):
     assert ours <= theirs
 
 
-def test_split_roundtrip_c2_full_size(torch_cuda):
-    """1D N=131072 x 256 forward then inverse (split): inv(fwd(x)) = N x."""
+@pytest.mark.parametrize("scaling", ["auto", "balanced"])
+def test_split_roundtrip_c2_full_size(torch_cuda, scaling):
+    """1D N=131072 x 256 forward then inverse (split): inv(fwd(x)) = N x.  The
+    inverse takes the forward's per-transform exponents as its input exponents,
+    so its own exponents give the true result."""
     import torch
     n, b = 131072, 256
     x = _input(n * b, seed=2)
     xq = _quantised(x, "fp16")
-    pf = mc.Plan([n], b, "fp16", "split")
-    pi = mc.Plan([n], b, "split", "split")
+    pf = mc.Plan([n], b, "fp16", "split", scaling)
+    pi = mc.Plan([n], b, "split", "split", scaling)
     d0 = mc.to_device(x, "fp16")
     d1 = mc.empty(n * b, "split")
     d2 = mc.empty(n * b, "split")
     pf.exec(d0, d1, mc.FORWARD)
-    pi.exec(d1, d2, mc.INVERSE)
+    pi.exec(d1, d2, mc.INVERSE, in_exp=pf)
     torch.cuda.synchronize()
-    y = mc.from_device(d2, "split", pf.exponent + pi.exponent) / n
+    y = pi.result(d2) / n
     err = oracle.rel_l2(y, xq)
-    print(f"roundtrip rel_l2={err:.3e}")
+    print(f"roundtrip {scaling} rel_l2={err:.3e}")
     assert err <= 2e-6
 
 
@@ -147,18 +150,22 @@ def test_gpu_error_norms_match_host(torch_cuda, fmt):
     assert abs(m - oracle.max_abs(y, ref)) <= 1e-12 * max(1.0, m)
 
 
-@pytest.mark.parametrize("kernel,fmt", [("mcfft_pass_big_0", "fp16"), ("mcfft_pass_big16_0", "fp16"),
-                                        ("mcfft_pass_big16_1", "bf16")])
-def test_single_pass_65536_variants(torch_cuda, kernel, fmt, monkeypatch):
-    """Both wave counts of the single-pass 65536 kernel (pinned with
-    MCFFT_FORCE_KERNEL), against the oracle."""
+@pytest.mark.parametrize("kernel,fmt,scaling", [("mcfft_pass_big_0", "fp16", "balanced"),
+                                                ("mcfft_pass_big16_0", "fp16", "balanced"),
+                                                ("mcfft_pass_big16_1", "bf16", "balanced"),
+                                                ("mcfft_pass_bigauto_0", "fp16", "auto"),
+                                                ("mcfft_pass_bigauto16_0", "fp16", "auto"),
+                                                ("mcfft_pass_bigauto_1", "bf16", "auto")])
+def test_single_pass_65536_variants(torch_cuda, kernel, fmt, scaling, monkeypatch):
+    """Both wave counts of the single-pass 65536 kernel, statically scaled and
+    AUTO forms (pinned with MCFFT_FORCE_KERNEL), against the oracle."""
     monkeypatch.setenv("MCFFT_FORCE_KERNEL", kernel)
     n, batch = 65536, 3
     x = _input(n * batch, seed=11)
-    p = mc.Plan([n], batch, fmt, fmt)
+    p = mc.Plan([n], batch, fmt, fmt, scaling)
     assert kernel in p.describe()
     p.destroy()
-    y = run([n], batch, fmt, fmt, x)
+    y = run([n], batch, fmt, fmt, x, scaling=scaling)
     err = oracle.rel_l2(y, oracle.fft(_quantised(x, fmt), (n,)))
     print(f"65536 {kernel} rel_l2={err:.3e}")
     assert err <= TOL[fmt]

@@ -26,7 +26,7 @@ def test_r2c(torch_cuda, n, batch, fi, fo):
     d_out = mc.empty((n // 2 + 1) * batch, fo)
     p.exec(d_in, d_out, mc.FORWARD)
     torch.cuda.synchronize()
-    y = mc.from_device(d_out, fo, p.exponent)
+    y = p.result(d_out)
     p.destroy()
     ref = oracle.fft(xq.reshape(-1).astype(np.complex128), (n,)).reshape(batch, n)[:, : n // 2 + 1]
     err = oracle.rel_l2(y, ref.reshape(-1))
@@ -51,7 +51,7 @@ def test_c2r(torch_cuda, n, batch, fi, fo):
     d_out = mc.empty(h * batch, fo)
     p.exec(d_in, d_out, mc.INVERSE)
     torch.cuda.synchronize()
-    y = mc.pairs_to_real(mc.from_device(d_out, fo, p.exponent), n)
+    y = mc.pairs_to_real(p.result(d_out), n)
     p.destroy()
     err = oracle.rel_l2(y.reshape(-1).astype(np.complex128), ref.reshape(-1).astype(np.complex128))
     print(f"c2r n={n} b={batch} {fi}->{fo} rel_l2={err:.3e}")
@@ -71,9 +71,9 @@ def test_r2c_then_c2r_round_trip(torch_cuda):
     d_mid = mc.empty((n // 2 + 1) * batch, "split")
     d_out = mc.empty(n // 2 * batch, "split")
     pf.exec(d_in, d_mid, mc.FORWARD)
-    pi.exec(d_mid, d_out, mc.INVERSE)
+    pi.exec(d_mid, d_out, mc.INVERSE, in_exp=pf)   # the forward's exponents carried through
     torch.cuda.synchronize()
-    y = mc.pairs_to_real(mc.from_device(d_out, "split", pf.exponent + pi.exponent), n) / n
+    y = mc.pairs_to_real(pi.result(d_out), n) / n
     pf.destroy()
     pi.destroy()
     assert oracle.rel_l2(y.reshape(-1).astype(np.complex128), xq.reshape(-1).astype(np.complex128)) <= 1e-6
@@ -84,11 +84,11 @@ def test_real_plan_exponent_queries_agree(torch_cuda):
     exponent for real plans (C2R carries one more power of two)."""
     import ctypes
     for kind in ("r2c", "c2r"):
-        p = mc.RealPlan(4096, 4, kind, "fp16", "fp16")
+        p = mc.RealPlan(4096, 4, kind, "fp16", "fp16", "balanced")
         e = ctypes.c_int()
         assert mc.lib().mcfft_get_item_exponents(p._h, ctypes.byref(e)) == 0
         assert e.value == p.exponent
-        q = mc.Plan([2048], 4, "fp16", "fp16")
+        q = mc.Plan([2048], 4, "fp16", "fp16", "balanced")
         assert p.exponent == q.exponent + (1 if kind == "c2r" else 0)
         p.destroy()
         q.destroy()

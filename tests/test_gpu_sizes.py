@@ -32,7 +32,7 @@ def _run(torch, dims, batch, fmt, scaling, seed):
     p.exec(d_in, d_out, mc.FORWARD)
     torch.cuda.synchronize()
     e = p.exponent
-    y = mc.from_device(d_out, out_fmt, e)
+    y = p.result(d_out)
     p.destroy()
     ref = oracle.fft(mc.decode(mc.encode(x, in_fmt), in_fmt), tuple(dims))
     return oracle.rel_l2(y, ref), e
@@ -40,12 +40,13 @@ def _run(torch, dims, batch, fmt, scaling, seed):
 
 @pytest.mark.parametrize("lg", list(range(4, 24)))
 @pytest.mark.parametrize("fmt", ["fp16", "bf16", "split"])
-def test_every_1d_length(torch_cuda, lg, fmt):
+@pytest.mark.parametrize("scaling", ["auto", "balanced"])
+def test_every_1d_length(torch_cuda, lg, fmt, scaling):
     """Each power of two the 1D planner accepts; at least 2^18 elements per case."""
     n = 1 << lg
     batch = max(1, (1 << 18) // n)
-    print(f"1d n=2^{lg} b={batch} {fmt}", flush=True)
-    err, _ = _run(torch_cuda, [n], batch, fmt, "balanced", seed=lg)
+    print(f"1d n=2^{lg} b={batch} {fmt} {scaling}", flush=True)
+    err, _ = _run(torch_cuda, [n], batch, fmt, scaling, seed=lg)
     print(f"  rel_l2 {err:.3e}", flush=True)
     assert err <= TOL[fmt]
 
