@@ -116,11 +116,23 @@ def ref_metric(tested, standard) -> float:
     return lib().orc_ref_metric(_dptr(t.view(np.float64)), _dptr(s.view(np.float64)), t.size)
 
 
-def ref_fp16_exec(x, n: int, rounding: bool = True) -> np.ndarray:
-    """Emulate the reference kernels at MCTYPE=half on natural-order input."""
+# rounding points of ref_fp16_exec (flags; see oracle.c)
+R_INPUT, R_TABLE, R_MMA, R_VALU, R_FUSED = 1, 2, 4, 8, 16
+R_ALL = R_INPUT | R_TABLE | R_MMA | R_VALU
+
+
+def ref_fp16_exec(x, n: int, rounding: bool = True, points: int | None = None) -> np.ndarray:
+    """Emulate the reference kernels at MCTYPE=half on natural-order input.
+
+    rounding=False keeps everything in FP64; ``points`` (R_* flags) selects the
+    rounding points instead: R_ALL is the default, R_ALL | R_FUSED the variant
+    whose complex_mul keeps FP32 across its four mma_sync."""
     a = _as_c128(x)
     out = np.empty_like(a)
-    rc = lib().orc_ref_fp16_exec(_dptr(a.view(np.float64)), _dptr(out.view(np.float64)), n, a.size // n, int(rounding))
+    mode = (points << 1) if points is not None else int(bool(rounding))
+    if points is not None and points == 0:
+        mode = 0
+    rc = lib().orc_ref_fp16_exec(_dptr(a.view(np.float64)), _dptr(out.view(np.float64)), n, a.size // n, mode)
     if rc != 0:
         raise ValueError(f"the reference has no plan for n={n}")
     return out

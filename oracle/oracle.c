@@ -24,10 +24,16 @@
  *
  * Parity pins: see tests/test_oracle.py (golden vectors from the reference's
  * captured outputs 3.txt / 2.txt / 1.log, brute-force DFT, closed forms,
- * numpy.fft).  The FP16 emulator's rocWMMA half-accumulate semantics
- * (FP32 MFMA then round-to-half per mma_sync) are an assumption: parity
- * unpinned for orc_ref_fp16_exec beyond its FP64 limit (it must converge to
- * orc_fft_many when rounding is disabled, which the tests check).
+ * numpy.fft).  orc_ref_fp16_exec is pinned by its algebra (no rounding: equal
+ * to orc_fft_many), by FP16-exact inputs (a delta gives exactly all ones at
+ * every rounding point), and by a first-order error model: each class of
+ * rounding points, switched on alone, must produce the rel-L2 that its
+ * rounded tables (computed independently) or the FP16 rounding variance of
+ * its values predicts, and the classes must add in quadrature.  What rocWMMA
+ * does with a half accumulator cannot be run here (the reference's FP16
+ * build does not compile), so it is bracketed: per mma_sync rounding (the
+ * default) and FP32 across a complex_mul's four mma_sync (ORC_R_FUSED); the
+ * build's plain FP16 is held to the tighter of the two.
  */
 #include <math.h>
 #include <stdint.h>
@@ -317,7 +323,25 @@ double orc_max_abs(const double *a, const double *ref, long n_total)
 /* Emulation of the reference kernels at MCTYPE=half.                  */
 /* ================================================================== */
 
-static int g_round = 1; /* 0 = keep FP64 everywhere (used to check the emulator's algebra) */
+/* Rounding points of the FP16 emulation (bit flags; 0 = FP64 everywhere, used
+ * to check the emulator's algebra):
+ *   ORC_R_INPUT  input converted to half (setup, source/mcfft_1d_utils.cpp:50-51)
+ *   ORC_R_TABLE  DFT matrix and W_N_K twiddles: cosf/sinf of a float argument,
+ *                stored as half (source/mcfft_1d_device.cpp:32-36, 777-785)
+ *   ORC_R_MMA    the half accumulator of each mma_sync rounded back to half
+ *   ORC_R_VALU   half arithmetic of cmul/cadd/csub, the twiddle recurrence and
+ *                the radix-2/4 tails (source/mcfft_1d_device.cpp:39-55, 81-82, ...)
+ *   ORC_R_FUSED  with ORC_R_MMA: the four mma_sync of one complex_mul
+ *                accumulate in FP32 and round once per output (the tighter of
+ *                the two accumulator semantics; without it every mma_sync rounds)
+ * The default (round = 1 from the API) is INPUT|TABLE|MMA|VALU. */
+#define ORC_R_INPUT 1
+#define ORC_R_TABLE 2
+#define ORC_R_MMA 4
+#define ORC_R_VALU 8
+#define ORC_R_FUSED 16
+static int g_rflags = ORC_R_INPUT | ORC_R_TABLE | ORC_R_MMA | ORC_R_VALU;
+static int g_round = 1; /* rounding of the current rounding point (rh) */
 
 /* round a double to the nearest IEEE binary16 value (RNE, subnormals, overflow to inf) */
 static double rh(double x)
@@ -338,9 +362,21 @@ static double rh(double x)
 
 typedef struct { double x, y; } hc; /* value held in a half2 register */
 
-static hc hmul(hc a, hc b) { hc t = { rh(rh(a.x * b.x) - rh(a.y * b.y)), rh(rh(a.x * b.y) + rh(a.y * b.x)) }; return t; }
-static hc hadd(hc a, hc b) { hc t = { rh(a.x + b.x), rh(a.y + b.y) }; return t; }
-static hc hsub(hc a, hc b) { hc t = { rh(a.x - b.x), rh(a.y - b.y) }; return t; }
+/* rounding at one class of rounding points */
+static double rp(int flag, double x)
+{
+    if (!(g_rflags & flag)) return x;
+    const int saved = g_round;
+    g_round = 1;
+    const double r = rh(x);
+    g_round = saved;
+    return r;
+}
+#define rv(x) rp(ORC_R_VALU, (x))
+
+static hc hmul(hc a, hc b) { hc t = { rv(rv(a.x * b.x) - rv(a.y * b.y)), rv(rv(a.x * b.y) + rv(a.y * b.x)) }; return t; }
+static hc hadd(hc a, hc b) { hc t = { rv(a.x + b.x), rv(a.y + b.y) }; return t; }
+static hc hsub(hc a, hc b) { hc t = { rv(a.x - b.x), rv(a.y - b.y) }; return t; }
 
 /* source/mcfft_1d_device.cpp:32-36: {cosf(2*M_PI*K*M/N), -sinf(...)}; the double
  * argument is converted to float for cosf/sinf, the float result stored to half. */
@@ -348,13 +384,13 @@ static hc W_N_K(long N, long K, long M)
 {
     double arg = 2 * M_PI * (double)K * (double)M / (double)N;
     hc t;
-    if (!g_round) {
+    if (!(g_rflags & ORC_R_TABLE)) {
         twiddle_ld((long)(((unsigned long)K * (unsigned long)M) % (unsigned long)N), N, -1, &t.x, &t.y);
         return t;
     }
     float af = (float)arg;
-    t.x = rh((double)cosf(af));
-    t.y = rh(-(double)sinf(af));
+    t.x = rp(ORC_R_TABLE, (double)cosf(af));
+    t.y = rp(ORC_R_TABLE, -(double)sinf(af));
     return t;
 }
 
@@ -365,10 +401,10 @@ static void init_F(void)
 {
     for (int i = 0; i < 16; ++i)
         for (int j = 0; j < 16; ++j) {
-            if (g_round) {
+            if (g_rflags & ORC_R_TABLE) {
                 float a = (float)(2 * M_PI * i * j / 16);
-                g_Fr[i][j] = rh((double)cosf(a));
-                g_Fi[i][j] = rh(-(double)sinf(a));
+                g_Fr[i][j] = rp(ORC_R_TABLE, (double)cosf(a));
+                g_Fi[i][j] = rp(ORC_R_TABLE, -(double)sinf(a));
             } else {
                 twiddle_ld((long)(i * j % 16), 16, -1, &g_Fr[i][j], &g_Fi[i][j]);
             }
@@ -377,18 +413,20 @@ static void init_F(void)
 
 
 /* One rocWMMA mma_sync with a half accumulator (source/mcfft_1d_device.cpp:23-29):
- * assumed to be FP32 MFMA products summed onto the accumulator in FP32, then the
- * accumulator rounded back to half.  A is 16x16, b is one 16-entry column. */
-static double mma_el(const double A[16][16], int row, const double *b, double acc)
+ * FP32 MFMA products summed onto the accumulator in FP32 (products of two halves
+ * are exact in FP32), then -- unless the complex_mul keeps FP32 across its four
+ * mma_sync (ORC_R_FUSED) -- the accumulator rounded back to half.  A is 16x16,
+ * b is one 16-entry column. */
+static double mma_el(const double A[16][16], int row, const double *b, double acc, int round_out)
 {
-    if (!g_round) {
+    if (!(g_rflags & ORC_R_MMA)) {
         double s = acc;
         for (int k = 0; k < 16; ++k) s += A[row][k] * b[k];
         return s;
     }
     float s = (float)acc;
     for (int k = 0; k < 16; ++k) s += (float)(A[row][k] * b[k]);
-    return rh((double)s);
+    return round_out ? rp(ORC_R_MMA, (double)s) : (double)s;
 }
 
 /* complex_mul (source/mcfft_1d_device.cpp:10-30) for one 16-point column:
@@ -396,13 +434,14 @@ static double mma_el(const double A[16][16], int row, const double *b, double ac
 static void cmma(const hc *in, hc *out)
 {
     double br[16], bi[16];
+    const int each = !(g_rflags & ORC_R_FUSED);  /* round after every mma_sync */
     for (int k = 0; k < 16; ++k) { br[k] = in[k].x; bi[k] = in[k].y; }
     for (int i = 0; i < 16; ++i) {
-        double r = mma_el(g_Fi, i, bi, 0.0);
+        double r = mma_el(g_Fi, i, bi, 0.0, each);
         r = -r;
-        r = mma_el(g_Fr, i, br, r);
-        double m = mma_el(g_Fr, i, bi, 0.0);
-        m = mma_el(g_Fi, i, br, m);
+        r = mma_el(g_Fr, i, br, r, 1);
+        double m = mma_el(g_Fr, i, bi, 0.0, each);
+        m = mma_el(g_Fi, i, br, m, 1);
         out[i].x = r;
         out[i].y = m;
     }
@@ -414,7 +453,7 @@ static void cmma(const hc *in, hc *out)
 static hc tw_chain(long N, long col, int row)
 {
     hc t = W_N_K(N, col, 4 * (row / 4));
-    if (!g_round) return W_N_K(N, col, row);
+    if (!(g_rflags & (ORC_R_TABLE | ORC_R_VALU))) return W_N_K(N, col, row);
     hc f = W_N_K(N, col, 1);
     for (int j = 0; j < row % 4; ++j) t = hmul(t, f);
     return t;
@@ -461,18 +500,19 @@ static void l1024_chunk(hc *c)
     hc s[1024];
     for (int q = 0; q < 4; ++q) l256_chunk(c + 256 * q, s + 256 * q);
     for (int u = 0; u < 256; ++u) {
+        const int chain = g_rflags & (ORC_R_TABLE | ORC_R_VALU);
         hc t1 = W_N_K(1024, u, 1);
-        hc t2 = g_round ? hmul(t1, t1) : W_N_K(1024, u, 2);
-        hc t3 = g_round ? hmul(t2, t1) : W_N_K(1024, u, 3);
+        hc t2 = chain ? hmul(t1, t1) : W_N_K(1024, u, 2);
+        hc t3 = chain ? hmul(t2, t1) : W_N_K(1024, u, 3);
         hc e0 = s[u], e1 = hmul(s[u + 256], t1), e2 = hmul(s[u + 512], t2), e3 = hmul(s[u + 768], t3);
-        c[u].x = rh(rh(rh(e0.x + e1.x) + e2.x) + e3.x);
-        c[u].y = rh(rh(rh(e0.y + e1.y) + e2.y) + e3.y);
-        c[u + 256].x = rh(rh(rh(e0.x + e1.y) - e2.x) - e3.y);
-        c[u + 256].y = rh(rh(rh(e0.y - e1.x) - e2.y) + e3.x);
-        c[u + 512].x = rh(rh(rh(e0.x - e1.x) + e2.x) - e3.x);
-        c[u + 512].y = rh(rh(rh(e0.y - e1.y) + e2.y) - e3.y);
-        c[u + 768].x = rh(rh(rh(e0.x - e1.y) - e2.x) + e3.y);
-        c[u + 768].y = rh(rh(rh(e0.y + e1.x) - e2.y) - e3.x);
+        c[u].x = rv(rv(rv(e0.x + e1.x) + e2.x) + e3.x);
+        c[u].y = rv(rv(rv(e0.y + e1.y) + e2.y) + e3.y);
+        c[u + 256].x = rv(rv(rv(e0.x + e1.y) - e2.x) - e3.y);
+        c[u + 256].y = rv(rv(rv(e0.y - e1.x) - e2.y) + e3.x);
+        c[u + 512].x = rv(rv(rv(e0.x - e1.x) + e2.x) - e3.x);
+        c[u + 512].y = rv(rv(rv(e0.y - e1.y) + e2.y) - e3.y);
+        c[u + 768].x = rv(rv(rv(e0.x - e1.y) - e2.x) + e3.y);
+        c[u + 768].y = rv(rv(rv(e0.y + e1.x) - e2.y) - e3.x);
     }
 }
 
@@ -506,11 +546,14 @@ static void l256_1_group(hc *d, long step)
  *   1024: {16,16,4}, layer_1024_0
  *   131072: {16,16,2,16,16}, layer_512_0 then layer_256_1(step=512)
  * x: natural-order FP64 input (n*batch complex), y: natural-order output.
- * round=0 keeps every value in FP64 (algebra check).  Returns 0, or -1 if n
- * has no reference plan.
+ * round: 0 keeps every value in FP64 (algebra check), 1 rounds at every
+ * rounding point (ORC_R_INPUT|TABLE|MMA|VALU), any other value is a set of
+ * ORC_R_* flags << 1 (bit 1 = ORC_R_INPUT, ...).  Returns 0, or -1 if n has no
+ * reference plan.
  */
 int orc_ref_fp16_exec(const double *x, double *y, long n, long batch, int round)
 {
+    g_rflags = round == 1 ? (ORC_R_INPUT | ORC_R_TABLE | ORC_R_MMA | ORC_R_VALU) : (round >> 1);
     int rad[5], nr;
     switch (n) {
     case 256: rad[0] = 16; rad[1] = 16; nr = 2; break;
@@ -519,14 +562,17 @@ int orc_ref_fp16_exec(const double *x, double *y, long n, long batch, int round)
     case 131072: rad[0] = 16; rad[1] = 16; rad[2] = 2; rad[3] = 16; rad[4] = 16; nr = 5; break;
     default: return -1;
     }
-    g_round = round;
+    g_round = 1;
     init_F();
     int *rev = (int *)malloc(sizeof(int) * (size_t)n);
     orc_gen_rev(n, rev, rad, nr);
     hc *d = (hc *)malloc(sizeof(hc) * (size_t)n);
     for (long b = 0; b < batch; ++b) {
         const double *xb = x + 2 * n * b;
-        for (long i = 0; i < n; ++i) { d[i].x = rh(xb[2 * rev[i]]); d[i].y = rh(xb[2 * rev[i] + 1]); }
+        for (long i = 0; i < n; ++i) {
+            d[i].x = rp(ORC_R_INPUT, xb[2 * rev[i]]);
+            d[i].y = rp(ORC_R_INPUT, xb[2 * rev[i] + 1]);
+        }
         if (n == 256) l256_chunk(d, d);
         else if (n == 512) l512_chunk(d);
         else if (n == 1024) l1024_chunk(d);
@@ -539,7 +585,7 @@ int orc_ref_fp16_exec(const double *x, double *y, long n, long batch, int round)
     }
     free(d);
     free(rev);
-    g_round = 1;
+    g_rflags = ORC_R_INPUT | ORC_R_TABLE | ORC_R_MMA | ORC_R_VALU;
     return 0;
 }
 

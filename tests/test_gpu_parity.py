@@ -99,16 +99,20 @@ def test_inplace_matches_out_of_place(torch_cuda):
 @pytest.mark.parametrize("n", [256, 512, 1024, 131072])
 def test_plain_no_worse_than_reference_fp16(torch_cuda, n):
     """One-sided bound: plain FP16 is at least as accurate as the reference's
-    algorithm run at MCTYPE=half (emulated by the oracle), on the same inputs."""
+    algorithm run at MCTYPE=half (emulated by the oracle), on the same inputs.
+    What rocWMMA does with a half accumulator is bracketed by two emulations
+    (each mma_sync rounds to half; a complex_mul's four mma_sync share an FP32
+    accumulator and round once): held to the tighter of the two."""
     batch = max(1, 4096 // n)
     x = _input(n * batch, seed=11)
     xq = _quantised(x, "fp16")
     ref = oracle.fft(xq, (n,))
     y = run([n], batch, "fp16", "fp16", x)
     ours = oracle.rel_l2(y, ref)
-    theirs = oracle.rel_l2(oracle.ref_fp16_exec(xq, n), ref)
-    print(f"n={n} ours={ours:.3e} reference-fp16={theirs:.3e}")
-    assert ours <= theirs
+    per_mma = oracle.rel_l2(oracle.ref_fp16_exec(xq, n), ref)
+    fused = oracle.rel_l2(oracle.ref_fp16_exec(xq, n, points=oracle.R_ALL | oracle.R_FUSED), ref)
+    print(f"n={n} ours={ours:.3e} reference-fp16 per-mma={per_mma:.3e} fused={fused:.3e}")
+    assert ours <= min(per_mma, fused)
 
 
 @pytest.mark.parametrize("scaling", ["auto", "balanced"])

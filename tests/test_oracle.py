@@ -132,6 +132,158 @@ def test_reference_fp16_emulator_error_band(n):
     assert 1e-4 < err < 5e-3
 
 
+# ---- the FP16 emulator's rounding points, pinned by what the mathematics fixes ----
+U = 2.0 ** -11   # FP16 unit roundoff
+# radix-16 MMA stages per length (the 512/1024 tails are VALU radix-2/4 steps)
+MMA_STAGES = {256: 2, 512: 2, 1024: 2, 131072: 4}
+
+
+def _f16(a):
+    """numpy's own FP16 rounding (independent of oracle.round_half)."""
+    return np.asarray(a).astype(np.float16).astype(np.float64)
+
+
+def _gauss_rho2():
+    """Relative variance of FP16 rounding of a Gaussian (scale-invariant under
+    powers of two, and within 0.5% across an octave)."""
+    v = np.random.default_rng(0).standard_normal(2_000_000)
+    return float(np.mean((_f16(v) - v) ** 2) / np.mean(v ** 2))
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 131072])
+@pytest.mark.parametrize("fused", [False, True])
+def test_reference_fp16_emulator_exact_inputs(n, fused):
+    """FP16-exact inputs.  A delta gives exactly all ones at every rounding
+    point: every value it reaches is 1 or 0 (column 0 of every table and every
+    twiddle it meets is 1).  A single exponential at k0 in {0, N/4, N/2}
+    (entries 0, +-1, +-i) gives exactly N in the real part of bin k0; the other
+    bins hold only what the tables' zeros leave, at the 2^-24 level: rounded as
+    the reference builds them (cosf of a float argument, so cos(pi/2) becomes
+    -2^-24 in FP16), or FP64 tables whose products the MFMA takes in FP32.
+    131072 exceeds FP16's range (65504), so its exponentials are not run."""
+    extra = oracle.R_FUSED if fused else 0
+    d = np.zeros(n, complex)
+    d[0] = 1
+    assert np.array_equal(oracle.ref_fp16_exec(d, n, points=oracle.R_ALL | extra), np.ones(n))
+    if n > 65504:
+        return
+    m = np.arange(n)
+    for k0 in (0, n // 4, n // 2):
+        x = np.exp(2j * np.pi * k0 * m / n)
+        x = np.round(x.real) + 1j * np.round(x.imag)
+        want = np.zeros(n, complex)
+        want[k0] = n
+        for pts in (oracle.R_ALL, oracle.R_ALL & ~oracle.R_TABLE):
+            y = oracle.ref_fp16_exec(x, n, points=pts | extra)
+            assert y[k0].real == n, (k0, pts)
+            assert np.max(np.abs(y - want)) <= n * 2.0 ** -18, (k0, pts)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 131072])
+def test_reference_fp16_emulator_mma_error_model(n):
+    """First-order model of the half accumulator: every radix-16 stage rounds
+    each output once and, per mma_sync rounding, also the partial -(Fi Xi) --
+    half the output's variance on the 14 rows whose Fi is not zero -- so
+    rel_l2^2 = rho^2 S (1 + 7/16), or rho^2 S when the four mma_sync share one
+    FP32 accumulator (rho^2: FP16 rounding variance of a Gaussian, S: stages).
+    Band +-2%; the two predictions are 20% apart, so moving the rounding point
+    fails the check (asserted both ways)."""
+    rho2 = _gauss_rho2()
+    b = max(1, (1 << 19) // n)
+    rng = np.random.default_rng(1)
+    xq = oracle.round_half(rng.standard_normal(n * b) + 1j * rng.standard_normal(n * b))
+    ref = oracle.fft(xq, (n,))
+    s = MMA_STAGES[n]
+    pred = {False: np.sqrt(rho2 * s * (1 + 7 / 16)), True: np.sqrt(rho2 * s)}
+    for fused in (False, True):
+        pts = oracle.R_MMA | (oracle.R_FUSED if fused else 0)
+        meas = oracle.rel_l2(oracle.ref_fp16_exec(xq, n, points=pts), ref)
+        assert abs(meas / pred[fused] - 1) < 0.02, (fused, meas / U, pred[fused] / U)
+        assert abs(meas / pred[not fused] - 1) > 0.08, fused
+
+
+def _tw_chain(N, col, row, tables, valu):
+    """The stated twiddle recurrence: t = W(N, col*4*(row//4)), then
+    row % 4 times t = t * W(N, col); rounded tables and/or rounded complex
+    products (two products and their sum/difference each rounded)."""
+    def w(k):
+        z = np.exp(-2j * np.pi * k / N)
+        return complex(_f16(z.real), _f16(z.imag)) if tables else z
+    t, f = w(col * 4 * (row // 4)), w(col)
+    for _ in range(row % 4):
+        if valu:
+            t = complex(_f16(_f16(t.real * f.real) - _f16(t.imag * f.imag)),
+                        _f16(_f16(t.real * f.imag) + _f16(t.imag * f.real)))
+        else:
+            t = t * f
+    return t
+
+
+def _factorised_256(x, tables, valu, chain=True):
+    """X[k1 + 16 k2] = sum_m2 F[k2, m2] T[k1, m2] sum_m1 F[k1, m1] x[16 m1 + m2]
+    (Cooley-Tukey 16 x 16) with the 16-point matrix F and the twiddles T rounded
+    to FP16 (tables), and the twiddle products rounded (valu); chain=False takes
+    T straight from the rounded table (a moved rounding point)."""
+    k = np.arange(16)
+    F = np.exp(-2j * np.pi * np.outer(k, k) / 16)
+    if tables:
+        F = _f16(F.real) + 1j * _f16(F.imag)
+    if chain:
+        T = np.array([[_tw_chain(256, k1, m2, tables, valu) for m2 in k] for k1 in k])
+    else:
+        T = np.exp(-2j * np.pi * np.outer(k, k) / 256)
+        T = _f16(T.real) + 1j * _f16(T.imag)
+    X = x.reshape(-1, 16, 16)
+    Y = np.einsum("km,bmn->bkn", F, X)
+    P = Y * T[None]
+    if valu:
+        tr, ti = T.real[None], T.imag[None]
+        P = (_f16(_f16(Y.real * tr) - _f16(Y.imag * ti))
+             + 1j * _f16(_f16(Y.real * ti) + _f16(Y.imag * tr)))
+    return np.einsum("jn,bkn->bjk", F, P).reshape(-1)
+
+
+def test_reference_fp16_emulator_table_and_twiddle_points_256():
+    """The emulator's table and twiddle-arithmetic rounding points against the
+    textbook 16 x 16 factorisation with the same stated roundings (numpy's own
+    FP16 conversion, FP64 tables): the error each class adds agrees to 1e-4 (the
+    only differences are ulp-level flips where a float-argument cosf and an
+    FP64 cos round apart), and taking the twiddles straight from the table
+    instead of the recurrence moves the error by more than 10%.  With every
+    point on, the classes add in quadrature with the MMA model within 2%."""
+    rng = np.random.default_rng(2)
+    n, b = 256, 2048
+    xq = oracle.round_half(rng.standard_normal(n * b) + 1j * rng.standard_normal(n * b))
+    ref = oracle.fft(xq, (n,))
+    meas = {}
+    for tables, valu in ((True, False), (False, True), (True, True)):
+        pts = (oracle.R_TABLE if tables else 0) | (oracle.R_VALU if valu else 0)
+        meas[tables, valu] = oracle.rel_l2(oracle.ref_fp16_exec(xq, n, points=pts), ref)
+        model = oracle.rel_l2(_factorised_256(xq, tables, valu), ref)
+        assert abs(meas[tables, valu] / model - 1) < 1e-4, (tables, valu)
+        moved = oracle.rel_l2(_factorised_256(xq, tables, valu, chain=False), ref)
+        assert abs(meas[tables, valu] / moved - 1) > 0.10, (tables, valu)
+    rho2 = _gauss_rho2()
+    for fused in (False, True):
+        pts = oracle.R_ALL | (oracle.R_FUSED if fused else 0)
+        whole = oracle.rel_l2(oracle.ref_fp16_exec(xq, n, points=pts), ref)
+        mma2 = rho2 * MMA_STAGES[n] * (1 if fused else 1 + 7 / 16)
+        assert abs(whole / np.sqrt(meas[True, True] ** 2 + mma2) - 1) < 0.02, fused
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 131072])
+def test_reference_fp16_emulator_variants_bracket(n):
+    """The fused-accumulator variant rounds a subset of the default's points,
+    so on random inputs it is the more accurate (the tighter bound)."""
+    rng = np.random.default_rng(3)
+    b = max(1, 4096 // n)
+    xq = oracle.round_half(rng.standard_normal(n * b) + 1j * rng.standard_normal(n * b))
+    ref = oracle.fft(xq, (n,))
+    per_mma = oracle.rel_l2(oracle.ref_fp16_exec(xq, n), ref)
+    fused = oracle.rel_l2(oracle.ref_fp16_exec(xq, n, points=oracle.R_ALL | oracle.R_FUSED), ref)
+    assert fused < per_mma
+
+
 def test_round_half_known_values():
     v = np.array([1.0, 1 + 2 ** -11, 1 + 3 * 2 ** -11, 65504.0, 65520.0, 2 ** -25, 3 * 2 ** -25])
     r = oracle.round_half(v.astype(complex)).real
