@@ -1,6 +1,6 @@
 """Throughput and accuracy of every workload the library is measured on.
 
-    python -m mcfft_amd.benchmarks [--config c1,c2,c3,c4,c5,c6,ragged_small,real,layout] [--reps 10]
+    python benchmarks.py [--config c1,c2,c3,c4,c5,c6,ragged_small,real,layout] [--reps 10]
 
 One JSON line per (config, precision): GFLOP/s (5 N log2 N, N = elements per
 transform), fraction of the 8 TB/s HBM roofline computed from algorithmic bytes
@@ -27,7 +27,7 @@ import sys
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -95,7 +95,7 @@ def run_batched(torch, name, dims, batch, x, fi, fo, scaling, reps, check_transf
     p.exec(ins[0], d_out, direction)
     torch.cuda.synchronize()
     k = min(batch, check_transforms)
-    y = mc.from_device(d_out[: k * n], fo, p.exponent)
+    y = p.result(d_out, k)
     xq = mc.decode(mc.encode(x[: k * n], fi), fi)
     ref = oracle.fft(xq, tuple(dims), sign=-1 if direction == mc.FORWARD else 1)
     res.update({"rel_l2": oracle.rel_l2(y, ref), "max_abs": oracle.max_abs(y, ref), "checked": k})
@@ -347,7 +347,7 @@ def bench_real(torch, reps, n=131072, batch=256):
     p.exec(d_in, d_out, mc.FORWARD)
     torch.cuda.synchronize()
     k = 4
-    y = mc.from_device(d_out[: k * (h + 1)], "fp16", p.exponent)
+    y = p.result(d_out, k)
     xq = mc.pairs_to_real(mc.decode(mc.encode(z[: k * h], "fp16"), "fp16"), n)
     ref = oracle.fft(xq.reshape(-1).astype(np.complex128), (n,)).reshape(k, n)[:, : h + 1].reshape(-1)
     flops = 2.5 * n * math.log2(n) * batch   # half of the complex transform's 5 N log2 N
@@ -408,11 +408,11 @@ def main():
     cfgs = args.config.split(",")
     rng = np.random.default_rng(1)
     table = {
-        "c1": ([256], 4096, "balanced", 4096),
-        "c2": ([131072], 256, "balanced", 8),
-        "c3": ([4096, 4096], 4, "full", 1),
-        "c4": ([512, 512, 512], 1, "balanced", 1),
-        "c6": ([65536], 4096, "balanced", 16),
+        "c1": ([256], 4096, "auto", 4096),
+        "c2": ([131072], 256, "auto", 8),
+        "c3": ([4096, 4096], 4, "auto", 1),
+        "c4": ([512, 512, 512], 1, "auto", 1),
+        "c6": ([65536], 4096, "auto", 16),
     }
     for cfg in cfgs:
         if cfg == "layout":
@@ -438,7 +438,7 @@ def main():
             r = run_batched(torch, cfg, dims, batch, x, fi, fo, scaling, args.reps, check)
             print(json.dumps(r), flush=True)
         if cfg == "c2":  # the inverse leg of the forward+inverse config
-            r = run_batched(torch, "c2-inverse", dims, batch, x, "split", "split", "balanced", args.reps, check,
+            r = run_batched(torch, "c2-inverse", dims, batch, x, "split", "split", "auto", args.reps, check,
                             direction=mc.INVERSE)
             print(json.dumps(r), flush=True)
         del x

@@ -1,9 +1,9 @@
 """Condense rocprofv3 output into the short summaries kept under profiles/.
 
-    python -m mcfft_amd.profile_summary stats <kernel_stats.csv> [--title T] [--command C]
-    python -m mcfft_amd.profile_summary traffic --fetch F.csv --write W.csv \
+    python profile_summary.py stats <kernel_stats.csv> [--title T] [--command C]
+    python profile_summary.py traffic --fetch F.csv --write W.csv \
         --copy-fetch CF.csv --copy-write CW.csv --copy-bytes B [--command C]
-    python -m mcfft_amd.profile_summary mfma --pmc P.csv --stats S.csv [--elements k=n,...]
+    python profile_summary.py mfma --pmc P.csv --stats S.csv [--elements k=n,...]
 
 `stats` prints a table of the per-kernel summary (`--kernel-trace --stats`).
 `traffic` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) of two
