@@ -34,7 +34,9 @@
  * the stores of statically scaled 1D plans of exactly 65536 points (their
  * single-pass kernel has no registers to spare).
  *
- * Sizes: every length a power of two; 1D 16 <= N <= 2^23; 2D/3D outer sides
+ * Sizes: every length a power of two; 1D 16 <= N <= 2^27 (one pass up to 2^14,
+ * 2^16 for FP16/BF16 output; two passes up to 2^23; three passes from 2^24,
+ * one transform at a time, with a workspace of one transform); 2D/3D outer sides
  * 16..4096 and innermost side up to 2^16 for FP16/BF16 output, 2^14 for split
  * output (the innermost axis runs as one single-pass row kernel, and a split
  * row of 2^15 points does not fit the LDS).  The number of

@@ -48,6 +48,9 @@ inline void fill_bfp(PassArgs& a, const PlanCore& c, int pi, long long t0, long 
   a.bfp_gpt = pp.groups_per_transform > 0 ? (int)pp.groups_per_transform : 1;
   a.bfp_q0_lg = ilog2(c.passes[0].k->Q);
   a.bfp_tpg0 = c.passes[0].tiles_per_group;
+  a.bfp_rg = pp.rc_g < 0 ? (int)pp.L : pp.rc_g;
+  a.bfp_rm = pp.rc_m;
+  a.bfp_rq = pp.rc_q;
   a.bfp_state = b.state;
   a.bfp_slots = b.slots;
   a.bfp_tile_e = b.tile_e;

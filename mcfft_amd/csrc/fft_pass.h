@@ -50,12 +50,12 @@ constexpr float kSplitLoInv = 1.0f / 2048.0f;
 // evaluate them in the kernel like FP16/BF16 passes (false).
 constexpr bool kSplitTableTwiddles = true;
 
-enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4 };
 
 // Element (q, m) of a tile (instance q, position m) is at, in elements,
 //   ROW : in/out  base + q*L + m                  (contiguous rows)
 //   ROWT: in      base + q*L + m,  out  base + q + m*stride_out  (transposed store)
-//   COL, COLF: in base + q + m*stride_in, out base + q + m*stride_out (columns)
+//   COL, COLF, COLM: in base + q + m*stride_in, out base + q + m*stride_out (columns)
 // with base = group*sg + tile*st for the tile's group/tile indices.
 struct PassArgs {
   const void* in;
@@ -70,8 +70,10 @@ struct PassArgs {
   float out_scale;            // last-stage scale when there is no four-step table
   const f32x2* stage_tw;      // post-twiddle tables [stage][j][16] (see plan_core)
   const u32x4* aops;          // DFT-matrix MFMA operands [64 lanes][4]: A1, A2, A1lo, A2lo
-  const f32x2* fs_tw;         // four-step twiddles W_N^(m*column), N1 x fs_cols entries
+  const f32x2* fs_tw;         // four-step twiddles W_N^(m*column), N1 x fs_cols entries, or (fs_lo_bits > 0)
+                              // W_N^(m*(column % 2^b)) [N1][2^b] then W_N^(m*(column - column % 2^b)) [N1][fs_cols >> b]
   int fs_cols;                // N2 (columns of the four-step decomposition)
+  int fs_lo_bits;             // b: split four-step table factorised in two (0: one table)
   long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
   int stage_tw_entries, fs_tw_entries;
   float st_scale[6];          // 2^-shift of stage P (twiddles computed in the kernel carry it)
@@ -85,6 +87,8 @@ struct PassArgs {
   int bfp_gpt;                // groups of this pass per transform
   int bfp_nt;                 // transforms of the launch
   int bfp_q0_lg, bfp_tpg0;    // first pass: log2 of its tile width, tiles per group
+  int bfp_rg, bfp_rm, bfp_rq; // reconcile pass: first-pass instance of element (q, m) of group g
+                              // (see rescale_stage0)
   ExecState* bfp_state;
   unsigned long long* bfp_slots;  // [2t]: max e, [2t+1]: min e of t's first-pass tiles (see bfp.h)
   signed char* bfp_tile_e;    // exponents of the launch's first-pass tiles
@@ -278,8 +282,10 @@ MCFFT_DEV void dft_small(f32x2 (&v)[T], bool inv) {
 constexpr int ipow16(int s) { return s == 0 ? 1 : 16 * ipow16(s - 1); }
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
-// W_M^e * scale for e < 2^24 (forward exp(-2 pi i e / M), inverse exp(+...));
-// inv_m = 1/M is a power of two, so e * inv_m is exact
+// W_M^e * scale (forward exp(-2 pi i e / M), inverse exp(+...)); inv_m = 1/M
+// is a power of two, so e * inv_m is exact for e < 2^24 (larger e, from the
+// first pass of a 1D plan above 2^24 points, round to 24 bits: an angle error
+// below 2^-24 of a revolution)
 MCFFT_DEV f32x2 twiddle_rev(int e, float inv_m, bool inv, float scale) {
   f32x2 w = cis_rev((float)e * inv_m);
   w.x *= scale;
@@ -304,11 +310,12 @@ template <int E> struct For<E, E> {
 //   Q         instances per workgroup tile
 //   NW        waves per workgroup
 //   CM        compute/storage mode
-//   KIND      ROW / ROWT / COL / COLF (addressing above).  COL and COLF run
-//             the MFMA columns and LDS rows over instances (q fastest); COLF
-//             multiplies its outputs by W_N^(m * column); ROWT's last stage writes
-//             LDS and a copy loop stores instance-fastest (coalesced for the
-//             transposed four-step store).
+//   KIND      ROW / ROWT / COL / COLF / COLM (addressing above).  The column
+//             kinds run the MFMA columns and LDS rows over instances (q
+//             fastest); COLF (a plan's first pass) and COLM (the middle pass of a
+//             three-pass 1D plan) multiply their outputs by W_N^(m * column);
+//             ROWT's last stage writes LDS and a copy loop stores
+//             instance-fastest (coalesced for the transposed four-step store).
 // ---------------------------------------------------------------------------
 template <int S, int T, int Q, int NW, int CM, int KIND>
 struct PassCfg {
@@ -316,9 +323,9 @@ struct PassCfg {
   static constexpr int Q_ = Q, NW_ = NW, CM_ = CM;
   static constexpr int NT = NW * 64;
   static constexpr int NSTAGE = S + (T > 1 ? 1 : 0);
-  static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF);
+  static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF || KIND == KIND_COLM);
   static constexpr bool STAGED = (KIND == KIND_ROWT);
-  static constexpr bool FOURSTEP = (KIND == KIND_COLF);
+  static constexpr bool FOURSTEP = (KIND == KIND_COLF || KIND == KIND_COLM);
   static constexpr int ROW = L + L / 16 + 1;  // padded LDS row (contiguous-row layout)
   static constexpr int LDS_ELEMS = QFAST ? Q * L : Q * ROW;
   typedef typename Traits<CM>::T El;
@@ -420,23 +427,28 @@ MCFFT_DEV void bfp_last_tile(const PassArgs& a, int Q, int inst0, int grp, int t
 // AUTO scaling, pass after the first, when the first pass's tiles of this
 // transform chose different exponents: bring every stage-0 input to the common
 // exponent E (the largest), multiplying by 2^(e_tile - E) <= 1.  Element (q, m) of
-// group `grp` is instance i0 = (grp % gpt) * L + m of first-pass group grp / gpt
-// (2D and four-step 1D: gpt = 1, i0 = m; 3D y-axis pass: gpt = NX), made by
-// first-pass tile (grp / gpt) * tpg0 + (i0 >> q0_lg).
+// group `grp` (tile instances from inst0) is instance
+//   i0 = (grp % gpt) * rg + m * rm + (inst0 + q) * rq
+// of first-pass group grp / gpt -- 2D and four-step 1D: gpt = 1, i0 = m; 3D
+// y-axis pass: gpt = NX, rg = L; middle pass of a three-pass 1D plan
+// N1 x N2 x N3: rg = 0, rm = N3, rq = 1 -- made by first-pass tile
+// (grp / gpt) * tpg0 + (i0 >> q0_lg).
 template <class C, class Tr>
-MCFFT_DEV void rescale_stage0(const PassArgs& a, int wave, int lane, int grp, int E, typename C::El (*x)[8]) {
+MCFFT_DEV void rescale_stage0(const PassArgs& a, int wave, int lane, int grp, int inst0, int E,
+                              typename C::El (*x)[8]) {
   constexpr int L = C::L;
   const int h = lane >> 5;
-  const int g0 = grp / a.bfp_gpt, ibase = (grp - g0 * a.bfp_gpt) * L;
+  const int g0 = grp / a.bfp_gpt, ibase = (grp - g0 * a.bfp_gpt) * a.bfp_rg;
   const long long tbase = (long long)g0 * a.bfp_tpg0;
   for (int k = 0; k < C::TPW; ++k) {
     const int ti = wave + k * C::NW_;
     if (ti < C::NCOLT) {
       int q, j;
       C::template colmap<16>(ti * 32 + (lane & 31), q, j);
+      const int iq = ibase + (inst0 + q) * a.bfp_rq;
       for (int t = 0; t < 8; ++t) {
         const int m = j + (8 * h + t) * (L / 16);
-        const int e = (int)g_ld(a.bfp_tile_e, tbase + ((ibase + m) >> a.bfp_q0_lg));
+        const int e = (int)g_ld(a.bfp_tile_e, tbase + ((iq + m * a.bfp_rm) >> a.bfp_q0_lg));
         const int d = e - E;
         if constexpr (C::CM_ == CM_F16) {
           // 2^d as binary16 (normal down to 2^-14, subnormal to 2^-24), both
@@ -534,7 +546,11 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
     const float inv_n = 1.0f / ((float)L * (float)a.fs_cols);
     auto fs_twiddle = [&](int q, int m) -> f32x2 {
       if constexpr (TABLE_TW) {
-        return __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + inst0 + q) * 8));
+        const int col = inst0 + q, b = a.fs_lo_bits;
+        if (b == 0) return __builtin_bit_cast(f32x2, buf_ld64(gfs, (m * a.fs_cols + col) * 8));
+        const int lo = (m << b) + (col & ((1 << b) - 1));
+        const int hi = (L << b) + m * (a.fs_cols >> b) + (col >> b);
+        return cmul(__builtin_bit_cast(f32x2, buf_ld64(gfs, lo * 8)), __builtin_bit_cast(f32x2, buf_ld64(gfs, hi * 8)));
       } else {
         return twiddle_rev(m * (inst0 + q), inv_n, inv, last_scale);
       }
@@ -594,7 +610,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
           nsc = bfp_first_tile(a, acc, CM, (unsigned*)((unsigned char*)lds + C::RED_OFF), NW, tile, t_loc, e_out,
                                *sacc);
         } else if (BFP && !FIRST_KIND && rescale) {
-          rescale_stage0<C, Tr>(a, wave, lane, grp, e_out - a.bfp_e_base, x);
+          rescale_stage0<C, Tr>(a, wave, lane, grp, inst0, e_out - a.bfp_e_base, x);
         }
       } else {
         for (int k = 0; k < TPW; ++k) {
@@ -806,7 +822,7 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   // b % 8), each XCD with its own L2, so within every aligned group of 32 blocks
   // the 4 blocks b, b+8, b+16, b+24 (one XCD) take 4 adjacent tiles.  This only
   // changes which block runs which tile.
-  constexpr bool XCD_GROUP = (KIND == KIND_COL || KIND == KIND_COLF) && Q * (int)sizeof(typename Traits<CM>::T) < 128;
+  constexpr bool XCD_GROUP = PassCfg<S, T, Q, NW, CM, KIND>::QFAST && Q * (int)sizeof(typename Traits<CM>::T) < 128;
   int b = block_id();
   const int g = grid_size();
   if constexpr (XCD_GROUP) {

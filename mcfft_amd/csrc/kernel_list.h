@@ -1,12 +1,13 @@
 // kernel_list.h -- the compiled pass configurations (X-macro lists).
 //
 // X(S, T, Q, NW, CM, KIND): L = 16^S * T points, Q instances per workgroup,
-// NW waves, compute mode CM, KIND 0 ROW / 1 ROWT / 2 COL / 3 COLF.  Tile sizes keep one workgroup's LDS at <= 128 KiB
+// NW waves, compute mode CM, KIND 0 ROW / 1 ROWT / 2 COL / 3 COLF / 4 COLM.  Tile sizes keep one workgroup's LDS at <= 128 KiB
 // (gfx950 has 160 KiB per CU) and a tile at >= 32 sixteen-point MFMA columns.
 //   ROW   contiguous rows (1D single pass, 2D/3D innermost axis)
 //   ROWT  contiguous rows, transposed store (last pass of a four-step 1D plan)
 //   COL   strided columns (2D/3D outer axes)
 //   COLF  strided columns + four-step twiddle (first pass of a four-step plan)
+//   COLM  strided columns + four-step twiddle (middle pass of a three-pass 1D plan)
 #pragma once
 
 // ---- ROW: tile of 2048 elements (4096 at L = 4096) ----
@@ -126,6 +127,21 @@
   MCFFT_COL_TUNE_CM(X, 0, 3)                       \
   X(2, 1, 16, 4, 2, 3)
 
+// COLM: the middle pass of a three-pass 1D plan (N2 = 2^4..2^9 points, columns
+// 2^5..2^9 apart)
+#define MCFFT_COLM_LIST_CM(X, CM, Q512) \
+  X(1, 1, 32, 4, CM, 4)                 \
+  X(1, 2, 32, 4, CM, 4)                 \
+  X(1, 4, 32, 4, CM, 4)                 \
+  X(1, 8, 32, 4, CM, 4)                 \
+  X(2, 1, 32, 8, CM, 4)                 \
+  X(2, 2, Q512, 8, CM, 4)
+
+#define MCFFT_COLM_LIST(X)         \
+  MCFFT_COLM_LIST_CM(X, 0, 32)     \
+  MCFFT_COLM_LIST_CM(X, 1, 32)     \
+  MCFFT_COLM_LIST_CM(X, 2, 16)
+
 namespace mcfft {
 struct KernelInfo {
   const char* name;
@@ -143,6 +159,8 @@ extern const KernelInfo kColKernels[];
 extern const int kColKernelCount;
 extern const KernelInfo kColFKernels[];
 extern const int kColFKernelCount;
+extern const KernelInfo kColMKernels[];
+extern const int kColMKernelCount;
 // ragged batches: pass variants per compute mode and the persistent kernels
 extern const KernelInfo kRaggedVariantInfo[3][30];
 extern const KernelInfo kRaggedKernels[3];

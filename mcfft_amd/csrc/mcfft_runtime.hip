@@ -1132,6 +1132,7 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
         a.fs_tw = plan->tw[d] + pp.fs_tw_off;
         a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);
         a.fs_cols = pp.fs_cols;
+        a.fs_lo_bits = pp.fs_lo_bits;
       }
       fill_bfp(a, c, pi, t0, nt, plan->real_kind == 2, in_exp,
                BfpBuffers{plan->state, plan->slots, plan->tile_e, plan->out_exp});

@@ -2,6 +2,7 @@
 #include "plan_core.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <sstream>
 
 namespace mcfft {
@@ -30,7 +31,7 @@ bool split_len(long long L, int& S, int& T) {
   return true;
 }
 
-enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3 };
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4 };
 
 // Default tile shapes that measured fastest on MI355X (autotuner sweep over
 // batched 1D/2D/3D plans); other combinations use the first compiled variant.
@@ -60,8 +61,9 @@ bool kernel_serves(const KernelInfo& k, int scaling) {
 }
 
 static const KernelInfo* find_kernel(int S, int T, int cm, int kind, int scaling, long long ninst = 0) {
-  const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
-  const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  const KernelInfo* lists[5] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels, kColMKernels};
+  const int counts[5] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount,
+                         kColMKernelCount};
   auto divides = [&](const KernelInfo& k) { return (ninst <= 0 || ninst % k.Q == 0) && kernel_serves(k, scaling); };
   for (const PreferredShape& p : kPreferred) {
     if (p.kind != kind || p.S != S || p.T != T || p.cm != cm) continue;
@@ -117,6 +119,128 @@ std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, 
     ns = nsn;
   }
   return shifts;
+}
+
+// 1D lengths of at least 2^this run as three passes (MCFFT_THREE_PASS_MIN_LG
+// lowers it, down to 2^15, so that the structure is exercised at small sizes)
+static int three_pass_min_lg() {
+  int v = kThreePassMinLg;
+  if (const char* e = getenv("MCFFT_THREE_PASS_MIN_LG")) {
+    const int w = atoi(e);
+    if (w >= 15 && w <= kThreePassMinLg) v = w;
+  }
+  return v;
+}
+
+// Three-pass 1D plan, one transform per chunk: N = N1 * N2 * N3 (n = n1*N2*N3 +
+// n2*N3 + n3, output k = k1 + N1*k2 + N1*N2*k3).
+//   A (COLF, in -> out):  N1-point columns (stride M = N2*N3) times W_N^(k1*m),
+//                         m = n2*N3 + n3, in place
+//   B (COLM, out -> ws):  N2-point columns (stride N3) of each row k1 times
+//                         W_M^(k2*n3), stored at (k2*N1 + k1)*N3 + n3
+//   C (ROWT, ws -> out):  N3-point rows (k2*N1 + k1) stored transposed at
+//                         (k2*N1 + k1) + N1*N2*k3
+// Pass B's store puts the rows pass C needs in the order of its transposed
+// store, so every pass is one of the kernel family's addressing modes.
+static int plan_three_pass(PlanCore& p, long long N, int cm, int scaling, int ws_fmt, int& total_shift) {
+  const int lg = ilog2(N);
+  if (lg > kMaxLg1D) return ST_UNSUPPORTED;
+  const int lg3 = lg / 3, lg1 = (lg - lg3 + 1) / 2, lg2 = lg - lg3 - lg1;
+  const long long N1 = 1LL << lg1, N2 = 1LL << lg2, N3 = 1LL << lg3, M = N2 * N3;
+  int S1, T1, S2, T2, S3, T3;
+  if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2) || !split_len(N3, S3, T3)) return ST_UNSUPPORTED;
+  const KernelInfo* ka = find_kernel(S1, T1, cm, KIND_COLF, scaling, M);
+  const KernelInfo* kb = find_kernel(S2, T2, cm, KIND_COLM, scaling, N3);
+  const KernelInfo* kc = find_kernel(S3, T3, cm, KIND_ROWT, scaling, N1 * N2);
+  if (!ka || !kb || !kc) return ST_UNSUPPORTED;
+  p.chunk = 1;
+  p.n_chunks = p.batch;
+  p.ws_elems = N;
+  // A: columns of length N1 at stride M, one group per transform
+  PassPlan a;
+  a.k = ka;
+  a.kind = KIND_COLF;
+  a.L = N1;
+  a.src = BUF_IN;
+  a.dst = BUF_OUT;
+  a.src_fmt = p.in_fmt;
+  a.stride_in = a.stride_out = (int)M;
+  a.in_sg = a.out_sg = N;
+  a.in_st = a.out_st = ka->Q;
+  a.n_inst = M;
+  a.tiles_per_group = (int)(M / ka->Q);
+  a.groups = 1;
+  a.groups_per_transform = 1;
+  a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
+  a.fourstep = true;
+  a.out_scale = pow2(-a.shifts.back());
+  a.fs_cols = (int)M;
+  a.fs_tw_off = (long long)p.tw.size();
+  if (cm == 2) {
+    // W_N^(k1*m) = W_N^(k1*(m % 2^b)) * W_N^(k1*(m - m % 2^b)): two tables of
+    // N1 * 2^b and N1 * M / 2^b entries instead of one of N
+    const int b = (lg2 + lg3 + 1) / 2;
+    a.fs_lo_bits = b;
+    for (long long k1 = 0; k1 < N1; ++k1)
+      for (long long c = 0; c < (1LL << b); ++c) p.tw.push_back({(k1 * c) % N, N, a.out_scale});
+    for (long long k1 = 0; k1 < N1; ++k1)
+      for (long long c = 0; c < (M >> b); ++c) p.tw.push_back({(k1 * (c << b)) % N, N, 1.f});
+  }
+  for (int s : a.shifts) total_shift += s;
+  p.passes.push_back(a);
+  // B: group k1 = row of M elements; N3 columns of length N2 (stride N3), stored
+  // at stride N1*N3 from k1*N3
+  PassPlan b;
+  b.k = kb;
+  b.kind = KIND_COLM;
+  b.L = N2;
+  b.src = BUF_OUT;
+  b.dst = BUF_WS;
+  b.src_fmt = p.out_fmt;
+  b.stride_in = (int)N3;
+  b.stride_out = (int)(N1 * N3);
+  b.in_sg = M;
+  b.out_sg = N3;
+  b.in_st = b.out_st = kb->Q;
+  b.n_inst = N3;
+  b.tiles_per_group = (int)(N3 / kb->Q);
+  b.groups = N1;
+  b.groups_per_transform = N1;
+  b.shifts = add_stage_tables(p.tw, b, S2, T2, scaling);
+  b.fourstep = true;
+  b.out_scale = pow2(-b.shifts.back());
+  b.fs_cols = (int)N3;
+  b.fs_tw_off = (long long)p.tw.size();
+  if (cm == 2)
+    for (long long k2 = 0; k2 < N2; ++k2)
+      for (long long n3 = 0; n3 < N3; ++n3) p.tw.push_back({(k2 * n3) % M, M, b.out_scale});
+  // reconcile: element (q = n3, m = n2) came from pass A's instance n2*N3 + n3
+  b.rc_g = 0;
+  b.rc_m = (int)N3;
+  b.rc_q = 1;
+  for (int s : b.shifts) total_shift += s;
+  p.passes.push_back(b);
+  // C: N1*N2 rows of N3 points, transposed store at stride N1*N2
+  PassPlan c;
+  c.k = kc;
+  c.kind = KIND_ROWT;
+  c.L = N3;
+  c.src = BUF_WS;
+  c.dst = BUF_OUT;
+  c.src_fmt = ws_fmt;
+  c.stride_out = (int)(N1 * N2);
+  c.in_sg = c.out_sg = N;
+  c.in_st = (long long)kc->Q * N3;
+  c.out_st = kc->Q;
+  c.n_inst = N1 * N2;
+  c.tiles_per_group = (int)(N1 * N2 / kc->Q);
+  c.groups = 1;
+  c.groups_per_transform = 1;
+  c.shifts = add_stage_tables(p.tw, c, S3, T3, scaling);
+  c.out_scale = pow2(-c.shifts.back());
+  for (int s : c.shifts) total_shift += s;
+  p.passes.push_back(c);
+  return ST_OK;
 }
 
 int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
@@ -201,15 +325,17 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   int st = ST_OK;
   if (rank == 1) {
     const long long N = p.dims[0];
+    const int lg = ilog2(N);
     int S0, T0;
     if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW, scaling)) {  // single pass: 2^15, 2^16 in FP16/BF16
       p.chunk = batch;
       p.n_chunks = 1;
       st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);
+    } else if (lg >= three_pass_min_lg()) {
+      st = plan_three_pass(p, N, cm, scaling, ws_fmt, total_shift);
     } else {
       // four-step: N = N1 * N2; columns of length N1 (stride N2) with twiddle
       // W_N^(k1*n2), then rows of length N2 stored transposed X[k1 + N1*k2]
-      const int lg = ilog2(N);
       int lg2 = (lg + 1) / 2;
       if (lg2 > 11) lg2 = 11;
       const int lg1 = lg - lg2;
@@ -304,8 +430,8 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
 
 std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm, int scaling) {
   std::vector<const KernelInfo*> out;
-  const KernelInfo* lists[4] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels};
-  const int counts[4] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount};
+  const KernelInfo* lists[5] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels, kColMKernels};
+  const int counts[5] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount, kColMKernelCount};
   if (!pp.k) return out;
   for (int i = 0; i < counts[pp.kind]; ++i) {
     const KernelInfo& k = lists[pp.kind][i];

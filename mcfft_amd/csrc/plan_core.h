@@ -25,6 +25,10 @@ enum Scaling { SCALE_NONE = 0, SCALE_BALANCED = 1, SCALE_FULL = 2, SCALE_AUTO = 
 // maxima lie within one step of 2^4 agree on their exponent, so the pass after
 // the first has nothing to reconcile for such data
 constexpr int kBfpQuantumMultiPass = 4;
+// 1D lengths: single pass up to 2^16 (FP16/BF16), two-pass four-step up to
+// 2^23, three passes from 2^24 to 2^kMaxLg1D
+constexpr int kThreePassMinLg = 24;
+constexpr int kMaxLg1D = 27;
 
 // one twiddle-table entry: exp(sign * 2 pi i * e / M) * scale (sign = -1 forward)
 struct TwSpec {
@@ -34,7 +38,7 @@ struct TwSpec {
 
 struct PassPlan {
   const KernelInfo* k = nullptr;
-  int kind = 0;                 // 0 ROW, 1 ROWT, 2 COL, 3 COLF
+  int kind = 0;                 // 0 ROW, 1 ROWT, 2 COL, 3 COLF, 4 COLM
   long long L = 0;              // points per instance
   int src = BUF_IN, dst = BUF_OUT;
   int src_fmt = 0;
@@ -49,6 +53,10 @@ struct PassPlan {
   bool fourstep = false;
   long long fs_tw_off = 0;      // four-step table W_N^(k1*n2): N1 x fs_cols entries
   int fs_cols = 0;
+  int fs_lo_bits = 0;           // > 0: the table in two factors (fft_pass.h, PassArgs::fs_tw)
+  // reconcile pass (AUTO): element (q, m) of group g came from first-pass instance
+  // (g % groups_per_transform) * rc_g + m * rc_m + q * rc_q (rc_g < 0: L)
+  int rc_g = -1, rc_m = 1, rc_q = 0;
   std::vector<int> shifts;      // per stage, for reporting
 };
 

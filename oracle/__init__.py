@@ -39,6 +39,8 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         L.orc_fft_many.argtypes = [dp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), ctypes.c_long, ctypes.c_int]
         L.orc_fft_many.restype = None
+        L.orc_fft_1d_large.argtypes = [dp, ctypes.c_long, ctypes.c_long, ctypes.c_int]
+        L.orc_fft_1d_large.restype = None
         L.orc_dft_naive.argtypes = [dp, dp, ctypes.c_long, ctypes.c_int]
         L.orc_dft_naive.restype = None
         L.orc_generate_data.argtypes = [dp, ctypes.c_long, ctypes.c_long, ctypes.c_int]
@@ -73,7 +75,22 @@ def fft(x, axes_shape, sign: int = -1) -> np.ndarray:
     dims = (ctypes.c_long * len(axes_shape))(*axes_shape)
     per = int(np.prod(axes_shape))
     batch = a.size // per
-    lib().orc_fft_many(_dptr(a.view(np.float64)), len(axes_shape), dims, batch, int(sign))
+    if len(axes_shape) == 1 and per >= LARGE_1D and batch < num_threads() and per & (per - 1) == 0:
+        lib().orc_fft_1d_large(_dptr(a.view(np.float64)), per, batch, int(sign))
+    else:
+        lib().orc_fft_many(_dptr(a.view(np.float64)), len(axes_shape), dims, batch, int(sign))
+    return a
+
+
+# single 1D transforms from this length on (fewer than the threads) run the
+# four-step path, parallel inside each transform
+LARGE_1D = 1 << 20
+
+
+def fft_large(x, n: int, sign: int = -1) -> np.ndarray:
+    """The four-step path directly (for its pin against orc_fft_many)."""
+    a = _as_c128(x).copy()
+    lib().orc_fft_1d_large(_dptr(a.view(np.float64)), n, a.size // n, int(sign))
     return a
 
 

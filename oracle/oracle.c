@@ -228,6 +228,68 @@ void orc_fft_many(double *data, int rank, const long *dims, long batch, int sign
     }
 }
 
+/*
+ * One long 1D FFT (n = 2^k, k >= 2) per transform, parallel inside the
+ * transform: the textbook four-step factorisation n = n1 * n2,
+ *   X[k1 + n1 k2] = sum_n2 W_n2^(n2 k2) W_n^(n2 k1) sum_n1 x[n1 n2_count + n2] W_n1^(n1 k1),
+ * with every short FFT done by fft1 above (FP64 Stockham) and the twiddle
+ * W_n^(n2 k1) from twiddle_ld.  orc_fft_many runs a single transform on one
+ * thread; at 2^24..2^27 points that takes minutes, this minutes / threads.
+ * Pinned by equality with orc_fft_many (tests/test_oracle.py).
+ */
+void orc_fft_1d_large(double *data, long n, long batch, int sign)
+{
+    int lg = 0;
+    while ((1L << lg) < n) ++lg;
+    const long n1 = 1L << (lg / 2), n2 = n / n1;
+    fftplan p1, p2;
+    plan_init(&p1, n1, sign);
+    plan_init(&p2, n2, sign);
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    for (long b = 0; b < batch; ++b) {
+        double *x = data + 2 * n * b;
+#pragma omp parallel
+        {
+            double *col = (double *)malloc(sizeof(double) * 2 * (size_t)n1);
+            double *work = (double *)malloc(sizeof(double) * 2 * (size_t)(n1 > n2 ? n1 : n2));
+            double *scratch = (double *)malloc(sizeof(double) * 2 * 16);
+            /* columns n2: length n1 at stride n2, then the twiddle W_n^(k1 n2) */
+#pragma omp for schedule(dynamic, 16)
+            for (long c = 0; c < n2; ++c) {
+                for (long i = 0; i < n1; ++i) {
+                    col[2 * i] = x[2 * (i * n2 + c)];
+                    col[2 * i + 1] = x[2 * (i * n2 + c) + 1];
+                }
+                fft1(&p1, col, work, scratch);
+                for (long k1 = 0; k1 < n1; ++k1) {
+                    double wr, wi;
+                    twiddle_ld(k1 * c, n, sign, &wr, &wi);
+                    const double re = col[2 * k1], im = col[2 * k1 + 1];
+                    x[2 * (k1 * n2 + c)] = re * wr - im * wi;
+                    x[2 * (k1 * n2 + c) + 1] = re * wi + im * wr;
+                }
+            }
+            /* rows k1: length n2, contiguous; result k2 goes to k1 + n1 k2 */
+#pragma omp for schedule(dynamic, 16)
+            for (long k1 = 0; k1 < n1; ++k1) {
+                double *row = x + 2 * k1 * n2;
+                fft1(&p2, row, work, scratch);
+                for (long k2 = 0; k2 < n2; ++k2) {
+                    t[2 * (k1 + n1 * k2)] = row[2 * k2];
+                    t[2 * (k1 + n1 * k2) + 1] = row[2 * k2 + 1];
+                }
+            }
+            free(col);
+            free(work);
+            free(scratch);
+        }
+        memcpy(x, t, sizeof(double) * 2 * (size_t)n);
+    }
+    free(t);
+    plan_free(&p1);
+    plan_free(&p2);
+}
+
 /* Brute-force long-double DFT of one sequence (the pin for orc_fft_many). */
 void orc_dft_naive(const double *in, double *out, long n, int sign)
 {

@@ -186,6 +186,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
         a.fs_tw = tw.data() + pp.fs_tw_off;
         a.fs_tw_entries = (int)((long long)p.tw.size() - pp.fs_tw_off);
         a.fs_cols = pp.fs_cols;
+        a.fs_lo_bits = pp.fs_lo_bits;
       }
       fill_bfp(a, p, (int)pi, t0, nt, false, in_exp, bufs);
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;

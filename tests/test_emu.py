@@ -74,6 +74,51 @@ def test_four_step(n, batch, ws, fi, fo):
     check([n], batch, fi, fo, ws_limit=ws)
 
 
+@pytest.mark.parametrize("n,batch,fi,fo,scaling,inverse", [
+    (1 << 15, 2, emu.FMT_F16, emu.FMT_SPLIT, 1, False),
+    (1 << 15, 1, emu.FMT_F16, emu.FMT_SPLIT, 3, True),
+    (1 << 17, 2, emu.FMT_F16, emu.FMT_F16, 3, False),
+    (1 << 17, 1, emu.FMT_F16, emu.FMT_F16, 2, True),
+    (1 << 18, 1, emu.FMT_BF16, emu.FMT_BF16, 3, False)])
+def test_three_pass(monkeypatch, n, batch, fi, fo, scaling, inverse):
+    """The three-pass 1D plan (COLF -> COLM -> ROWT, the plan of 2^24..2^27),
+    forced down to 2^15..2^17 so that it runs here."""
+    monkeypatch.setenv("MCFFT_THREE_PASS_MIN_LG", "15")
+    d = emu.describe([n], batch, fi, fo, scaling)
+    assert d.count("pass ") == 3 and "_4 " in d.splitlines()[2], d
+    check([n], batch, fi, fo, inverse=inverse, scaling=scaling, seed=n + batch)
+
+
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_three_pass_reconcile(monkeypatch, fi, fo):
+    """AUTO, three passes: first-pass tiles (32 consecutive columns m of the
+    N1 x M view) with input maxima 2^4..2^8 apart; the middle pass maps each of
+    its elements (column n3, row n2) back to first-pass column n2*N3 + n3."""
+    monkeypatch.setenv("MCFFT_THREE_PASS_MIN_LG", "15")
+    n, batch = 1 << 17, 2
+    lg = 17
+    lg3 = lg // 3
+    lg1 = (lg - lg3 + 1) // 2
+    m_cols = n >> lg1
+    rng = np.random.default_rng(23)
+    scale = np.exp2(4.0 * (((np.arange(n) % m_cols) // 32) % 3))
+    x = (rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)) * np.tile(scale, batch)
+    y, e, ref = run_auto(x, [n], batch, fi, fo)
+    assert oracle.rel_l2(y, ref) <= TOL[fo if fo == emu.FMT_SPLIT else fi]
+
+
+def test_three_pass_limits():
+    """Plans: 2^24..2^27 three passes (split four-step table in two factors),
+    2^23 two passes, 2^28 refused."""
+    for lg in (24, 25, 26, 27):
+        for fi, fo in FMTS:
+            d = emu.describe([1 << lg], 1, fi, fo, 3)
+            assert d.count("pass ") == 3, (lg, d)
+    assert emu.describe([1 << 23], 1, emu.FMT_F16, emu.FMT_F16, 3).count("pass ") == 2
+    with pytest.raises(RuntimeError):
+        emu.describe([1 << 28], 1, emu.FMT_F16, emu.FMT_F16, 3)
+
+
 @pytest.mark.parametrize("dims,batch", [((64, 32), 2), ((16, 256), 1), ((256, 64), 1), ((16, 16, 32), 1),
                                         ((32, 64, 32), 1), ((512, 64), 1), ((512, 16, 32), 1)])
 def test_multi_dim(dims, batch):

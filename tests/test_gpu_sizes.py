@@ -1,5 +1,6 @@
-"""GPU: every supported 1D length (16 .. 2^23) in each format, and the NONE and
-FULL scaling modes (plans and ragged batches), against the FP64 oracle.
+"""GPU: every supported 1D length (16 .. 2^27) in each format, and the NONE and
+FULL scaling modes (plans and ragged batches), against the FP64 oracle; a
+forward + inverse round trip at 2^27 checked in FP64 on the GPU.
 
 Tolerances as in the parity tests: rel-L2 fp16 1.5e-3, bf16 1.2e-2, split 1e-6.
 Scaling: NONE stores the unnormalised result (exponent 0), FULL stores it
@@ -21,6 +22,19 @@ def _fmts(fmt):
     return ("fp16", "split") if fmt == "split" else (fmt, fmt)
 
 
+_REF = {}
+
+
+def _reference(x, in_fmt, dims, key):
+    """FP64 oracle FFT of the quantised input; the last two kept (the 2^24..2^27
+    cases of FP16 and split share one)."""
+    if key not in _REF:
+        if len(_REF) >= 2:
+            _REF.pop(next(iter(_REF)))
+        _REF[key] = oracle.fft(mc.decode(mc.encode(x, in_fmt), in_fmt), tuple(dims))
+    return _REF[key]
+
+
 def _run(torch, dims, batch, fmt, scaling, seed):
     in_fmt, out_fmt = _fmts(fmt)
     n = int(np.prod(dims))
@@ -34,7 +48,8 @@ def _run(torch, dims, batch, fmt, scaling, seed):
     e = p.exponent
     y = p.result(d_out)
     p.destroy()
-    ref = oracle.fft(mc.decode(mc.encode(x, in_fmt), in_fmt), tuple(dims))
+    del d_in, d_out
+    ref = _reference(x, in_fmt, dims, (tuple(dims), batch, "bf16" if in_fmt == "bf16" else "fp16", seed))
     return oracle.rel_l2(y, ref), e
 
 
@@ -49,6 +64,50 @@ def test_every_1d_length(torch_cuda, lg, fmt, scaling):
     err, _ = _run(torch_cuda, [n], batch, fmt, scaling, seed=lg)
     print(f"  rel_l2 {err:.3e}", flush=True)
     assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("lg", [24, 25, 26, 27])
+@pytest.mark.parametrize("fmt", ["fp16", "split", "bf16"])
+def test_every_1d_length_three_pass(torch_cuda, lg, fmt):
+    """2^24 .. 2^27 (three-pass plans), batch 1, default scaling; 2^24 also
+    with static BALANCED scaling."""
+    n = 1 << lg
+    for scaling in (["auto", "balanced"] if lg == 24 else ["auto"]):
+        print(f"1d n=2^{lg} b=1 {fmt} {scaling}", flush=True)
+        err, _ = _run(torch_cuda, [n], 1, fmt, scaling, seed=lg)
+        print(f"  rel_l2 {err:.3e}", flush=True)
+        assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "split"])
+def test_roundtrip_2_27(torch_cuda, fmt):
+    """N = 2^27 x 2: forward then inverse (the inverse takes the forward's
+    exponents), inv(fwd(x)) = N x checked per transform in FP64 on the GPU;
+    the second transform runs in place."""
+    torch = torch_cuda
+    n, batch = 1 << 27, 2
+    in_fmt, out_fmt = _fmts(fmt)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(27)
+    x = torch.randn((n * batch, 2), dtype=torch.float16, device="cuda", generator=g)
+    ref = x.double().reshape(-1)
+    pf = mc.Plan([n], batch, in_fmt, out_fmt)
+    pi = mc.Plan([n], batch, out_fmt, out_fmt)
+    mid, out = mc.empty(n * batch, out_fmt), mc.empty(n * batch, out_fmt)
+    pf.exec(x, mid, mc.FORWARD)
+    pi.exec(mid, out, mc.INVERSE, in_exp=pf)
+    e = pi.exponents()
+    assert pf.status_flags() == 0 and pi.status_flags() == 0
+    for b in range(batch):
+        rel, _ = mc.error_norms(out[b * n:(b + 1) * n], out_fmt, int(e[b]) - 27, ref[2 * b * n:2 * (b + 1) * n])
+        print(f"2^27 round trip {fmt} transform {b}: rel_l2 {rel:.3e}", flush=True)
+        assert rel <= (2e-6 if fmt == "split" else 3e-3)
+    # in place (input = output buffer) gives the same values
+    pi.exec(mid, mid, mc.INVERSE, in_exp=pf)
+    torch.cuda.synchronize()
+    assert torch.equal(mid.view(torch.int16), out.view(torch.int16))
+    pf.destroy()
+    pi.destroy()
 
 
 @pytest.mark.parametrize("dims,batch", [((256,), 64), ((4096,), 8), ((131072,), 2), ((1 << 20,), 1),

@@ -106,6 +106,20 @@ def test_nd_separable_and_numpy(dims):
     assert oracle.rel_l2(y, np.fft.fftn(x, axes=tuple(range(1, len(dims) + 1)))) < 1e-14
 
 
+@pytest.mark.parametrize("lg,batch", [(2, 3), (9, 2), (15, 1), (20, 2)])
+def test_fft_1d_large_matches_stockham(lg, batch):
+    """The four-step path (single long transforms, parallel inside one
+    transform) equals the pinned Stockham path, both directions."""
+    n = 1 << lg
+    rng = np.random.default_rng(lg)
+    x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
+    for sign in (-1, 1):
+        a = oracle.fft_large(x, n, sign)
+        b = x.copy()
+        oracle.lib().orc_fft_many(oracle._dptr(b.view(np.float64)), 1, (oracle.ctypes.c_long * 1)(n), batch, sign)
+        assert oracle.rel_l2(a, b) < 1e-14, sign
+
+
 def test_reference_metric_definition():
     s = np.array([1 + 1j, 2 - 2j])
     t = np.array([1.5 + 1j, 2 - 10j])
