@@ -69,9 +69,9 @@ def test_argument_errors_are_status_codes():
     assert L.mcfft_plan_many(ctypes.byref(h), 1, dims, 1, 2, 0, 1) == 2   # split -> fp16
     bad = (ctypes.c_longlong * 1)(1000)
     assert L.mcfft_plan_many(ctypes.byref(h), 1, bad, 1, 0, 0, 1) == 2   # not a power of two
-    # outside the documented sizes: 1D below 16 or above 2^23, an outer side above
+    # outside the documented sizes: 1D below 16 or above 2^27, an outer side above
     # 4096, fewer columns on a 16-point outer axis than the 32 of its column tile
-    for shape in [(8,), (1 << 24,), (8192, 64), (16, 16), (16, 16, 16)]:
+    for shape in [(8,), (1 << 28,), (8192, 64), (16, 16), (16, 16, 16)]:
         d = (ctypes.c_longlong * len(shape))(*shape)
         assert L.mcfft_plan_many(ctypes.byref(h), len(shape), d, 1, 0, 0, 1) == 2, shape
     assert L.mcfft_exec(None, None, None, -1, None) == 6                  # invalid plan
