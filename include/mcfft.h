@@ -217,15 +217,27 @@ mcfftStatus mcfft_get_transform_exponents(mcfftPlan plan, int* exponents);
  * as another exec's in_exp.  Regular plans only. */
 mcfftStatus mcfft_get_exponent_buffer(mcfftPlan plan, const int** exponents);
 
-/* Autotuner: times every compiled kernel variant (tile size, waves per
- * workgroup) of each pass of `plan` on the given device buffers, `reps` launches
- * each, keeps the fastest and records the choice in the tuning cache, keyed by
- * GPU architecture and (rank, dims, batch, formats, scaling).  Blocking; the
- * contents of `out` (and of `in` for an in-place plan) are overwritten.  Plans
- * created later read the cache, so a shape is tuned once per machine.  Cache
- * file: $MCFFT_TUNE_CACHE, default $HOME/.cache/mcfft/tune-<arch>.txt;
- * MCFFT_TUNE_CACHE=off disables it.  Not for ragged plans. */
+/* Autotuner: for each structure the plan's shape can take -- the radix that
+ * splits a 2D plan's outer axis (none, 2, 4, 8), the N1 x N2 split of a two-pass
+ * 1D plan (the default and N2 halved or doubled) -- times every compiled kernel
+ * variant (tile size, waves per workgroup) of each pass on the given device
+ * buffers, `reps` launches each, keeps the fastest per pass, then times whole
+ * execs and keeps the fastest structure.  The choice (structure and kernels) is
+ * recorded in the tuning cache, keyed by GPU architecture and (rank, dims,
+ * batch, formats, scaling); plans created later read it, so a shape is tuned
+ * once per machine.  The cache file names the library build that wrote it
+ * (mcfft_build_id): one written by another build is ignored.  Blocking; the
+ * contents of `out` (and of `in` for an in-place plan) are overwritten, and the
+ * plan's device buffers (workspace, exponent buffer) may be replaced, so query
+ * mcfft_get_exponent_buffer after tuning.  Cache file: $MCFFT_TUNE_CACHE, default
+ * $HOME/.cache/mcfft/tune-<arch>.txt; MCFFT_TUNE_CACHE=off disables it.  Not for
+ * ragged or real plans; a layout plan keeps its structure.  No reference
+ * counterpart (its size switch, source/mcfft_1d_device.cpp:695-775, is fixed). */
 mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps);
+
+/* Identifier of the kernel sources this library was built from (a hash), as
+ * written into the tuning cache. */
+mcfftStatus mcfft_build_id(const char** id);
 
 /* Error norms on the GPU in FP64: rel-L2 = ||y - ref||_2 / ||ref||_2 and
  * max-abs = max |y - ref| over re and im, where y is `n` elements stored in

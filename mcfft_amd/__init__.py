@@ -95,16 +95,26 @@ def lib() -> ctypes.CDLL:
         L.mcfft_error_norms.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), vp]
         L.mcfft_error_norms_noalloc.argtypes = L.mcfft_error_norms.argtypes
+        L.mcfft_build_id.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
         for f in ("mcfft_plan_many", "mcfft_plan_1d", "mcfft_plan_2d", "mcfft_plan_3d", "mcfft_exec",
                   "mcfft_exec_exp", "mcfft_get_transform_exponents", "mcfft_get_exponent_buffer",
                   "mcfft_exec_passes", "mcfft_exec_timed", "mcfft_get_pass_count", "mcfft_plan_ragged", "mcfft_get_item_exponents",
                   "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_error_norms_noalloc",
+                  "mcfft_build_id",
                   "mcfft_plan_r2c_1d",
                   "mcfft_plan_c2r_1d", "mcfft_plan_many_layout", "mcfft_exec_layout",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
     return _lib
+
+
+def build_id() -> str:
+    """Identifier of the kernel sources libmcfft.so was built from (the tuning
+    cache records it and ignores caches of other builds)."""
+    v = ctypes.c_char_p()
+    _check(lib().mcfft_build_id(ctypes.byref(v)), "mcfft_build_id")
+    return v.value.decode()
 
 
 def _check(st: int, what: str):
@@ -240,8 +250,10 @@ class Plan:
         self.exec_ptr(inp.data_ptr(), out.data_ptr(), direction, stream, ptr)
 
     def tune(self, inp, out, direction: int = FORWARD, reps: int = 5, stream=None):
-        """Time every compiled kernel variant of each pass on these buffers, keep
-        the fastest and record it in the tuning cache (blocking; overwrites out)."""
+        """Autotune on these buffers: for each plan structure (2D outer-axis radix,
+        1D four-step split) every compiled kernel variant of each pass, then whole
+        execs; keep the fastest and record it in the tuning cache (blocking;
+        overwrites out, may replace the plan's device buffers)."""
         import torch
         self._check_io(inp, out, contiguous=True)
         if stream is None:

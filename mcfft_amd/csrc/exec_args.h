@@ -1,9 +1,11 @@
-// exec_args.h -- the block-floating-point and bookkeeping part of a pass
-// launch's arguments (bfp.h), shared by the runtime and the CPU emulation build
-// so both launch the kernels identically.
+// exec_args.h -- a pass launch's arguments (PassArgs) from its plan, including
+// the block-floating-point and bookkeeping part (bfp.h), shared by the runtime
+// and the CPU emulation build so both launch the kernels identically.
 #pragma once
 #include "fft_pass.h"
 #include "plan_core.h"
+
+#include <cstring>
 
 namespace mcfft {
 
@@ -56,6 +58,44 @@ inline void fill_bfp(PassArgs& a, const PlanCore& c, int pi, long long t0, long 
   a.bfp_tile_e = b.tile_e;
   a.bfp_out_exp = b.out_exp;
   a.bfp_in_exp = in_exp;
+}
+
+// Everything of pass pp's launch arguments but the bfp_* fields and n_tiles:
+// `in`/`out` already point at the chunk's data, in_elems/out_elems are the
+// element counts addressable from them, `tw` is the plan's twiddle table for
+// the direction and `aops` its DFT-matrix operands.
+inline void fill_pass_args(PassArgs& a, const PlanCore& c, const PassPlan& pp, const void* in, void* out,
+                           long long in_elems, long long out_elems, int inverse, const f32x2* tw,
+                           const u32x4* aops) {
+  memset(&a, 0, sizeof(a));
+  a.in = in;
+  a.out = out;
+  a.stride_in = pp.stride_in;
+  a.stride_out = pp.stride_out;
+  a.in_sg = pp.in_sg;
+  a.in_st = pp.in_st;
+  a.out_sg = pp.out_sg;
+  a.out_st = pp.out_st;
+  a.n_inst = (int)pp.n_inst;
+  a.tiles_per_group = pp.tiles_per_group;
+  a.in_fmt = pp.src_fmt;
+  a.in_elems = in_elems;
+  a.out_elems = out_elems;
+  a.stage_tw_entries = (int)((long long)c.tw.size() - pp.stage_tw_off);
+  a.inverse = inverse;
+  a.out_scale = pp.out_scale;
+  for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
+  a.stage_tw = tw + pp.stage_tw_off;
+  a.aops = aops;
+  if (pp.fourstep) {
+    a.fs_tw = tw + pp.fs_tw_off;
+    a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);
+    a.fs_cols = pp.fs_cols;
+    a.fs_lo_bits = pp.fs_lo_bits;
+  } else if (pp.x_tw_off >= 0) {  // split ROWX: its cross-row twiddles
+    a.fs_tw = tw + pp.x_tw_off;
+    a.fs_tw_entries = (int)((long long)c.tw.size() - pp.x_tw_off);
+  }
 }
 
 // sizes of the bookkeeping buffers of plan c: slots (64-bit) and tile

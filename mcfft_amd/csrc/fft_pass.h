@@ -50,13 +50,23 @@ constexpr float kSplitLoInv = 1.0f / 2048.0f;
 // evaluate them in the kernel like FP16/BF16 passes (false).
 constexpr bool kSplitTableTwiddles = true;
 
-enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4 };
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4, KIND_ROWX = 5 };
 
 // Element (q, m) of a tile (instance q, position m) is at, in elements,
 //   ROW : in/out  base + q*L + m                  (contiguous rows)
 //   ROWT: in      base + q*L + m,  out  base + q + m*stride_out  (transposed store)
 //   COL, COLF, COLM: in base + q + m*stride_in, out base + q + m*stride_out (columns)
+//   ROWX: in  base + q*stride_in + m,  out  base + q*stride_out + m  (rows, instance strides)
 // with base = group*sg + tile*st for the tile's group/tile indices.
+//
+// ROWX is the row pass of a 2D plan whose outer axis (NX points) is split as
+// NX = R * (NX/R): its tile holds the R rows r1 + (NX/R)*r2 (r2 < R) of tile
+// r1, and before the row FFT it takes the R-point DFT across them (the outer
+// axis's first factor) times W_NX^(r1*k2), storing result k2 as row R*r1 + k2.
+// The column pass then has NX/R-point columns at stride R rows.  Q = R and the
+// waves cover exactly one row per MFMA tile set (NW = L/512), so a thread's R
+// MFMA tiles are the R rows at the same positions and the butterfly needs no
+// exchange.
 struct PassArgs {
   const void* in;
   void* out;
@@ -71,12 +81,14 @@ struct PassArgs {
   const f32x2* stage_tw;      // post-twiddle tables [stage][j][16] (see plan_core)
   const u32x4* aops;          // DFT-matrix MFMA operands [64 lanes][4]: A1, A2, A1lo, A2lo
   const f32x2* fs_tw;         // four-step twiddles W_N^(m*column), N1 x fs_cols entries, or (fs_lo_bits > 0)
+                              // (split ROWX: W_NX^(r1*k2) at r1*R + k2, unscaled)
                               // W_N^(m*(column % 2^b)) [N1][2^b] then W_N^(m*(column - column % 2^b)) [N1][fs_cols >> b]
   int fs_cols;                // N2 (columns of the four-step decomposition)
   int fs_lo_bits;             // b: split four-step table factorised in two (0: one table)
   long long in_elems, out_elems;  // elements addressable from in / out (bounds of the buffer views)
   int stage_tw_entries, fs_tw_entries;
-  float st_scale[6];          // 2^-shift of stage P (twiddles computed in the kernel carry it)
+  float st_scale[6];          // 2^-shift of stage P (twiddles computed in the kernel carry it);
+                              // ROWX: [NSTAGE] is its cross-row butterfly's
   // block floating point and exec bookkeeping (bfp.h); zero flags: static
   // scaling, nothing recorded
   int bfp_flags;
@@ -320,10 +332,11 @@ template <int E> struct For<E, E> {
 template <int S, int T, int Q, int NW, int CM, int KIND>
 struct PassCfg {
   static constexpr int L = ipow16(S) * T;
-  static constexpr int Q_ = Q, NW_ = NW, CM_ = CM;
+  static constexpr int Q_ = Q, NW_ = NW, CM_ = CM, KIND_ = KIND;
   static constexpr int NT = NW * 64;
   static constexpr int NSTAGE = S + (T > 1 ? 1 : 0);
   static constexpr bool QFAST = (KIND == KIND_COL || KIND == KIND_COLF || KIND == KIND_COLM);
+  static constexpr bool XROWS = (KIND == KIND_ROWX);
   static constexpr bool STAGED = (KIND == KIND_ROWT);
   static constexpr bool FOURSTEP = (KIND == KIND_COLF || KIND == KIND_COLM);
   static constexpr int ROW = L + L / 16 + 1;  // padded LDS row (contiguous-row layout)
@@ -345,6 +358,8 @@ struct PassCfg {
   static_assert(S >= 1, "at least one radix-16 stage");
   static_assert((Q * L) % 512 == 0, "tile must hold a multiple of 32 sixteen-point columns");
   static_assert(T == 1 || T == 2 || T == 4 || T == 8, "tail radix");
+  // ROWX: wave w's MFMA tile k is instance k at columns 32w..32w+31 (one row per NW tiles)
+  static_assert(!XROWS || ((Q == 2 || Q == 4 || Q == 8) && NW * 512 == L), "ROWX tile shape");
 
   static MCFFT_DEV int lidx(int q, int m) {
     if constexpr (QFAST) return m * Q + q;
@@ -358,10 +373,12 @@ struct PassCfg {
   }
   static MCFFT_DEV int in_off(int q, int m, int stride) {
     if constexpr (QFAST) return q + m * stride;
+    else if constexpr (XROWS) return q * stride + m;
     else return q * L + m;
   }
   static MCFFT_DEV int out_off(int q, int m, int stride) {
     if constexpr (KIND == KIND_ROW) return q * L + m;
+    else if constexpr (XROWS) return q * stride + m;
     else return q + m * stride;
   }
 };
@@ -468,12 +485,66 @@ MCFFT_DEV void rescale_stage0(const PassArgs& a, int wave, int lane, int grp, in
   }
 }
 
+// ROWX, after the stage-0 loads: the radix-Q DFT across the tile's rows (the
+// outer axis's first factor, in FP32), result k2 times W_NX^(r1*k2), the
+// butterfly's stage scale and the AUTO normalisation `nsc`, rounded once.  The
+// thread's MFMA tile k holds row k at the same positions (PassCfg::XROWS), so
+// the butterfly stays in registers.  Row r1 = the tile's index in its group;
+// NX = Q * tiles_per_group.
+template <class C, class Tr>
+MCFFT_DEV void cross_rows(const PassArgs& a, int r1, bool inv, float nsc, typename C::El (*x)[8]) {
+  constexpr int R = C::Q_;
+  const float sc = a.st_scale[C::NSTAGE] * nsc;
+  f32x2 w[R];
+  if constexpr (C::CM_ == CM_SPLIT && kSplitTableTwiddles) {
+    const GBuf g = make_gbuf(a.fs_tw, (unsigned)a.fs_tw_entries * 8u);
+    for (int k = 0; k < R; ++k) w[k] = __builtin_bit_cast(f32x2, buf_ld64(g, (r1 * R + k) * 8)) * sc;
+  } else {
+    const float inv_nx = 1.0f / (float)(R * a.tiles_per_group);
+    for (int k = 0; k < R; ++k) w[k] = twiddle_rev(r1 * k, inv_nx, inv, sc);
+  }
+  for (int t = 0; t < 8; ++t) {
+    f32x2 v[R];
+    for (int k = 0; k < R; ++k) v[k] = Tr::unpack(x[k][t]);
+    dft_small<R>(v, inv);
+    for (int k = 0; k < R; ++k) x[k][t] = Tr::pack(cmul(v[k], w[k]));
+  }
+}
+
+// The input view of `tile`: its group/tile base and, for ROW tiles (contiguous
+// instances), an end at the last valid instance, so that loads beyond read 0.
+// Column kinds always run full tiles (the planner guarantees it).
+template <class C>
+MCFFT_DEV GBuf tile_in_view(const PassArgs& a, int tile, int esz_in) {
+  const int grp = uniform(tile / a.tiles_per_group), til = uniform(tile - grp * a.tiles_per_group);
+  const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
+  long long in_end = a.in_elems;
+  if constexpr (C::KIND_ == KIND_ROW) {
+    const int inst0 = til * C::Q_;
+    const long long vq = (long long)(a.n_inst - inst0 < C::Q_ ? a.n_inst - inst0 : C::Q_);
+    in_end = ib + vq * C::L < in_end ? ib + vq * C::L : in_end;
+  }
+  return clamp_view(a.in, ib, in_end, esz_in);
+}
+
+// Stage-0 inputs of `tile` (what fft_tile loads first), issued ahead of the
+// tile so that they arrive while the workgroup computes the one before it.
+template <int S, int T, int Q, int NW, int CM, int KIND>
+MCFFT_DEV void tile_prefetch(const PassArgs& a, int tile, typename Traits<CM>::T (*x)[8]) {
+  typedef PassCfg<S, T, Q, NW, CM, KIND> C;
+  const int tid = opaque(thread_id());
+  const int esz_in = (CM == CM_SPLIT && a.in_fmt == FMT_SPLIT) ? 8 : 4;
+  load_stage0<C, Traits<CM>>(a, tile_in_view<C>(a, tile, esz_in), tid >> 6, tid & 63, x);
+}
+
 // One tile of a pass: stages on the tile's Q instances, ending with a barrier
 // (LDS is reused by the workgroup's next tile).  BFP: the tile takes part in the
 // block floating point and exec bookkeeping of bfp.h (off in the ragged kernel,
 // whose passes are statically scaled).
+// `pre`: the tile's stage-0 inputs, already loaded (tile_prefetch); null: load them here.
 template <int S, int T, int Q, int NW, int CM, int KIND, bool BFP = true>
-MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sacc = nullptr) {
+MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sacc = nullptr,
+                        const typename Traits<CM>::T (*pre)[8] = nullptr) {
   typedef PassCfg<S, T, Q, NW, CM, KIND> C;
   typedef Traits<CM> Tr;
   typedef typename C::El El;
@@ -495,25 +566,21 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
   {
     // wave-uniform (scalar registers): kept to the end of the tile by the bookkeeping
     const int grp = uniform(tile / a.tiles_per_group), til = uniform(tile - grp * a.tiles_per_group);
-    const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
     const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
     const int inst0 = til * Q;
-    // Column kinds always run full tiles (the planner guarantees it).  A ROW
-    // tile's instances are contiguous, so its buffer views end at the last valid
-    // instance: loads beyond read 0 and stores beyond are dropped.
-    long long in_end = a.in_elems, out_end = a.out_elems;
+    // A ROW tile's instances are contiguous, so its buffer views end at the last
+    // valid instance: loads beyond read 0 and stores beyond are dropped.
+    long long out_end = a.out_elems;
     if constexpr (KIND == KIND_ROW) {
       const long long vq = (long long)(a.n_inst - inst0 < Q ? a.n_inst - inst0 : Q);
-      in_end = ib + vq * L < in_end ? ib + vq * L : in_end;
       out_end = ob + vq * L < out_end ? ob + vq * L : out_end;
     }
-    const GBuf gin = clamp_view(a.in, ib, in_end, esz_in);
     const GBuf gout = clamp_view(a.out, ob, out_end, esz_out);
 
     // ---- block floating point (bfp.h) ----
     // Row and four-step column passes are always a plan's first pass; column and
     // transposed-row passes never are (so only they can need reconciling).
-    constexpr bool FIRST_KIND = (KIND == KIND_ROW || KIND == KIND_COLF);
+    constexpr bool FIRST_KIND = (KIND == KIND_ROW || KIND == KIND_COLF || KIND == KIND_ROWX);
     const int bf = BFP ? a.bfp_flags : 0;
     const int t_loc = (BFP && a.bfp_gpt > 0) ? uniform(grp / a.bfp_gpt) : 0;  // transform of the tile in the launch
     float nsc = 1.0f;        // AUTO first pass: 2^n applied to the stage-0 results
@@ -600,7 +667,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
       };
       El x[TPW][8];
       if constexpr (P == 0) {
-        load_stage0<C, Tr>(a, gin, wave, lane, x);
+        if (pre) {
+          for (int k = 0; k < TPW; ++k)
+            for (int t = 0; t < 8; ++t) x[k][t] = pre[k][t];
+        } else {
+          load_stage0<C, Tr>(a, tile_in_view<C>(a, tile, esz_in), wave, lane, x);
+        }
         if (BFP && FIRST_KIND && (bf & BFP_FIRST)) {
           // AUTO first pass: the tile's largest |component| sets its normalisation
           uint32_t acc = 0;
@@ -611,6 +683,11 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
                                *sacc);
         } else if (BFP && !FIRST_KIND && rescale) {
           rescale_stage0<C, Tr>(a, wave, lane, grp, inst0, e_out - a.bfp_e_base, x);
+        }
+        if constexpr (C::XROWS) {
+          // the normalisation rides on the cross-row butterfly instead of stage 0
+          cross_rows<C, Tr>(a, til, inv, nsc, x);
+          nsc = 1.0f;
         }
       } else {
         for (int k = 0; k < TPW; ++k) {
@@ -833,7 +910,27 @@ MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   // the last pass of an AUTO multi-pass exec advances the epoch (nothing in it
   // reads the epoch; the next first pass is a later launch)
   if ((a.bfp_flags & BFP_EPOCH) && block_id() == 0 && thread_id() == 0) atomic_add_dev64(&a.bfp_state->epoch, 1);
-  for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A, &sacc);
+  // Row kinds in FP16/BF16: a software pipeline over the workgroup's tiles, the
+  // next tile's stage-0 loads issued before the current tile is computed (a
+  // workgroup barrier waits for LDS only, so they stay in flight through the
+  // tile's stages).  Measured on MI355X: 3-10 % faster row passes; column and
+  // split passes ran 4-90 % slower with it (the extra registers cost occupancy).
+  constexpr bool PREFETCH = (KIND == KIND_ROW || KIND == KIND_ROWX) && CM != CM_SPLIT;
+  if constexpr (PREFETCH) {
+    typedef typename Traits<CM>::T El;
+    constexpr int TPW = PassCfg<S, T, Q, NW, CM, KIND>::TPW;
+    El nxt[TPW][8];
+    if (b < a.n_tiles) tile_prefetch<S, T, Q, NW, CM, KIND>(a, b, nxt);
+    for (int tile = b; tile < a.n_tiles; tile += g) {
+      El cur[TPW][8];
+      for (int k = 0; k < TPW; ++k)
+        for (int t = 0; t < 8; ++t) cur[k][t] = nxt[k][t];
+      if (tile + g < a.n_tiles) tile_prefetch<S, T, Q, NW, CM, KIND>(a, tile + g, nxt);
+      fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A, &sacc, cur);
+    }
+  } else {
+    for (int tile = b; tile < a.n_tiles; tile += g) fft_tile<S, T, Q, NW, CM, KIND>(a, tile, A, &sacc);
+  }
   slot_flush(sacc, a.bfp_slots);
 }
 

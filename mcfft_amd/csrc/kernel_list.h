@@ -1,13 +1,15 @@
 // kernel_list.h -- the compiled pass configurations (X-macro lists).
 //
 // X(S, T, Q, NW, CM, KIND): L = 16^S * T points, Q instances per workgroup,
-// NW waves, compute mode CM, KIND 0 ROW / 1 ROWT / 2 COL / 3 COLF / 4 COLM.  Tile sizes keep one workgroup's LDS at <= 128 KiB
+// NW waves, compute mode CM, KIND 0 ROW / 1 ROWT / 2 COL / 3 COLF / 4 COLM / 5 ROWX.  Tile sizes keep one workgroup's LDS at <= 128 KiB
 // (gfx950 has 160 KiB per CU) and a tile at >= 32 sixteen-point MFMA columns.
 //   ROW   contiguous rows (1D single pass, 2D/3D innermost axis)
 //   ROWT  contiguous rows, transposed store (last pass of a four-step 1D plan)
 //   COL   strided columns (2D/3D outer axes)
 //   COLF  strided columns + four-step twiddle (first pass of a four-step plan)
 //   COLM  strided columns + four-step twiddle (middle pass of a three-pass 1D plan)
+//   ROWX  Q rows NX/Q apart + radix-Q butterfly across them (row pass of a 2D plan
+//         with a split outer axis of NX points)
 #pragma once
 
 // ---- ROW: tile of 2048 elements (4096 at L = 4096) ----
@@ -34,6 +36,13 @@
   X(3, 1, 1, 4, CM, 0)            \
   X(3, 1, 2, 8, CM, 0)
 
+// four MFMA tiles per wave (NW = Q*L/2048): a tile's fixed costs (barriers,
+// load latency) spread over four times the data; 512..2048-point rows
+#define MCFFT_ROW_WIDE_CM(X, CM) \
+  X(2, 2, 8, 2, CM, 0)           \
+  X(2, 4, 4, 2, CM, 0)           \
+  X(2, 8, 4, 4, CM, 0)
+
 // split-mode alternatives for short rows (latency-bound small batches)
 #define MCFFT_ROW_TUNE_SPLIT(X) \
   X(2, 1, 4, 2, 2, 0)           \
@@ -51,6 +60,9 @@
   X(3, 8, 1, 16, 1, 0)      \
   MCFFT_ROW_TUNE_CM(X, 0)   \
   MCFFT_ROW_TUNE_CM(X, 1)   \
+  MCFFT_ROW_WIDE_CM(X, 0)   \
+  MCFFT_ROW_WIDE_CM(X, 1)   \
+  MCFFT_ROW_WIDE_CM(X, 2)   \
   MCFFT_ROW_TUNE_SPLIT(X)
 
 // ---- ROWT: Q rows stored transposed (segments of Q elements) ----
@@ -107,12 +119,21 @@
   X(1, 4, 16, 4, CM, 2)            \
   X(1, 8, 16, 4, CM, 2)
 
+// 128-byte (FP16/BF16: 32 columns) and 256-byte row segments for the column
+// pass of a 2D plan with a split outer axis (1024 and 512 points)
+#define MCFFT_COL_WIDE_CM(X, CM) \
+  X(2, 4, 32, 16, CM, 2)         \
+  X(2, 4, 32, 8, CM, 2)          \
+  X(2, 2, 64, 16, CM, 2)
+
 #define MCFFT_COL_LIST(X)                          \
   MCFFT_COL_LIST_CM(X, 0, 2, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 1, 2, 32, 16, 16, 8)        \
   MCFFT_COL_LIST_CM(X, 2, 2, 16, 16, 8, 4)         \
   MCFFT_COL_TUNE_CM(X, 0, 2)                       \
   MCFFT_COL_TUNE_CM(X, 1, 2)                       \
+  MCFFT_COL_WIDE_CM(X, 0)                          \
+  MCFFT_COL_WIDE_CM(X, 1)                          \
   X(2, 1, 16, 4, 2, 2)                             \
   X(2, 2, 16, 8, 0, 2)                             \
   X(2, 2, 16, 8, 1, 2)                             \
@@ -142,6 +163,28 @@
   MCFFT_COLM_LIST_CM(X, 1, 32)     \
   MCFFT_COLM_LIST_CM(X, 2, 16)
 
+// ---- ROWX: Q = R rows, NW = L / 512 waves (one MFMA tile set per row) ----
+#define MCFFT_ROWX_L_CM(X, S, T, NW, CM) \
+  X(S, T, 2, NW, CM, 5)                  \
+  X(S, T, 4, NW, CM, 5)
+
+#define MCFFT_ROWX_CM(X, CM)          \
+  MCFFT_ROWX_L_CM(X, 2, 2, 1, CM)     \
+  MCFFT_ROWX_L_CM(X, 2, 4, 2, CM)     \
+  MCFFT_ROWX_L_CM(X, 2, 8, 4, CM)     \
+  MCFFT_ROWX_L_CM(X, 3, 1, 8, CM)
+
+// R = 8 where the tile fits the LDS (<= 139 KiB)
+#define MCFFT_ROWX_LIST(X) \
+  MCFFT_ROWX_CM(X, 0)      \
+  MCFFT_ROWX_CM(X, 1)      \
+  MCFFT_ROWX_CM(X, 2)      \
+  X(2, 8, 8, 4, 0, 5)      \
+  X(3, 1, 8, 8, 0, 5)      \
+  X(2, 8, 8, 4, 1, 5)      \
+  X(3, 1, 8, 8, 1, 5)      \
+  X(2, 8, 8, 4, 2, 5)
+
 namespace mcfft {
 struct KernelInfo {
   const char* name;
@@ -161,6 +204,8 @@ extern const KernelInfo kColFKernels[];
 extern const int kColFKernelCount;
 extern const KernelInfo kColMKernels[];
 extern const int kColMKernelCount;
+extern const KernelInfo kRowXKernels[];
+extern const int kRowXKernelCount;
 // ragged batches: pass variants per compute mode and the persistent kernels
 extern const KernelInfo kRaggedVariantInfo[3][30];
 extern const KernelInfo kRaggedKernels[3];

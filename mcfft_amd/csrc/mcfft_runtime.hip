@@ -222,7 +222,13 @@ mcfftStatus set_lds_attr(const KernelInfo* k) {
 }
 
 // ---------------- tuning cache ----------------
-// file format: first line "mcfft-tune-cache 1", then "arch|signature=k0,k1,..."
+// Text format and build check: tune_cache_parse / tune_cache_format (plan_core).
+// MCFFT_BUILD_ID identifies the kernel sources this library was built from
+// (a hash the Makefile computes); a cache written by another build is ignored
+// and replaced on the next tune.
+#ifndef MCFFT_BUILD_ID
+#define MCFFT_BUILD_ID "unversioned"
+#endif
 std::mutex g_cache_mu;
 std::map<std::string, std::string> g_cache;
 bool g_cache_loaded = false;
@@ -242,12 +248,10 @@ void load_cache_locked(const std::string& arch) {
   const std::string path = cache_path(arch);
   if (path.empty()) return;
   std::ifstream f(path);
-  std::string line;
-  if (!f || !std::getline(f, line) || line != "mcfft-tune-cache 1") return;  // absent or other version
-  while (std::getline(f, line)) {
-    const size_t eq = line.find('=');
-    if (eq != std::string::npos) g_cache[line.substr(0, eq)] = line.substr(eq + 1);
-  }
+  if (!f) return;
+  std::stringstream text;
+  text << f.rdbuf();
+  g_cache = tune_cache_parse(text.str(), MCFFT_BUILD_ID);
 }
 
 bool save_cache_locked(const std::string& arch) {
@@ -265,8 +269,7 @@ bool save_cache_locked(const std::string& arch) {
   }
   std::ofstream f(path + ".tmp");
   if (!f) return false;
-  f << "mcfft-tune-cache 1\n";
-  for (const auto& kv : g_cache) f << kv.first << "=" << kv.second << "\n";
+  f << tune_cache_format(g_cache, MCFFT_BUILD_ID);
   f.close();
   return rename((path + ".tmp").c_str(), path.c_str()) == 0;
 }
@@ -279,17 +282,17 @@ std::string arch_of(int device) {
   return c == std::string::npos ? a : a.substr(0, c);
 }
 
-// apply a cached kernel choice to a freshly built plan (silently keeps the
-// defaults when an entry no longer matches the compiled kernels)
-void apply_cached(mcfftPlan p) {
+// the cached tuning choice for a plan of this signature, if any
+bool cached_choice(const std::string& arch, const PlanCore& core, TuneChoice& c) {
   std::lock_guard<std::mutex> g(g_cache_mu);
-  load_cache_locked(p->arch);
-  const auto it = g_cache.find(p->arch + "|" + plan_signature(p->core));
-  if (it == g_cache.end()) return;
-  std::vector<std::string> names;
-  std::stringstream ss(it->second);
-  std::string n;
-  while (std::getline(ss, n, ',')) names.push_back(n);
+  load_cache_locked(arch);
+  const auto it = g_cache.find(arch + "|" + plan_signature(core));
+  return it != g_cache.end() && parse_tune_choice(it->second, c);
+}
+
+// run pass i on kernel names[i] where that kernel can run it (silently keeps
+// the default where a name no longer matches the compiled kernels)
+void apply_kernels(mcfftPlan p, const std::vector<std::string>& names) {
   if (names.size() != p->core.passes.size()) return;
   for (size_t i = 0; i < names.size(); ++i) {
     PassPlan& pp = p->core.passes[i];
@@ -387,35 +390,38 @@ mcfftStatus alloc_exec_state(mcfftPlan p) {
   return MCFFT_SUCCESS;
 }
 
-}  // namespace
-
-extern "C" {
-
-mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
-                            mcfftFormat out_fmt, mcfftScaling scaling) {
-  if (!plan) return MCFFT_INVALID_VALUE;
-  *plan = nullptr;
+// A regular plan: structure `opts` (null: the tuning cache's choice for this
+// signature, else the planner's defaults), the cache's kernel choice when its
+// structure matches, device tables, workspace and exec bookkeeping.
+mcfftStatus create_plan(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                        mcfftFormat out_fmt, mcfftScaling scaling, const PlanOpts* opts) {
   mcfftPlan p = new (std::nothrow) mcfftPlan_st();
   if (!p) return MCFFT_ALLOC_FAILED;
   // MCFFT_WS_BYTES overrides the four-step workspace bound (bytes)
   long long ws_limit = 0;
   if (const char* env = getenv("MCFFT_WS_BYTES")) ws_limit = atoll(env);
-  int st = plan_build(p->core, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, ws_limit);
+  int st = plan_build(p->core, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, ws_limit,
+                      opts ? *opts : PlanOpts());
+  if (st == ST_OK && hipGetDevice(&p->device) != hipSuccess) st = ST_HIP_ERROR;
+  int n_cu = 0;
+  if (st == ST_OK &&
+      (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || n_cu < 1))
+    st = ST_HIP_ERROR;
   if (st != ST_OK) {
     delete p;
     return (mcfftStatus)st;
   }
-  if (hipGetDevice(&p->device) != hipSuccess) {
-    delete p;
-    return MCFFT_HIP_ERROR;
-  }
-  int n_cu = 0;
-  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || n_cu < 1) {
-    delete p;
-    return MCFFT_HIP_ERROR;
-  }
   p->arch = arch_of(p->device);
-  apply_cached(p);
+  TuneChoice tc;
+  if (!opts && cached_choice(p->arch, p->core, tc)) {
+    const PlanOpts have = p->core.opts;
+    if (tc.opts.outer_radix != have.outer_radix || tc.opts.fourstep_lg2 != have.fourstep_lg2) {
+      PlanCore alt;
+      if (plan_build(alt, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, ws_limit, tc.opts) == ST_OK)
+        p->core = std::move(alt);
+    }
+    apply_kernels(p, tc.kernels);
+  }
   apply_forced(p);
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);
@@ -443,6 +449,17 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
   }
   *plan = p;
   return MCFFT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                            mcfftFormat out_fmt, mcfftScaling scaling) {
+  if (!plan) return MCFFT_INVALID_VALUE;
+  *plan = nullptr;
+  return create_plan(plan, rank, dims, batch, in_fmt, out_fmt, scaling, nullptr);
 }
 
 // Real-input forward / real-output inverse plans (see include/mcfft.h).
@@ -900,65 +917,131 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
   return mcfft_exec_exp(plan, in, nullptr, out, direction, stream);
 }
 
-mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
-  if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
-  if (!in || !out || reps < 1) return MCFFT_INVALID_VALUE;
+// Every compiled kernel variant of each pass of `p`, timed over `reps`
+// launches on the given buffers; the fastest is kept per pass.
+static mcfftStatus tune_passes(mcfftPlan p, const void* in, void* out, int direction, void* stream, int reps,
+                               int n_cu, hipEvent_t e0, hipEvent_t e1, bool verbose) {
   hipStream_t s = (hipStream_t)stream;
-  int n_cu = 0;
-  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, plan->device) != hipSuccess)
-    return MCFFT_HIP_ERROR;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MCFFT_HIP_ERROR;
   mcfftStatus rs = MCFFT_SUCCESS;
-  std::string choice;
-  const char* venv = getenv("MCFFT_TUNE_VERBOSE");  // print every variant's time to stderr
-  const bool verbose = venv && venv[0] == '1';
-  for (size_t pi = 0; pi < plan->core.passes.size() && rs == MCFFT_SUCCESS; ++pi) {
-    PassPlan best = plan->core.passes[pi];
-    int best_cap = plan->grid_cap[pi];
+  for (size_t pi = 0; pi < p->core.passes.size() && rs == MCFFT_SUCCESS; ++pi) {
+    PassPlan best = p->core.passes[pi];
+    int best_cap = p->grid_cap[pi];
     float best_ms = 1e30f;
-    for (const KernelInfo* k : pass_candidates(plan->core.passes[pi], plan->core.cm, plan->core.scaling)) {
-      PassPlan trial = plan->core.passes[pi];
+    for (const KernelInfo* k : pass_candidates(p->core.passes[pi], p->core.cm, p->core.scaling)) {
+      PassPlan trial = p->core.passes[pi];
       if (!set_pass_kernel(trial, k)) continue;
       int per_cu = 0;
       if (set_lds_attr(k) != MCFFT_SUCCESS ||
           hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, k->nw * 64, k->lds_bytes) != hipSuccess ||
           per_cu < 1)
         continue;
-      const PassPlan saved = plan->core.passes[pi];
-      const int saved_cap = plan->grid_cap[pi];
-      plan->core.passes[pi] = trial;
-      plan->grid_cap[pi] = per_cu * n_cu;
-      rs = mcfft_exec_passes(plan, in, out, direction, stream, (int)pi, 1);  // warm-up
+      const PassPlan saved = p->core.passes[pi];
+      const int saved_cap = p->grid_cap[pi];
+      p->core.passes[pi] = trial;
+      p->grid_cap[pi] = per_cu * n_cu;
+      rs = mcfft_exec_passes(p, in, out, direction, stream, (int)pi, 1);  // warm-up
       float ms = 0.f;
       if (rs == MCFFT_SUCCESS && hipEventRecord(e0, s) == hipSuccess) {
         for (int r = 0; r < reps && rs == MCFFT_SUCCESS; ++r)
-          rs = mcfft_exec_passes(plan, in, out, direction, stream, (int)pi, 1);
+          rs = mcfft_exec_passes(p, in, out, direction, stream, (int)pi, 1);
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
           rs = MCFFT_HIP_ERROR;
       }
       if (rs == MCFFT_SUCCESS && verbose)
-        fprintf(stderr, "mcfft tune: pass %zu %s %.4f ms/launch\n", pi, k->name, ms / (float)reps);
+        fprintf(stderr, "mcfft tune: %s pass %zu %s %.4f ms/launch\n", opts_string(p->core.opts).c_str(), pi,
+                k->name, ms / (float)reps);
       if (rs == MCFFT_SUCCESS && ms < best_ms) {
         best_ms = ms;
         best = trial;
         best_cap = per_cu * n_cu;
       }
-      plan->core.passes[pi] = saved;
-      plan->grid_cap[pi] = saved_cap;
+      p->core.passes[pi] = saved;
+      p->grid_cap[pi] = saved_cap;
     }
-    plan->core.passes[pi] = best;
-    plan->grid_cap[pi] = best_cap;
-    choice += (pi ? "," : "") + std::string(best.k->name);
+    p->core.passes[pi] = best;
+    p->grid_cap[pi] = best_cap;
+  }
+  return rs;
+}
+
+// mean time of one whole exec of `p` (ms) over `reps` execs after one warm-up
+static mcfftStatus time_exec(mcfftPlan p, const void* in, void* out, int direction, void* stream, int reps,
+                             hipEvent_t e0, hipEvent_t e1, float* ms) {
+  hipStream_t s = (hipStream_t)stream;
+  const int np = (int)p->core.passes.size();
+  mcfftStatus rs = mcfft_exec_passes(p, in, out, direction, stream, 0, np);
+  if (rs == MCFFT_SUCCESS && hipEventRecord(e0, s) != hipSuccess) rs = MCFFT_HIP_ERROR;
+  for (int r = 0; r < reps && rs == MCFFT_SUCCESS; ++r) rs = mcfft_exec_passes(p, in, out, direction, stream, 0, np);
+  if (rs == MCFFT_SUCCESS && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    rs = MCFFT_HIP_ERROR;
+  *ms /= (float)reps;
+  return rs;
+}
+
+mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
+  if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
+  if (!in || !out || reps < 1) return MCFFT_INVALID_VALUE;
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, plan->device) != hipSuccess)
+    return MCFFT_HIP_ERROR;
+  const char* venv = getenv("MCFFT_TUNE_VERBOSE");  // print every variant's time to stderr
+  const bool verbose = venv && venv[0] == '1';
+  const PlanCore& c0 = plan->core;
+  // structures to try (a layout plan keeps its own: its staging is sized for it)
+  std::vector<PlanOpts> structs;
+  if (!plan->layout) {
+    long long ws_limit = 0;
+    if (const char* env = getenv("MCFFT_WS_BYTES")) ws_limit = atoll(env);
+    structs = plan_structures(c0.rank, c0.dims, c0.batch, c0.in_fmt, c0.out_fmt, c0.scaling, ws_limit);
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MCFFT_HIP_ERROR;
+  mcfftStatus rs = tune_passes(plan, in, out, direction, stream, reps, n_cu, e0, e1, verbose);
+  float best_ms = 0.f;
+  if (rs == MCFFT_SUCCESS) rs = time_exec(plan, in, out, direction, stream, reps, e0, e1, &best_ms);
+  if (rs == MCFFT_SUCCESS && verbose) fprintf(stderr, "mcfft tune: %s exec %.4f ms\n", opts_string(plan->core.opts).c_str(), best_ms);
+  mcfftPlan best = plan;
+  for (const PlanOpts& o : structs) {
+    if (rs != MCFFT_SUCCESS) break;
+    if (o.outer_radix == c0.opts.outer_radix && o.fourstep_lg2 == c0.opts.fourstep_lg2) continue;
+    mcfftPlan cand = nullptr;
+    if (create_plan(&cand, c0.rank, c0.dims, c0.batch, (mcfftFormat)c0.in_fmt, (mcfftFormat)c0.out_fmt,
+                    (mcfftScaling)c0.scaling, &o) != MCFFT_SUCCESS)
+      continue;  // e.g. the structure's workspace does not fit: not a candidate
+    float ms = 0.f;
+    rs = tune_passes(cand, in, out, direction, stream, reps, n_cu, e0, e1, verbose);
+    if (rs == MCFFT_SUCCESS) rs = time_exec(cand, in, out, direction, stream, reps, e0, e1, &ms);
+    if (rs == MCFFT_SUCCESS && verbose) fprintf(stderr, "mcfft tune: %s exec %.4f ms\n", opts_string(o).c_str(), ms);
+    if (rs == MCFFT_SUCCESS && ms < best_ms) {
+      best_ms = ms;
+      if (best != plan) free_plan(best);
+      best = cand;
+    } else {
+      free_plan(cand);
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (best != plan) {
+    if (rs == MCFFT_SUCCESS) std::swap(*plan, *best);  // the plan takes the faster structure's resources
+    free_plan(best);
+  }
   if (rs != MCFFT_SUCCESS) return rs;
+  TuneChoice choice;
+  choice.opts = plan->core.opts;
+  for (const PassPlan& pp : plan->core.passes) choice.kernels.push_back(pp.k->name);
   std::lock_guard<std::mutex> g(g_cache_mu);
   load_cache_locked(plan->arch);
-  g_cache[plan->arch + "|" + plan_signature(plan->core)] = choice;
+  g_cache[plan->arch + "|" + plan_signature(plan->core)] = format_tune_choice(choice);
   return save_cache_locked(plan->arch) ? MCFFT_SUCCESS : MCFFT_CACHE_ERROR;
+}
+
+mcfftStatus mcfft_build_id(const char** id) {
+  if (!id) return MCFFT_INVALID_VALUE;
+  *id = MCFFT_BUILD_ID;
+  return MCFFT_SUCCESS;
 }
 
 // The error-norm reduction into `part` (3 * kNormBlocks doubles of device
@@ -1098,42 +1181,18 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
     const long long nt = (t0 + c.chunk <= c.batch) ? c.chunk : c.batch - t0;
     for (int pi = first; pi < first + count; ++pi) {
       const PassPlan& pp = c.passes[pi];
-      PassArgs a;
-      memset(&a, 0, sizeof(a));
       auto bufptr = [&](int b, int fmt) -> char* {
         if (b == BUF_IN) return (char*)in + (size_t)(t0 * c.n) * (size_t)elem_bytes(fmt);
         if (b == BUF_OUT) return (char*)out + (size_t)(t0 * c.n) * (size_t)elem_bytes(fmt);
         return (char*)plan->ws;
       };
-      const int dst_fmt = (pp.dst == BUF_OUT) ? c.out_fmt : ws_fmt;
-      a.in = bufptr(pp.src, pp.src_fmt);
-      a.out = bufptr(pp.dst, dst_fmt);
-      a.stride_in = pp.stride_in;
-      a.stride_out = pp.stride_out;
-      a.in_sg = pp.in_sg;
-      a.in_st = pp.in_st;
-      a.out_sg = pp.out_sg;
-      a.out_st = pp.out_st;
-      a.n_inst = (int)pp.n_inst;
-      a.tiles_per_group = pp.tiles_per_group;
-      a.in_fmt = pp.src_fmt;
       auto view_elems = [&](int b) -> long long {
         return b == BUF_WS ? c.ws_elems : (c.batch - t0) * c.n;
       };
-      a.in_elems = view_elems(pp.src);
-      a.out_elems = view_elems(pp.dst);
-      a.stage_tw_entries = (int)((long long)c.tw.size() - pp.stage_tw_off);
-      a.inverse = d;
-      a.out_scale = pp.out_scale;
-      for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
-      a.stage_tw = plan->tw[d] + pp.stage_tw_off;
-      a.aops = plan->aops[d];
-      if (pp.fourstep) {
-        a.fs_tw = plan->tw[d] + pp.fs_tw_off;
-        a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);
-        a.fs_cols = pp.fs_cols;
-        a.fs_lo_bits = pp.fs_lo_bits;
-      }
+      const int dst_fmt = (pp.dst == BUF_OUT) ? c.out_fmt : ws_fmt;
+      PassArgs a;
+      fill_pass_args(a, c, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
+                     view_elems(pp.dst), d, plan->tw[d], plan->aops[d]);
       fill_bfp(a, c, pi, t0, nt, plan->real_kind == 2, in_exp,
                BfpBuffers{plan->state, plan->slots, plan->tile_e, plan->out_exp});
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;

@@ -31,7 +31,15 @@ bool split_len(long long L, int& S, int& T) {
   return true;
 }
 
-enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4 };
+enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4, KIND_ROWX = 5 };
+constexpr int kKinds = 6;
+static const KernelInfo* const kLists[kKinds] = {kRowKernels, kRowTKernels, kColKernels,
+                                                 kColFKernels, kColMKernels, kRowXKernels};
+static int kind_count(int kind) {
+  const int counts[kKinds] = {kRowKernelCount, kRowTKernelCount, kColKernelCount,
+                              kColFKernelCount, kColMKernelCount, kRowXKernelCount};
+  return counts[kind];
+}
 
 // Default tile shapes that measured fastest on MI355X (autotuner sweep over
 // batched 1D/2D/3D plans); other combinations use the first compiled variant.
@@ -40,8 +48,13 @@ struct PreferredShape {
 };
 static const PreferredShape kPreferred[] = {
     {KIND_ROWT, 2, 2, 0, 16, 8}, {KIND_ROWT, 2, 2, 1, 16, 8}, {KIND_ROWT, 2, 2, 2, 16, 8},
-    {KIND_ROW, 2, 2, 0, 8, 8},   {KIND_ROW, 2, 2, 1, 8, 8},
+    // 512-point rows: 8 rows on 2 waves (1D 512 x 131072 FP16 0.117 -> 0.106 ms, split 0.209 -> 0.186)
+    {KIND_ROW, 2, 2, 0, 8, 2},   {KIND_ROW, 2, 2, 1, 8, 2},   {KIND_ROW, 2, 2, 2, 8, 2},
+    // 1024/2048-point rows: four MFMA tiles per wave (2048^2 x 8 row pass 256 -> ~110 us)
+    {KIND_ROW, 2, 4, 0, 4, 2},   {KIND_ROW, 2, 4, 1, 4, 2},   {KIND_ROW, 2, 4, 2, 4, 2},
+    {KIND_ROW, 2, 8, 0, 4, 4},   {KIND_ROW, 2, 8, 1, 4, 4},   {KIND_ROW, 2, 8, 2, 4, 4},
     {KIND_COL, 2, 1, 0, 64, 8},  {KIND_COL, 2, 1, 1, 64, 8},
+    {KIND_COL, 2, 4, 0, 32, 16}, {KIND_COL, 2, 4, 1, 32, 16},  // 1024 points: 128-byte segments
     {KIND_ROW, 4, 1, 0, 1, 16},  {KIND_ROW, 4, 1, 1, 1, 16},   // single-pass 65536: 16 waves
 };
 
@@ -60,20 +73,22 @@ bool kernel_serves(const KernelInfo& k, int scaling) {
   return k.mode == 0 || (k.mode == 2) == (scaling == SCALE_AUTO);
 }
 
-static const KernelInfo* find_kernel(int S, int T, int cm, int kind, int scaling, long long ninst = 0) {
-  const KernelInfo* lists[5] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels, kColMKernels};
-  const int counts[5] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount,
-                         kColMKernelCount};
-  auto divides = [&](const KernelInfo& k) { return (ninst <= 0 || ninst % k.Q == 0) && kernel_serves(k, scaling); };
+// ROWX (`qx` > 0): the tile must hold exactly qx rows (the outer radix).
+static const KernelInfo* find_kernel(int S, int T, int cm, int kind, int scaling, long long ninst = 0, int qx = 0) {
+  const KernelInfo* list = kLists[kind];
+  const int count = kind_count(kind);
+  auto divides = [&](const KernelInfo& k) {
+    return (ninst <= 0 || ninst % k.Q == 0) && (qx <= 0 || k.Q == qx) && kernel_serves(k, scaling);
+  };
   for (const PreferredShape& p : kPreferred) {
     if (p.kind != kind || p.S != S || p.T != T || p.cm != cm) continue;
-    for (int i = 0; i < counts[kind]; ++i) {
-      const KernelInfo& k = lists[kind][i];
+    for (int i = 0; i < count; ++i) {
+      const KernelInfo& k = list[i];
       if (k.S == S && k.T == T && k.cm == cm && k.Q == p.Q && k.nw == p.nw && divides(k)) return &k;
     }
   }
-  for (int i = 0; i < counts[kind]; ++i) {
-    const KernelInfo& k = lists[kind][i];
+  for (int i = 0; i < count; ++i) {
+    const KernelInfo& k = list[i];
     if (k.S == S && k.T == T && k.cm == cm && divides(k)) return &k;
   }
   return nullptr;
@@ -119,6 +134,29 @@ std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, 
     ns = nsn;
   }
   return shifts;
+}
+
+// Radix R of a 2D plan's outer-axis split (NX = R * NX/R, ROWX + COL), 0 for
+// the plain row + column plan.  A column tile of NX points holds 128 KiB / NX /
+// (4 or 8 bytes) columns: under 32 (FP16/BF16) or 16 (split) columns, i.e.
+// row segments under 128 bytes, from 1024 (FP16/BF16) or 2048 (split) points.
+// Measured on MI355X (NX, NY in 1024..4096, all formats): R = 4 is the fastest
+// or within 4 % of it everywhere except 4096-point rows, whose 4-row tiles hold
+// 164-180 VGPRs (FP16/BF16) or 139 KiB of LDS (split): R = 2 there (4096^2 x 4
+// FP16 0.290 -> 0.281 ms, split 0.587 -> 0.533).  ROWX rows need at least 512
+// points.  `forced` >= 0 (a tuned or requested structure) replaces
+// the choice; MCFFT_2D_OUTER_RADIX overrides both.
+static int outer_radix(long long NX, long long NY, int forced) {
+  int r = 0;
+  if (NX >= kOuterSplitMin && NY >= 512) r = NY >= 4096 ? 2 : 4;
+  if (forced >= 0) r = forced;
+  if (const char* e = getenv("MCFFT_2D_OUTER_RADIX")) {
+    const int v = atoi(e);
+    if (v == 0 || v == 1) r = 0;
+    if (v == 2 || v == 4 || v == 8) r = v;
+  }
+  if (r < 2 || NY < 512 || NX / r < 16) return 0;
+  return r > 8 ? 0 : r;
 }
 
 // 1D lengths of at least 2^this run as three passes (MCFFT_THREE_PASS_MIN_LG
@@ -244,8 +282,10 @@ static int plan_three_pass(PlanCore& p, long long N, int cm, int scaling, int ws
 }
 
 int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
-               long long ws_limit_bytes) {
+               long long ws_limit_bytes, const PlanOpts& opt) {
   p = PlanCore();
+  p.opts.outer_radix = 0;
+  p.opts.fourstep_lg2 = 0;
   if (rank < 1 || rank > 3 || batch < 1 || !dims) return ST_INVALID_VALUE;
   if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2) return ST_INVALID_VALUE;
   if (scaling < 0 || scaling > 3) return ST_INVALID_VALUE;
@@ -338,7 +378,12 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
       // W_N^(k1*n2), then rows of length N2 stored transposed X[k1 + N1*k2]
       int lg2 = (lg + 1) / 2;
       if (lg2 > 11) lg2 = 11;
+      // a requested split: N2 = 2^4 .. 2^11 (ROWT), N1 = 2^4 .. 2^12 (COLF)
+      if (opt.fourstep_lg2 >= 4 && opt.fourstep_lg2 <= 11 && lg - opt.fourstep_lg2 >= 4 &&
+          lg - opt.fourstep_lg2 <= 12)
+        lg2 = opt.fourstep_lg2;
       const int lg1 = lg - lg2;
+      p.opts.fourstep_lg2 = lg2;
       if (lg1 > 12 || lg2 > 12) return ST_UNSUPPORTED;
       const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
       int S1, T1, S2, T2;
@@ -413,8 +458,64 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     if (chunk > batch) chunk = batch;
     p.chunk = chunk;
     p.n_chunks = (batch + chunk - 1) / chunk;
-    st = add_row(NY, NX, BUF_IN, BUF_OUT, in_fmt, true);
-    if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
+    const int R = outer_radix(NX, NY, opt.outer_radix);
+    int SY, TY, SX, TX;
+    const KernelInfo* kx = nullptr;
+    const KernelInfo* kcx = nullptr;
+    if (R > 1 && split_len(NY, SY, TY) && split_len(NX / R, SX, TX)) {
+      kx = find_kernel(SY, TY, cm, KIND_ROWX, scaling, 0, R);
+      kcx = find_kernel(SX, TX, cm, KIND_COL, scaling, R * NY);
+    }
+    if (kx && kcx && col_view_fits(NX / R, R * NY, kcx->Q, cm)) {
+      p.opts.outer_radix = R;
+      // Outer axis split NX = R * (NX/R), rows r = r1 + (NX/R)*r2:
+      //   ROWX (in -> ws): tile r1 takes rows r1 + (NX/R)*r2 (r2 < R), their
+      //     R-point DFT times W_NX^(r1*k2), then the NY-point row FFTs; result k2
+      //     stored as row R*r1 + k2
+      //   COL  (ws -> out): NX/R-point columns at stride R rows (column
+      //     k2*NY + c of the R*NY columns is rows R*r1 + k2), output row k2 + R*k1
+      // The tiles of ROWX write rows other tiles read, so it runs out of place
+      // through a workspace (one chunk).
+      p.ws_elems = chunk * p.n;
+      PassPlan a;
+      a.k = kx;
+      a.kind = KIND_ROWX;
+      a.L = NY;
+      a.src = BUF_IN;
+      a.dst = BUF_WS;
+      a.src_fmt = in_fmt;
+      a.stride_in = (int)(NX / R * NY);
+      a.stride_out = (int)NY;
+      a.in_sg = a.out_sg = p.n;
+      a.in_st = NY;
+      a.out_st = R * NY;
+      a.n_inst = NX;
+      a.tiles_per_group = (int)(NX / R);
+      a.groups = batch;
+      a.groups_per_transform = 1;
+      a.shifts = add_stage_tables(p.tw, a, SY, TY, scaling);
+      a.out_scale = pow2(-a.shifts.back());
+      a.shifts.push_back(stage_shift(R, scaling));
+      if (cm == 2) {
+        a.x_tw_off = (long long)p.tw.size();
+        for (long long r1 = 0; r1 < NX / R; ++r1)
+          for (long long k2 = 0; k2 < R; ++k2) p.tw.push_back({(r1 * k2) % NX, NX, 1.f});
+      }
+      for (int sh : a.shifts) total_shift += sh;
+      p.passes.push_back(a);
+      st = add_col(NX / R, R * NY, R * NY, batch, p.n, BUF_WS, BUF_OUT, ws_fmt);
+      if (st == ST_OK) {
+        // AUTO reconcile: element (q, m) of the column pass is row R*m + k2, made
+        // by ROWX tile m (first-pass tile = (m*R) >> log2(R))
+        PassPlan& c = p.passes.back();
+        c.rc_g = 0;
+        c.rc_m = R;
+        c.rc_q = 0;
+      }
+    } else {
+      st = add_row(NY, NX, BUF_IN, BUF_OUT, in_fmt, true);
+      if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
+    }
   } else {
     const long long NX = p.dims[0], NY = p.dims[1], NZ = p.dims[2];
     p.chunk = batch;
@@ -428,13 +529,86 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   return ST_OK;
 }
 
+std::string opts_string(const PlanOpts& o) {
+  return "R=" + std::to_string(o.outer_radix) + ";L2=" + std::to_string(o.fourstep_lg2);
+}
+
+bool parse_opts(const std::string& s, PlanOpts& o) {
+  return sscanf(s.c_str(), "R=%d;L2=%d", &o.outer_radix, &o.fourstep_lg2) == 2;
+}
+
+static const char kTuneCacheMagic[] = "mcfft-tune-cache 2";
+
+std::map<std::string, std::string> tune_cache_parse(const std::string& text, const std::string& build_id) {
+  std::map<std::string, std::string> out;
+  std::istringstream in(text);
+  std::string line;
+  if (!std::getline(in, line) || line != std::string(kTuneCacheMagic) + " " + build_id) return out;
+  while (std::getline(in, line)) {
+    const size_t eq = line.find('=');
+    if (eq != std::string::npos && eq > 0) out[line.substr(0, eq)] = line.substr(eq + 1);
+  }
+  return out;
+}
+
+std::string tune_cache_format(const std::map<std::string, std::string>& entries, const std::string& build_id) {
+  std::string s = std::string(kTuneCacheMagic) + " " + build_id + "\n";
+  for (const auto& kv : entries) s += kv.first + "=" + kv.second + "\n";
+  return s;
+}
+
+bool parse_tune_choice(const std::string& value, TuneChoice& c) {
+  const size_t k = value.find(";K=");
+  if (k == std::string::npos || !parse_opts(value.substr(0, k), c.opts)) return false;
+  c.kernels.clear();
+  std::stringstream ss(value.substr(k + 3));
+  std::string n;
+  while (std::getline(ss, n, ',')) c.kernels.push_back(n);
+  return !c.kernels.empty();
+}
+
+std::string format_tune_choice(const TuneChoice& c) {
+  std::string s = opts_string(c.opts) + ";K=";
+  for (size_t i = 0; i < c.kernels.size(); ++i) s += (i ? "," : "") + c.kernels[i];
+  return s;
+}
+
+std::vector<PlanOpts> plan_structures(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                      int scaling, long long ws_limit_bytes) {
+  std::vector<PlanOpts> out, tries;
+  tries.push_back(PlanOpts());
+  if (rank == 2) {
+    for (int r : {0, 2, 4, 8}) {
+      PlanOpts o;
+      o.outer_radix = r;
+      tries.push_back(o);
+    }
+  } else if (rank == 1 && dims[0] > 0) {
+    // the default split and its neighbours (N1 = N/N2 one factor of 2 either way)
+    const int lg = ilog2(dims[0]), d = (lg + 1) / 2 > 11 ? 11 : (lg + 1) / 2;
+    for (int l2 = d - 1; l2 <= d + 1; ++l2) {
+      PlanOpts o;
+      o.fourstep_lg2 = l2;
+      tries.push_back(o);
+    }
+  }
+  for (const PlanOpts& t : tries) {
+    PlanCore c;
+    if (plan_build(c, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit_bytes, t) != ST_OK) continue;
+    bool seen = false;
+    for (const PlanOpts& o : out) seen |= o.outer_radix == c.opts.outer_radix && o.fourstep_lg2 == c.opts.fourstep_lg2;
+    if (!seen) out.push_back(c.opts);
+  }
+  return out;
+}
+
 std::vector<const KernelInfo*> pass_candidates(const PassPlan& pp, int cm, int scaling) {
   std::vector<const KernelInfo*> out;
-  const KernelInfo* lists[5] = {kRowKernels, kRowTKernels, kColKernels, kColFKernels, kColMKernels};
-  const int counts[5] = {kRowKernelCount, kRowTKernelCount, kColKernelCount, kColFKernelCount, kColMKernelCount};
   if (!pp.k) return out;
-  for (int i = 0; i < counts[pp.kind]; ++i) {
-    const KernelInfo& k = lists[pp.kind][i];
+  for (int i = 0; i < kind_count(pp.kind); ++i) {
+    const KernelInfo& k = kLists[pp.kind][i];
+    // a ROWX tile's row count is the plan's outer radix: only its wave counts vary
+    if (pp.kind == KIND_ROWX && k.Q != pp.k->Q) continue;
     if (k.S == pp.k->S && k.T == pp.k->T && k.cm == cm && kernel_serves(k, scaling)) out.push_back(&k);
   }
   return out;
@@ -446,6 +620,9 @@ bool set_pass_kernel(PassPlan& pp, const KernelInfo* k) {
     case KIND_ROW:
       pp.in_st = pp.out_st = Q * pp.L;
       pp.tiles_per_group = (int)((pp.n_inst + Q - 1) / Q);
+      break;
+    case KIND_ROWX:
+      if (Q != pp.k->Q) return false;  // the tile's rows are the plan's outer radix
       break;
     case KIND_ROWT:
       if (pp.n_inst % Q) return false;
@@ -476,7 +653,7 @@ std::string PlanCore::describe() const {
   o << "rank=" << rank << " dims=" << dims[0];
   for (int d = 1; d < rank; ++d) o << "x" << dims[d];
   o << " batch=" << batch << " cm=" << cm << " in=" << in_fmt << " out=" << out_fmt << " exp=" << out_exponent
-    << " chunk=" << chunk << "x" << n_chunks << " ws=" << ws_elems << "\n";
+    << " chunk=" << chunk << "x" << n_chunks << " ws=" << ws_elems << " struct=" << opts_string(opts) << "\n";
   for (const auto& pp : passes) {
     o << "  pass " << pp.k->name << " src=" << pp.src << " dst=" << pp.dst << " groups=" << pp.groups
       << " tiles/group=" << pp.tiles_per_group << " n_inst=" << pp.n_inst << " stride(in,out)=(" << pp.stride_in

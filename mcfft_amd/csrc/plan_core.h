@@ -9,6 +9,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,10 @@ enum Scaling { SCALE_NONE = 0, SCALE_BALANCED = 1, SCALE_FULL = 2, SCALE_AUTO = 
 // maxima lie within one step of 2^4 agree on their exponent, so the pass after
 // the first has nothing to reconcile for such data
 constexpr int kBfpQuantumMultiPass = 4;
+// 2D plans split an outer axis of at least this many points (ROWX + COL), so
+// that its column pass keeps 128-byte row segments (MCFFT_2D_OUTER_RADIX
+// overrides the radix: 0 or 1 off, 2/4/8)
+constexpr long long kOuterSplitMin = 1024;
 // 1D lengths: single pass up to 2^16 (FP16/BF16), two-pass four-step up to
 // 2^23, three passes from 2^24 to 2^kMaxLg1D
 constexpr int kThreePassMinLg = 24;
@@ -38,7 +43,7 @@ struct TwSpec {
 
 struct PassPlan {
   const KernelInfo* k = nullptr;
-  int kind = 0;                 // 0 ROW, 1 ROWT, 2 COL, 3 COLF, 4 COLM
+  int kind = 0;                 // 0 ROW, 1 ROWT, 2 COL, 3 COLF, 4 COLM, 5 ROWX
   long long L = 0;              // points per instance
   int src = BUF_IN, dst = BUF_OUT;
   int src_fmt = 0;
@@ -54,14 +59,23 @@ struct PassPlan {
   long long fs_tw_off = 0;      // four-step table W_N^(k1*n2): N1 x fs_cols entries
   int fs_cols = 0;
   int fs_lo_bits = 0;           // > 0: the table in two factors (fft_pass.h, PassArgs::fs_tw)
+  long long x_tw_off = -1;      // split ROWX: W_NX^(r1*k2) table (NX entries), else -1
   // reconcile pass (AUTO): element (q, m) of group g came from first-pass instance
   // (g % groups_per_transform) * rc_g + m * rc_m + q * rc_q (rc_g < 0: L)
   int rc_g = -1, rc_m = 1, rc_q = 0;
-  std::vector<int> shifts;      // per stage, for reporting
+  std::vector<int> shifts;      // per stage (ROWX: the row stages, then its cross-row butterfly)
+};
+
+// Plan structure: the choices the autotuner enumerates (plan_structures) and
+// records; -1 = the planner's default rule.
+struct PlanOpts {
+  int outer_radix = -1;   // 2D: radix R of the outer-axis split (0: none; 2, 4, 8)
+  int fourstep_lg2 = -1;  // 1D two-pass four-step: log2 N2 (points of the second pass's rows)
 };
 
 struct PlanCore {
   int rank = 0;
+  PlanOpts opts;                // the structure this plan was built with (defaults resolved)
   long long dims[3] = {1, 1, 1};
   long long batch = 0;
   long long n = 0;              // elements per transform
@@ -106,8 +120,30 @@ float pow2(int e);                               // 2^e exactly
 std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, int T, int scaling);
 // compute mode from the input/output formats; -1 if the pair is not supported
 int compute_mode(int in_fmt, int out_fmt);
-// ws_limit_bytes: upper bound of the four-step workspace (0 = default)
+// ws_limit_bytes: upper bound of the four-step workspace (0 = default);
+// opt: structure (defaults: the planner's rules)
 int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
-               long long ws_limit_bytes);
+               long long ws_limit_bytes, const PlanOpts& opt = PlanOpts());
+// the distinct structures a plan of this shape can take (the default first;
+// each resolved as plan_build would record it)
+std::vector<PlanOpts> plan_structures(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                      int scaling, long long ws_limit_bytes);
+// "R=<outer radix>;L2=<four-step log2 N2>" and its inverse (false if malformed)
+std::string opts_string(const PlanOpts& o);
+bool parse_opts(const std::string& s, PlanOpts& o);
+
+// Tuning cache (one file per GPU architecture, kept by the runtime): a header
+// line "mcfft-tune-cache 2 <build id>", then one line per tuned plan,
+//     <arch>|<plan signature>=R=<r>;L2=<l>;K=<kernel>,<kernel>,...
+// A file whose header names another format or another build of the library
+// (whose kernels of the same name may compute differently) yields no entries.
+struct TuneChoice {
+  PlanOpts opts;
+  std::vector<std::string> kernels;  // one per pass
+};
+std::map<std::string, std::string> tune_cache_parse(const std::string& text, const std::string& build_id);
+std::string tune_cache_format(const std::map<std::string, std::string>& entries, const std::string& build_id);
+bool parse_tune_choice(const std::string& value, TuneChoice& c);
+std::string format_tune_choice(const TuneChoice& c);
 
 }  // namespace mcfft

@@ -167,6 +167,48 @@ def describe(dims, batch, in_fmt, out_fmt, scaling=1):
     return buf.value.decode()
 
 
+def structures(dims, batch, in_fmt, out_fmt, scaling=1) -> list:
+    """The plan structures the autotuner tries for this shape (opts strings,
+    "R=<outer radix>;L2=<four-step log2 N2>"), the default first."""
+    L = lib()
+    L.emu_plan_structures.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    buf = ctypes.create_string_buffer(4096)
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    L.emu_plan_structures(len(dims), d, batch, in_fmt, out_fmt, scaling, buf, 4096)
+    return buf.value.decode().split()
+
+
+def describe_opts(dims, batch, in_fmt, out_fmt, scaling=1, outer_radix=-1, fourstep_lg2=-1) -> str:
+    L = lib()
+    L.emu_plan_describe_opts.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_char_p, ctypes.c_int]
+    buf = ctypes.create_string_buffer(8192)
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    rc = L.emu_plan_describe_opts(len(dims), d, batch, in_fmt, out_fmt, scaling, outer_radix, fourstep_lg2, buf, 8192)
+    if rc != 0:
+        raise RuntimeError(f"plan status {rc}")
+    return buf.value.decode()
+
+
+def set_structure(outer_radix=-1, fourstep_lg2=-1):
+    """Structure of the plans of later exec_fft / exec_ptr calls (-1: defaults)."""
+    L = lib()
+    L.emu_set_structure.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.emu_set_structure(outer_radix, fourstep_lg2)
+
+
+def tune_cache_lookup(text: str, build_id: str, key: str):
+    """The tuning-cache entry for `key` as a library built as `build_id` reads
+    the cache file `text` (re-formatted), or None."""
+    L = lib()
+    L.emu_tune_cache_lookup.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_char_p, ctypes.c_int]
+    buf = ctypes.create_string_buffer(4096)
+    rc = L.emu_tune_cache_lookup(text.encode(), build_id.encode(), key.encode(), buf, 4096)
+    return buf.value.decode() if rc == 0 else None
+
+
 def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=False, nblocks=3, single=False):
     """Ragged batch: xs = list of complex arrays, shapes = list of dims lists.
     layout: list of (in_off, out_off) element offsets (default back to back).

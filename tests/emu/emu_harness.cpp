@@ -56,6 +56,41 @@ static void dft_table_host(int sign, uint16_t* t) {
   }
 }
 
+// the structures the autotuner would try for this plan shape, one
+// opts_string per line
+extern "C" int emu_plan_structures(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                   int scaling, char* buf, int buflen) {
+  std::string s;
+  for (const PlanOpts& o : plan_structures(rank, dims, batch, in_fmt, out_fmt, scaling, 0)) s += opts_string(o) + "\n";
+  snprintf(buf, buflen, "%s", s.c_str());
+  return 0;
+}
+
+// the tuning cache's parser: the entry for `key` in cache file `text` as the
+// runtime (built as `build_id`) would see it, re-formatted; 0 if found
+extern "C" int emu_tune_cache_lookup(const char* text, const char* build_id, const char* key, char* buf, int buflen) {
+  const auto m = tune_cache_parse(text, build_id);
+  const auto it = m.find(key);
+  if (it == m.end()) return 1;
+  TuneChoice c;
+  if (!parse_tune_choice(it->second, c)) return 2;
+  snprintf(buf, buflen, "%s", format_tune_choice(c).c_str());
+  return 0;
+}
+
+// a plan built with an explicit structure (the autotuner's candidates)
+extern "C" int emu_plan_describe_opts(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                      int scaling, int outer_radix, int fourstep_lg2, char* buf, int buflen) {
+  PlanCore p;
+  PlanOpts o;
+  o.outer_radix = outer_radix;
+  o.fourstep_lg2 = fourstep_lg2;
+  const int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, 0, o);
+  if (st) return st;
+  snprintf(buf, buflen, "%s", p.describe().c_str());
+  return 0;
+}
+
 extern "C" int emu_plan_describe(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
                                  int scaling, char* buf, int buflen) {
   PlanCore p;
@@ -97,10 +132,17 @@ extern "C" void emu_restrict(int pass, long long blk_first, long long blk_count)
   g_blk_count = blk_count;
 }
 
+// plan structure of the next emu_exec calls (the autotuner's candidates; -1: defaults)
+static PlanOpts g_opts;
+extern "C" void emu_set_structure(int outer_radix, int fourstep_lg2) {
+  g_opts.outer_radix = outer_radix;
+  g_opts.fourstep_lg2 = fourstep_lg2;
+}
+
 extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
                         int inverse, const void* in, void* out, long long ws_limit, int* out_exponent) {
   PlanCore p;
-  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit);
+  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit, g_opts);
   if (st) return st;
   if (!g_kernel_override.empty())
     for (PassPlan& pp : p.passes)
@@ -156,38 +198,18 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
     for (size_t pi = 0; pi < p.passes.size(); ++pi) {
       const PassPlan& pp = p.passes[pi];
       if (g_only_pass >= 0 && (int)pi != g_only_pass) continue;
-      PassArgs a;
-      memset(&a, 0, sizeof(a));
       auto bufptr = [&](int b, int fmt) -> char* {
         if (b == BUF_IN) return (char*)in + (size_t)(t0 * p.n * elem_bytes(fmt));
         if (b == BUF_OUT) return (char*)out + (size_t)(t0 * p.n * elem_bytes(fmt));
         return (char*)ws.data();
       };
-      const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
-      a.in = bufptr(pp.src, pp.src_fmt);
-      a.out = bufptr(pp.dst, dst_fmt);
-      a.stride_in = pp.stride_in; a.stride_out = pp.stride_out;
-      a.in_sg = pp.in_sg; a.in_st = pp.in_st; a.out_sg = pp.out_sg; a.out_st = pp.out_st;
-      a.n_inst = (int)pp.n_inst;
-      a.tiles_per_group = pp.tiles_per_group;
-      a.in_fmt = pp.src_fmt;
       auto view_elems = [&](int b) -> long long {
         return b == BUF_WS ? p.ws_elems : (p.batch - t0) * p.n;
       };
-      a.in_elems = view_elems(pp.src);
-      a.out_elems = view_elems(pp.dst);
-      a.stage_tw_entries = (int)((long long)p.tw.size() - pp.stage_tw_off);
-      a.inverse = inverse;
-      a.out_scale = pp.out_scale;
-      for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
-      a.stage_tw = tw.data() + pp.stage_tw_off;
-      a.aops = aops.data();
-      if (pp.fourstep) {
-        a.fs_tw = tw.data() + pp.fs_tw_off;
-        a.fs_tw_entries = (int)((long long)p.tw.size() - pp.fs_tw_off);
-        a.fs_cols = pp.fs_cols;
-        a.fs_lo_bits = pp.fs_lo_bits;
-      }
+      const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
+      PassArgs a;
+      fill_pass_args(a, p, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
+                     view_elems(pp.dst), inverse, tw.data(), aops.data());
       fill_bfp(a, p, (int)pi, t0, nt, false, in_exp, bufs);
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * pp.tiles_per_group;

@@ -393,9 +393,9 @@ def test_column_views_fit_32bit_byte_offsets():
 
 def test_col_pass_byte_offsets_beyond_2gib():
     """The x-axis pass of an FP16 1024^3 plan addresses its columns through one
-    buffer view of (1023 * 2^20 + 16) * 4 ~ 4.29e9 bytes, so half of every tile's
-    element offsets become byte offsets above 2^31.  Run the last 32 tiles of
-    that pass (columns 2^20 - 512 .. 2^20 - 1 of the y-z plane, every x row) in
+    buffer view of (1023 * 2^20 + Q) * 4 ~ 4.29e9 bytes, so half of every tile's
+    element offsets become byte offsets above 2^31.  Run the tiles of that pass
+    that cover columns 2^20 - 512 .. 2^20 - 1 of the y-z plane (every x row) in
     place on a 4 GiB anonymous mapping, of which only the touched pages are ever
     backed, and check each column against an FP64 FFT along x."""
     import mmap
@@ -414,10 +414,12 @@ def test_col_pass_byte_offsets_beyond_2gib():
         buf[idx] = emu.to_storage(x.reshape(-1), emu.FMT_F16).view(np.uint32).reshape(nx, 512)
         xq = emu.from_storage(buf[idx].reshape(-1), emu.FMT_F16).reshape(nx, 512)
         desc = emu.describe([nx, 1024, 1024], 1, emu.FMT_F16, emu.FMT_F16, 0)
-        assert "stride(in,out)=(1048576,1048576)" in desc.splitlines()[3]
+        line = desc.splitlines()[3]
+        assert "stride(in,out)=(1048576,1048576)" in line
+        q = int(line.split()[1].split("_")[4])   # columns per tile of the x-axis pass
         ptr = ctypes.addressof(ctypes.c_char.from_buffer(mm))
         emu.exec_ptr(ptr, ptr, [nx, 1024, 1024], 1, emu.FMT_F16, emu.FMT_F16, scaling=0, only_pass=2,
-                     blocks=(plane // 16 - 32, 32))
+                     blocks=(plane // q - 512 // q, 512 // q))
         y = emu.from_storage(buf[idx].reshape(-1), emu.FMT_F16).reshape(nx, 512)
         ref = np.fft.fft(xq, axis=0)
         err = oracle.rel_l2(y.reshape(-1), ref.reshape(-1))
