@@ -98,6 +98,17 @@ inline void fill_pass_args(PassArgs& a, const PlanCore& c, const PassPlan& pp, c
   }
 }
 
+// A gathered batch (a ragged plan's same-shape items at arbitrary offsets): the
+// pass's sides on the caller's buffers take transform t of the launch from
+// tab[side][t0 + t] (tab[0]: input offsets, tab[1]: output offsets, elements);
+// `in`/`out` given to fill_pass_args are then the buffers' bases.
+inline void gather_args(PassArgs& a, const PassPlan& pp, const long long* const tab[2], long long t0) {
+  auto side = [&](int b) -> const long long* { return b == BUF_WS ? nullptr : tab[b == BUF_IN ? 0 : 1] + t0; };
+  a.gather_in = side(pp.src);
+  a.gather_out = side(pp.dst);
+  a.gather_gpt = (int)pp.groups_per_transform;
+}
+
 // sizes of the bookkeeping buffers of plan c: slots (64-bit) and tile
 // exponents (bytes: one per first-pass instance of a chunk, at least as many
 // as the first pass's tiles whatever tile width it runs)

@@ -55,9 +55,9 @@ struct BigCfg {
 };
 
 // GBuf of transform `tile` (ROW layout, one transform per tile)
-MCFFT_DEV GBuf big_view(const void* p, long long sg, long long st, int tpg, int tile) {
+MCFFT_DEV GBuf big_view(const void* p, const long long* off, int gpt, long long sg, long long st, int tpg, int tile) {
   const int grp = tile / tpg, til = tile - grp * tpg;
-  const long long base = (long long)grp * sg + (long long)til * st;
+  const long long base = group_base(off, gpt, sg, grp) + (long long)til * st;
   return make_gbuf((const char*)p + base * 4, 65536u * 4u);
 }
 
@@ -81,8 +81,9 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
   const int tid = opaque(thread_id());
   const int lane = tid & 63, wave = tid >> 6, h = lane >> 5, c5 = lane & 31;
   const int low = 32 * wave + c5;  // column j of MFMA tile k is low + 32 NW k
-  const GBuf gout = big_view(a.out, a.out_sg, a.out_st, a.tiles_per_group, tile);
-  const GBuf gnext = big_view(a.in, a.in_sg, a.in_st, a.tiles_per_group, next < 0 ? tile : next);
+  const GBuf gout = big_view(a.out, a.gather_out, a.gather_gpt, a.out_sg, a.out_st, a.tiles_per_group, tile);
+  const GBuf gnext =
+      big_view(a.in, a.gather_in, a.gather_gpt, a.in_sg, a.in_st, a.tiles_per_group, next < 0 ? tile : next);
   const int rbase = low + (low >> 4) + 2 * h * 8704;
   const int vin = (low + 8 * h * B::LQ) * 4;  // stage-0 input r = 8h + t of column j: element j + r * 4096
 
@@ -216,7 +217,7 @@ MCFFT_DEV void fft_big_body(const PassArgs& a) {
   if (tile < a.n_tiles) {
     const int tid = thread_id(), lane = tid & 63, low = 32 * (tid >> 6) + (lane & 31);
     const int vin = (low + 8 * (lane >> 5) * B::LQ) * 4;
-    const GBuf gin = big_view(a.in, a.in_sg, a.in_st, a.tiles_per_group, tile);
+    const GBuf gin = big_view(a.in, a.gather_in, a.gather_gpt, a.in_sg, a.in_st, a.tiles_per_group, tile);
 #pragma unroll
     for (int k = 0; k < B::TPW; ++k)
 #pragma unroll

@@ -106,7 +106,20 @@ struct PassArgs {
   signed char* bfp_tile_e;    // exponents of the launch's first-pass tiles
   int* bfp_out_exp;           // per transform
   const int* bfp_in_exp;      // per transform (caller's input exponents), or null
+  // gathered batch: element offset of each transform of the launch from `in` /
+  // `out` (null: transforms follow each other, group g at g*sg); group g of
+  // transform t = g / gather_gpt starts at off[t] + (g % gather_gpt) * sg
+  const long long* gather_in;
+  const long long* gather_out;
+  int gather_gpt;
 };
+
+// first element of group `grp` of a side with group stride sg (see PassArgs::gather_*)
+MCFFT_DEV long long group_base(const long long* off, int gpt, long long sg, int grp) {
+  if (!off) return (long long)grp * sg;
+  const int t = grp / gpt;
+  return g_ld(off, (long long)t) + (long long)(grp - t * gpt) * sg;
+}
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:
 //   [0..15] re_hi f16, [16..31] im_hi f16, [32..47] re_lo f16 (x 2^11),
@@ -517,7 +530,7 @@ MCFFT_DEV void cross_rows(const PassArgs& a, int r1, bool inv, float nsc, typena
 template <class C>
 MCFFT_DEV GBuf tile_in_view(const PassArgs& a, int tile, int esz_in) {
   const int grp = uniform(tile / a.tiles_per_group), til = uniform(tile - grp * a.tiles_per_group);
-  const long long ib = (long long)grp * a.in_sg + (long long)til * a.in_st;
+  const long long ib = group_base(a.gather_in, a.gather_gpt, a.in_sg, grp) + (long long)til * a.in_st;
   long long in_end = a.in_elems;
   if constexpr (C::KIND_ == KIND_ROW) {
     const int inst0 = til * C::Q_;
@@ -566,7 +579,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
   {
     // wave-uniform (scalar registers): kept to the end of the tile by the bookkeeping
     const int grp = uniform(tile / a.tiles_per_group), til = uniform(tile - grp * a.tiles_per_group);
-    const long long ob = (long long)grp * a.out_sg + (long long)til * a.out_st;
+    const long long ob = group_base(a.gather_out, a.gather_gpt, a.out_sg, grp) + (long long)til * a.out_st;
     const int inst0 = til * Q;
     // A ROW tile's instances are contiguous, so its buffer views end at the last
     // valid instance: loads beyond read 0 and stores beyond are dropped.

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <fstream>
+#include <algorithm>
 #include <map>
 #include <sstream>
 #include <cstdio>
@@ -59,6 +60,11 @@ struct mcfftPlan_st {
   };
   std::vector<Bulk> bulk;
   bool has_queue = false;
+  // a gathered batch (a ragged plan's bulk run of same-shape items wherever they
+  // lie): per-transform element offsets into the caller's input [0] and output
+  // [1] buffers (device), and the element counts addressable from those buffers
+  long long* gather[2] = {nullptr, nullptr};
+  long long gather_ext[2] = {0, 0};
   std::vector<int> item_exp;    // per item, from whichever path runs it
   // real-input / real-output plans: `core` is the length-H complex plan
   int real_kind = 0;          // 0 complex, 1 R2C, 2 C2R
@@ -338,6 +344,8 @@ void free_plan(mcfftPlan p) {
   if (p->real_tmp) (void)hipFree(p->real_tmp);
   for (int d = 0; d < 2; ++d)
     if (p->real_tw[d]) (void)hipFree(p->real_tw[d]);
+  for (int d = 0; d < 2; ++d)
+    if (p->gather[d]) (void)hipFree(p->gather[d]);
   if (p->lay_buf[0]) (void)hipFree(p->lay_buf[0]);
   if (p->lay_buf[1] && p->lay_buf[1] != p->lay_buf[0]) (void)hipFree(p->lay_buf[1]);
   delete p;
@@ -666,6 +674,67 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
       rem_idx.push_back(k);
     }
     i = j;
+  }
+  // Gathered bulk runs: the remaining items of one shape wherever they lie, at
+  // least bulk_min elements together, run as one batched plan whose transforms
+  // are located through offset tables -- 2D/3D items and 1D items of >= 4096
+  // points (whose row tiles hold one transform; shorter rows share tiles, so
+  // they stay in the queue unless they lie back to back).
+  if (bulk_min > 0 && !spec.empty()) {
+    long long ext[2] = {0, 0};
+    for (int i = 0; i < n_items; ++i) {
+      long long n = 1;
+      for (int d = 0; d < items[i].rank && d < 3; ++d) n *= items[i].dims[d];
+      ext[0] = std::max(ext[0], items[i].in_offset + n);
+      ext[1] = std::max(ext[1], items[i].out_offset + n);
+    }
+    std::map<std::vector<long long>, std::vector<size_t>> by_shape;
+    for (size_t k = 0; k < spec.size(); ++k)
+      by_shape[{spec[k].rank, spec[k].dims[0], spec[k].rank > 1 ? spec[k].dims[1] : 1,
+                spec[k].rank > 2 ? spec[k].dims[2] : 1}].push_back(k);
+    std::vector<bool> taken(spec.size(), false);
+    for (const auto& kv : by_shape) {
+      const RaggedItemSpec& a = spec[kv.second.front()];
+      const long long cnt = (long long)kv.second.size();
+      const long long n = kv.first[1] * kv.first[2] * kv.first[3];
+      if ((a.rank == 1 && a.dims[0] < 4096) || cnt * n < bulk_min) continue;
+      PlanOpts o;
+      o.gathered = true;
+      const long long dims[3] = {a.dims[0], a.dims[1], a.dims[2]};
+      mcfftPlan sub = nullptr;
+      if (create_plan(&sub, a.rank, dims, cnt, in_fmt, out_fmt, scaling, &o) != MCFFT_SUCCESS) continue;
+      std::vector<long long> off[2];
+      for (size_t k : kv.second) {
+        off[0].push_back(spec[k].in_off);
+        off[1].push_back(spec[k].out_off);
+      }
+      bool ok = true;
+      for (int d = 0; d < 2 && ok; ++d)
+        ok = hipMalloc(&sub->gather[d], sizeof(long long) * (size_t)cnt) == hipSuccess &&
+             hipMemcpy(sub->gather[d], off[d].data(), sizeof(long long) * (size_t)cnt, hipMemcpyHostToDevice) ==
+                 hipSuccess;
+      sub->gather_ext[0] = ext[0];
+      sub->gather_ext[1] = ext[1];
+      int e = 0;
+      if (!ok || mcfft_get_output_exponent(sub, &e) != MCFFT_SUCCESS) {
+        free_plan(sub);
+        continue;
+      }
+      p->bulk.push_back({sub, 0, 0});
+      for (size_t k : kv.second) {
+        taken[k] = true;
+        p->item_exp[(size_t)rem_idx[k]] = e;
+      }
+    }
+    std::vector<RaggedItemSpec> spec2;
+    std::vector<int> rem2;
+    for (size_t k = 0; k < spec.size(); ++k)
+      if (!taken[k]) {
+        spec2.push_back(spec[k]);
+        rem2.push_back(rem_idx[k]);
+      }
+    spec.swap(spec2);
+    rem_idx.swap(rem2);
   }
   if (hipGetDevice(&p->device) != hipSuccess) {
     free_plan(p);
@@ -1181,18 +1250,25 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
     const long long nt = (t0 + c.chunk <= c.batch) ? c.chunk : c.batch - t0;
     for (int pi = first; pi < first + count; ++pi) {
       const PassPlan& pp = c.passes[pi];
+      const bool gathered = plan->gather[0] != nullptr;  // transforms located by offset tables
       auto bufptr = [&](int b, int fmt) -> char* {
-        if (b == BUF_IN) return (char*)in + (size_t)(t0 * c.n) * (size_t)elem_bytes(fmt);
-        if (b == BUF_OUT) return (char*)out + (size_t)(t0 * c.n) * (size_t)elem_bytes(fmt);
+        const size_t skip = gathered ? 0 : (size_t)(t0 * c.n) * (size_t)elem_bytes(fmt);
+        if (b == BUF_IN) return (char*)in + skip;
+        if (b == BUF_OUT) return (char*)out + skip;
         return (char*)plan->ws;
       };
       auto view_elems = [&](int b) -> long long {
-        return b == BUF_WS ? c.ws_elems : (c.batch - t0) * c.n;
+        if (b == BUF_WS) return c.ws_elems;
+        return gathered ? plan->gather_ext[b == BUF_IN ? 0 : 1] : (c.batch - t0) * c.n;
       };
       const int dst_fmt = (pp.dst == BUF_OUT) ? c.out_fmt : ws_fmt;
       PassArgs a;
       fill_pass_args(a, c, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
                      view_elems(pp.dst), d, plan->tw[d], plan->aops[d]);
+      if (gathered) {
+        const long long* const tab[2] = {plan->gather[0], plan->gather[1]};
+        gather_args(a, pp, tab, t0);
+      }
       fill_bfp(a, c, pi, t0, nt, plan->real_kind == 2, in_exp,
                BfpBuffers{plan->state, plan->slots, plan->tile_e, plan->out_exp});
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
@@ -1245,7 +1321,8 @@ mcfftStatus mcfft_plan_describe(mcfftPlan plan, char* buf, size_t len) {
     s = plan->has_queue ? plan->rcore.describe() : std::string("ragged items=0 (every item in a bulk plan)\n");
     for (const auto& b : plan->bulk) {
       const std::string d = b.plan->core.describe();
-      s += "  bulk in=" + std::to_string(b.in_off) + " out=" + std::to_string(b.out_off) + ": " +
+      s += (b.plan->gather[0] ? std::string("  bulk gathered: ")
+                              : "  bulk in=" + std::to_string(b.in_off) + " out=" + std::to_string(b.out_off) + ": ") +
            d.substr(0, d.find('\n')) + "\n";
     }
   } else {

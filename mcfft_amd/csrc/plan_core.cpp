@@ -286,6 +286,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   p = PlanCore();
   p.opts.outer_radix = 0;
   p.opts.fourstep_lg2 = 0;
+  p.opts.gathered = opt.gathered;
   if (rank < 1 || rank > 3 || batch < 1 || !dims) return ST_INVALID_VALUE;
   if (in_fmt < 0 || in_fmt > 2 || out_fmt < 0 || out_fmt > 2) return ST_INVALID_VALUE;
   if (scaling < 0 || scaling > 3) return ST_INVALID_VALUE;
@@ -370,7 +371,7 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW, scaling)) {  // single pass: 2^15, 2^16 in FP16/BF16
       p.chunk = batch;
       p.n_chunks = 1;
-      st = add_row(N, batch, BUF_IN, BUF_OUT, in_fmt, false);
+      st = add_row(N, opt.gathered ? 1 : batch, BUF_IN, BUF_OUT, in_fmt, opt.gathered);
     } else if (lg >= three_pass_min_lg()) {
       st = plan_three_pass(p, N, cm, scaling, ws_fmt, total_shift);
     } else {

@@ -71,6 +71,9 @@ struct PassPlan {
 struct PlanOpts {
   int outer_radix = -1;   // 2D: radix R of the outer-axis split (0: none; 2, 4, 8)
   int fourstep_lg2 = -1;  // 1D two-pass four-step: log2 N2 (points of the second pass's rows)
+  // gathered batch (transforms at arbitrary offsets, PassArgs::gather_*): every
+  // pass has per-transform groups (a 1D single-pass plan one row per group)
+  bool gathered = false;
 };
 
 struct PlanCore {

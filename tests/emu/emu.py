@@ -209,6 +209,31 @@ def tune_cache_lookup(text: str, build_id: str, key: str):
     return buf.value.decode() if rc == 0 else None
 
 
+def exec_gathered(buf_in, buf_out, dims, in_offs, out_offs, in_fmt, out_fmt, scaling=1, inverse=False, ws_limit=0):
+    """A gathered batched plan on storage arrays: transform t reads its
+    prod(dims) elements at element in_offs[t] of buf_in and writes them at
+    out_offs[t] of buf_out (the layout a ragged plan's same-shape items have).
+    Returns the per-transform output exponents."""
+    L = lib()
+    L.emu_set_gather.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                 ctypes.c_longlong]
+    io = np.ascontiguousarray(in_offs, dtype=np.int64)
+    oo = np.ascontiguousarray(out_offs, dtype=np.int64)
+    comp = lambda fmt: 4 if fmt == FMT_SPLIT else 2
+    L.emu_set_gather(io.ctypes.data, oo.ctypes.data, len(io), buf_in.size // comp(in_fmt),
+                     buf_out.size // comp(out_fmt))
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    e = ctypes.c_int(0)
+    try:
+        rc = L.emu_exec(len(dims), d, len(io), in_fmt, out_fmt, scaling, int(inverse), buf_in.ctypes.data,
+                        buf_out.ctypes.data, ws_limit, ctypes.byref(e))
+    finally:
+        L.emu_set_gather(None, None, 0, 0, 0)
+    if rc != 0:
+        raise RuntimeError(f"emu_exec status {rc}")
+    return transform_exponents(len(io))
+
+
 def ragged_exec(xs, shapes, in_fmt, out_fmt, layout=None, scaling=1, inverse=False, nblocks=3, single=False):
     """Ragged batch: xs = list of complex arrays, shapes = list of dims lists.
     layout: list of (in_off, out_off) element offsets (default back to back).

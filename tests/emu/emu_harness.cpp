@@ -134,6 +134,18 @@ extern "C" void emu_restrict(int pass, long long blk_first, long long blk_count)
 
 // plan structure of the next emu_exec calls (the autotuner's candidates; -1: defaults)
 static PlanOpts g_opts;
+// gathered batch for the next emu_exec calls (empty: transforms back to back):
+// per-transform element offsets into the input and output buffers, and the
+// buffers' element counts
+static std::vector<long long> g_gather[2];
+static long long g_gather_ext[2] = {0, 0};
+extern "C" void emu_set_gather(const long long* in_off, const long long* out_off, long long n, long long ext_in,
+                               long long ext_out) {
+  g_gather[0].assign(in_off, in_off + (in_off ? n : 0));
+  g_gather[1].assign(out_off, out_off + (out_off ? n : 0));
+  g_gather_ext[0] = ext_in;
+  g_gather_ext[1] = ext_out;
+}
 extern "C" void emu_set_structure(int outer_radix, int fourstep_lg2) {
   g_opts.outer_radix = outer_radix;
   g_opts.fourstep_lg2 = fourstep_lg2;
@@ -142,7 +154,11 @@ extern "C" void emu_set_structure(int outer_radix, int fourstep_lg2) {
 extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in_fmt, int out_fmt, int scaling,
                         int inverse, const void* in, void* out, long long ws_limit, int* out_exponent) {
   PlanCore p;
-  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit, g_opts);
+  const bool gathered = !g_gather[0].empty();
+  if (gathered && (long long)g_gather[0].size() != batch) return ST_INVALID_VALUE;
+  PlanOpts opts = g_opts;
+  opts.gathered = gathered;
+  int st = plan_build(p, rank, dims, batch, in_fmt, out_fmt, scaling, ws_limit, opts);
   if (st) return st;
   if (!g_kernel_override.empty())
     for (PassPlan& pp : p.passes)
@@ -161,8 +177,8 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
     aops_words(dft.data(), p.cm, lane, [&](int k, uint32_t w) { aops[lane * 4 + k / 4][k % 4] = w; });
   const int ws_fmt = (p.cm == 2) ? 2 : in_fmt;
   std::vector<unsigned char> ws((size_t)(p.ws_elems * elem_bytes(ws_fmt)) + 16);
-  const size_t in_bytes = (size_t)(p.n * batch * elem_bytes(in_fmt));
-  const size_t out_bytes = (size_t)(p.n * batch * elem_bytes(out_fmt));
+  const size_t in_bytes = (size_t)((gathered ? g_gather_ext[0] : p.n * batch) * elem_bytes(in_fmt));
+  const size_t out_bytes = (size_t)((gathered ? g_gather_ext[1] : p.n * batch) * elem_bytes(out_fmt));
   // bookkeeping buffers, as the runtime allocates them (all zero)
   ExecState state{0ull, 0u, 0u};
   std::vector<int> out_exp((size_t)batch, 0);
@@ -192,6 +208,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   emu::register_range(slots.data(), bfp_slot_count(p) * sizeof(unsigned long long));
   emu::register_range(tile_e.data(), bfp_tile_bytes(p));
   if (in_exp) emu::register_range(in_exp, g_in_exps.size() * sizeof(int));
+  for (int d = 0; d < 2 && gathered; ++d) emu::register_range(g_gather[d].data(), g_gather[d].size() * sizeof(long long));
   for (long long c = 0; c < p.n_chunks; ++c) {
     const long long t0 = c * p.chunk;
     const long long nt = (t0 + p.chunk <= batch) ? p.chunk : batch - t0;
@@ -199,17 +216,23 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       const PassPlan& pp = p.passes[pi];
       if (g_only_pass >= 0 && (int)pi != g_only_pass) continue;
       auto bufptr = [&](int b, int fmt) -> char* {
-        if (b == BUF_IN) return (char*)in + (size_t)(t0 * p.n * elem_bytes(fmt));
-        if (b == BUF_OUT) return (char*)out + (size_t)(t0 * p.n * elem_bytes(fmt));
+        const size_t skip = gathered ? 0 : (size_t)(t0 * p.n * elem_bytes(fmt));
+        if (b == BUF_IN) return (char*)in + skip;
+        if (b == BUF_OUT) return (char*)out + skip;
         return (char*)ws.data();
       };
       auto view_elems = [&](int b) -> long long {
-        return b == BUF_WS ? p.ws_elems : (p.batch - t0) * p.n;
+        if (b == BUF_WS) return p.ws_elems;
+        return gathered ? g_gather_ext[b == BUF_IN ? 0 : 1] : (p.batch - t0) * p.n;
       };
       const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
       PassArgs a;
       fill_pass_args(a, p, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
                      view_elems(pp.dst), inverse, tw.data(), aops.data());
+      if (gathered) {
+        const long long* const tab[2] = {g_gather[0].data(), g_gather[1].data()};
+        gather_args(a, pp, tab, t0);
+      }
       fill_bfp(a, p, (int)pi, t0, nt, false, in_exp, bufs);
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;
       const long long tiles = groups * pp.tiles_per_group;

@@ -160,3 +160,42 @@ def test_tune_cache_versioning():
     assert emu.tune_cache_lookup(text.replace("cache 2", "cache 1"), "abc123", key) is None  # another format
     assert emu.tune_cache_lookup(text, "abc123", key + "x") is None
     assert emu.tune_cache_lookup("mcfft-tune-cache 2 abc123\n" + key + "=K=a,b\n", "abc123", key) is None
+
+
+# ---------------- gathered batches (a ragged plan's same-shape items) ----------------
+GATHER_CASES = [([4096], 5, 3), ([1 << 17], 3, 0), ([1024, 512], 3, 0), ([64, 64, 32], 2, 3), ([8192], 3, 1)]
+
+
+@pytest.mark.parametrize("dims,count,scaling", GATHER_CASES)
+@pytest.mark.parametrize("fi,fo", [(emu.FMT_F16, emu.FMT_F16), (emu.FMT_F16, emu.FMT_SPLIT)])
+def test_gathered_batch(dims, count, scaling, fi, fo, monkeypatch):
+    """Transforms at scattered offsets (gaps, reversed order, input and output
+    placed differently) through one batched plan that locates them with offset
+    tables: each against the oracle, and nothing outside the transforms written."""
+    monkeypatch.delenv("MCFFT_2D_OUTER_RADIX", raising=False)
+    if scaling == AUTO:
+        scaling = 1   # ragged plans are statically scaled
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n + count)
+    slot_in = rng.permutation(count + 2)[:count]          # slots of n (+ a gap) elements
+    slot_out = rng.permutation(count + 1)[:count]
+    in_offs = slot_in * (n + 37)
+    out_offs = slot_out[::-1] * (n + 5)
+    x = rng.standard_normal((count, n)) + 1j * rng.standard_normal((count, n))
+    comp_in = 4 if fi == emu.FMT_SPLIT else 2
+    comp_out = 4 if fo == emu.FMT_SPLIT else 2
+    buf_in = np.zeros(((count + 2) * (n + 37)) * comp_in, dtype=np.uint16)
+    for t in range(count):
+        buf_in[in_offs[t] * comp_in:(in_offs[t] + n) * comp_in] = emu.to_storage(x[t], fi)
+    sentinel = 0x7BCD
+    buf_out = np.full(((count + 1) * (n + 5)) * comp_out, sentinel, dtype=np.uint16)
+    e = emu.exec_gathered(buf_in, buf_out, dims, in_offs, out_offs, fi, fo, scaling=scaling)
+    written = np.zeros(buf_out.size, dtype=bool)
+    for t in range(count):
+        sl = slice(out_offs[t] * comp_out, (out_offs[t] + n) * comp_out)
+        written[sl] = True
+        y = emu.from_storage(buf_out[sl], fo) * 2.0 ** e[t]
+        xq = emu.from_storage(emu.to_storage(x[t], fi), fi)
+        err = oracle.rel_l2(y, oracle.fft(xq, tuple(dims)))
+        assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (t, err)
+    assert np.all(buf_out[~written] == sentinel)

@@ -176,3 +176,46 @@ def test_graph_capture_replays(torch_cuda):
         assert torch.equal(d_out, ref), rep
         assert torch.equal(d_rout, rref), rep
     assert pr.status_flags() == 0
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "split"])
+def test_ragged_gathered_bulk_runs(torch_cuda, monkeypatch, fmt):
+    """Same-shape items scattered through the batch (drawn order, with gaps)
+    run as gathered batched plans when together they reach the bulk size
+    (MCFFT_RAGGED_BULK lowered to 2^18 here: the 8192- and 131072-point items);
+    items that reach it alone (the 2D and 3D ones) run as one-item batched plans;
+    short 1D rows stay in the queue.  Every item against the oracle; nothing
+    between items written."""
+    torch = torch_cuda
+    monkeypatch.setenv("MCFFT_RAGGED_BULK", str(1 << 18))
+    rng = np.random.default_rng(9)
+    shapes = [[8192]] * 40 + [[1 << 17]] * 3 + [[1024, 512]] * 2 + [[64, 64, 64]] * 2 + [[256]] * 30
+    rng.shuffle(shapes)
+    sizes = [int(np.prod(s)) for s in shapes]
+    gaps = rng.integers(0, 64, len(shapes))
+    offs, o = [], 0
+    for n, g in zip(sizes, gaps):
+        o += int(g)
+        offs.append(o)
+        o += n
+    total = o + 16
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    in_fmt, out_fmt = ("fp16", "split") if fmt == "split" else ("fp16", "fp16")
+    p = mc.RaggedPlan([(s, q, q) for s, q in zip(shapes, offs)], in_fmt, out_fmt)
+    desc = p.describe()
+    assert desc.count("bulk gathered") == 2 and desc.count("  bulk") == 6, desc
+    d_in = mc.to_device(x, in_fmt)
+    d_out = mc.to_device(np.full(total, 7.0 + 0j), out_fmt)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, out_fmt)
+    xq = mc.decode(mc.encode(x, in_fmt), in_fmt)
+    worst, covered = 0.0, np.zeros(total, dtype=bool)
+    for s, q, e in zip(shapes, offs, p.exponents):
+        n = int(np.prod(s))
+        covered[q:q + n] = True
+        worst = max(worst, oracle.rel_l2(y[q:q + n] * 2.0 ** e, oracle.fft(xq[q:q + n], tuple(s))))
+    print(f"ragged gathered {fmt}: {len(shapes)} items, worst rel_l2 {worst:.3e}")
+    assert worst <= TOL[fmt]
+    assert np.all(y[~covered] == 7.0)
