@@ -126,18 +126,27 @@ mcfftStatus mcfft_exec(mcfftPlan plan, const void* in, void* out, int direction,
  * be the same plan's own buffer).  No reference counterpart. */
 mcfftStatus mcfft_exec_exp(mcfftPlan plan, const void* in, const int* in_exp, void* out, int direction, void* stream);
 
-/* Real-input forward (R2C) and real-output inverse (C2R) 1D transforms; no
- * reference counterpart (the reference has only the forward C2C).
- * R2C: `batch` real sequences of length n (n a power of two, 32 <= n <= 2^24)
- *   -> n/2 + 1 complex outputs each (X[k], k = 0..n/2), exec with MCFFT_FORWARD.
- * C2R: n/2 + 1 complex inputs per transform (the non-redundant half of a
- *   Hermitian spectrum) -> n reals (unnormalised inverse), exec with MCFFT_INVERSE.
- * Real data is stored as pairs (x[2m], x[2m+1]) in the complex format of the
- * same name (FP16/BF16: 2 bytes per real; split: the split layout of the pair).
- * Computed as a length-n/2 complex FFT plus an FP32 pass with FP64-generated
- * twiddles W_n^k; the plan owns an n/2-element-per-transform buffer.  The C2R
+/* Real-input forward (R2C) and real-output inverse (C2R) transforms of rank 1
+ * to 3; no reference counterpart (the reference has only the forward C2C).
+ * The last dimension n is a power of two >= 32 (1D: n <= 2^28); the outer
+ * dimensions are those a complex plan of the same rank accepts.
+ * R2C: `batch` real arrays of shape dims -> outer x (n/2 + 1) complex outputs
+ *   each (X[m][k], k = 0..n/2), exec with MCFFT_FORWARD.
+ * C2R: outer x (n/2 + 1) complex inputs per transform (the non-redundant half of
+ *   a Hermitian spectrum) -> dims reals (unnormalised inverse), exec with
+ *   MCFFT_INVERSE.
+ * Real data is stored as pairs (x[2j], x[2j+1]) along the last dimension, in the
+ * complex format of the same name (FP16/BF16: 2 bytes per real; split: the split
+ * layout of the pair).  Computed as the complex FFT over (outer dims, n/2) plus
+ * an FP32 pass that pairs bin (m, k) with (-m, n/2 - k) (FP64-generated twiddles
+ * W_n^k); the plan owns an outer x n/2-element-per-transform buffer.  The C2R
  * plan reports one more power of two in its output exponent than the complex
- * plan of length n/2.  mcfft_exec_passes / mcfft_plan_tune do not apply. */
+ * plan.  mcfft_exec_passes / mcfft_plan_tune do not apply. */
+mcfftStatus mcfft_plan_r2c(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                           mcfftFormat out_fmt, mcfftScaling scaling);
+mcfftStatus mcfft_plan_c2r(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                           mcfftFormat out_fmt, mcfftScaling scaling);
+/* the rank-1 forms of mcfft_plan_r2c / mcfft_plan_c2r */
 mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt,
                               mcfftFormat out_fmt, mcfftScaling scaling);
 mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt,

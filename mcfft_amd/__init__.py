@@ -89,6 +89,8 @@ def lib() -> ctypes.CDLL:
         L.mcfft_exec_layout.argtypes = [vp, vp, vp, vp, vp, i, vp]
         L.mcfft_plan_r2c_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
         L.mcfft_plan_c2r_1d.argtypes = [ctypes.POINTER(vp), ll, ll, i, i, i]
+        L.mcfft_plan_r2c.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(ll), ll, i, i, i]
+        L.mcfft_plan_c2r.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(ll), ll, i, i, i]
         L.mcfft_get_item_exponents.argtypes = [vp, ctypes.POINTER(i)]
         L.mcfft_get_status_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.mcfft_plan_tune.argtypes = [vp, vp, vp, i, vp, i]
@@ -102,7 +104,7 @@ def lib() -> ctypes.CDLL:
                   "mcfft_get_status_flags", "mcfft_plan_tune", "mcfft_error_norms", "mcfft_error_norms_noalloc",
                   "mcfft_build_id",
                   "mcfft_plan_r2c_1d",
-                  "mcfft_plan_c2r_1d", "mcfft_plan_many_layout", "mcfft_exec_layout",
+                  "mcfft_plan_c2r_1d", "mcfft_plan_r2c", "mcfft_plan_c2r", "mcfft_plan_many_layout", "mcfft_exec_layout",
                   "mcfft_get_output_exponent", "mcfft_get_workspace_size", "mcfft_plan_describe", "mcfft_destroy"):
             getattr(L, f).restype = i
         _lib = L
@@ -351,28 +353,34 @@ class RaggedPlan(Plan):
 
 
 class RealPlan(Plan):
-    """1D real-input forward ("r2c": n reals -> n/2+1 complex per transform,
-    exec with FORWARD) or real-output inverse ("c2r": n/2+1 complex -> n reals,
-    exec with INVERSE).  Real data is stored as complex pairs (x[2m], x[2m+1])."""
+    """Real-input forward ("r2c": dims reals -> outer x (n/2+1) complex per
+    transform, exec with FORWARD) or real-output inverse ("c2r": outer x (n/2+1)
+    complex -> dims reals, exec with INVERSE); `dims` is n (1D) or the list of 1
+    to 3 dimensions, n the last.  Real data is stored as complex pairs
+    (x[2m], x[2m+1]) along the last dimension."""
 
-    def __init__(self, n: int, batch: int, kind: str = "r2c", in_fmt: str = "fp16", out_fmt: str | None = None,
+    def __init__(self, dims, batch: int, kind: str = "r2c", in_fmt: str = "fp16", out_fmt: str | None = None,
                  scaling: str = "auto"):
+        dims = [int(d) for d in (dims if hasattr(dims, "__len__") else [dims])]
         out_fmt = out_fmt or in_fmt
-        self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = [int(n)], int(batch), in_fmt, out_fmt, scaling
+        self.dims, self.batch, self.in_fmt, self.out_fmt, self.scaling = dims, int(batch), in_fmt, out_fmt, scaling
         self.kind = kind
         self._h = ctypes.c_void_p()
-        fn = lib().mcfft_plan_r2c_1d if kind == "r2c" else lib().mcfft_plan_c2r_1d
-        _check(fn(ctypes.byref(self._h), int(n), int(batch), FORMATS[in_fmt], FORMATS[out_fmt], SCALINGS[scaling]),
-               f"mcfft_plan_{kind}_1d")
+        arr = (ctypes.c_longlong * len(dims))(*dims)
+        fn = lib().mcfft_plan_r2c if kind == "r2c" else lib().mcfft_plan_c2r
+        _check(fn(ctypes.byref(self._h), len(dims), arr, int(batch), FORMATS[in_fmt], FORMATS[out_fmt],
+                  SCALINGS[scaling]), f"mcfft_plan_{kind}")
         self.exponent = _static_exponent(self._h, scaling)
-        self.n = int(n)
+        self.n = dims[-1]
+        self.rows = int(np.prod(dims[:-1])) if len(dims) > 1 else 1
 
     def output_elems_per_transform(self) -> int:
-        """complex elements per transform of the output (r2c: n/2 + 1; c2r: n/2 pairs)"""
-        return self.n // 2 + 1 if self.kind == "r2c" else self.n // 2
+        """complex elements per transform of the output (r2c: outer x (n/2 + 1);
+        c2r: outer x n/2 pairs)"""
+        return self.rows * (self.n // 2 + 1 if self.kind == "r2c" else self.n // 2)
 
     def exec_io_bytes(self):
-        h, c = self.n // 2, (self.n // 2 + 1)
+        h, c = self.rows * (self.n // 2), self.rows * (self.n // 2 + 1)
         if self.kind == "r2c":
             return h * self.batch * ELEM_BYTES[self.in_fmt], c * self.batch * ELEM_BYTES[self.out_fmt]
         return c * self.batch * ELEM_BYTES[self.in_fmt], h * self.batch * ELEM_BYTES[self.out_fmt]

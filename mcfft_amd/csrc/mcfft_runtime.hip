@@ -68,7 +68,9 @@ struct mcfftPlan_st {
   std::vector<int> item_exp;    // per item, from whichever path runs it
   // real-input / real-output plans: `core` is the length-H complex plan
   int real_kind = 0;          // 0 complex, 1 R2C, 2 C2R
-  long long real_h = 0;       // H = N / 2
+  long long real_h = 0;       // H = N / 2 (N: the last dimension)
+  long long real_rows = 1;    // rows of the last dimension per transform
+  long long real_outer[2] = {1, 1};  // outer dimensions (2D: NX, 1; 3D: NX, NY)
   void* real_tmp = nullptr;   // H complex per transform (C2C output / input)
   f32x2* real_tw[2] = {nullptr, nullptr};  // W_N^k, k = 0..H, per direction
   int real_tmp_fmt = 0;
@@ -471,27 +473,43 @@ mcfftStatus mcfft_plan_many(mcfftPlan* plan, int rank, const long long* dims, lo
 }
 
 // Real-input forward / real-output inverse plans (see include/mcfft.h).
-static mcfftStatus plan_real(mcfftPlan* plan, int kind, long long n, long long batch, mcfftFormat in_fmt,
-                             mcfftFormat out_fmt, mcfftScaling scaling) {
+static mcfftStatus plan_real(mcfftPlan* plan, int kind, int rank, const long long* dims, long long batch,
+                             mcfftFormat in_fmt, mcfftFormat out_fmt, mcfftScaling scaling) {
   if (!plan) return MCFFT_INVALID_VALUE;
   *plan = nullptr;
-  if (n < 32 || (n & (n - 1)) || batch < 1) return n < 1 || batch < 1 ? MCFFT_INVALID_VALUE : MCFFT_UNSUPPORTED;
+  if (rank < 1 || rank > 3 || !dims || batch < 1) return MCFFT_INVALID_VALUE;
+  long long rows = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (dims[d] < 1) return MCFFT_INVALID_VALUE;
+    if (d < rank - 1) rows *= dims[d];
+  }
+  const long long n = dims[rank - 1];
+  if (n < 32 || (n & (n - 1))) return MCFFT_UNSUPPORTED;
   const long long h = n / 2;
+  // the passes address a transform's rows x (H + 1) elements with 32-bit offsets
+  if (rows * (h + 1) * 8 > (1LL << 32)) return MCFFT_UNSUPPORTED;
   const int cm = compute_mode((int)in_fmt, (int)out_fmt);
   if (cm < 0) return MCFFT_UNSUPPORTED;
   // C2R keeps its pre-pass output in the complex plan's input format
   const int tmp_fmt = kind == 1 ? (int)out_fmt : (cm == CM_SPLIT ? FMT_SPLIT : (int)in_fmt);
+  long long cdims[3] = {dims[0], rank > 1 ? dims[1] : 1, rank > 2 ? dims[2] : 1};
+  cdims[rank - 1] = h;
   mcfftPlan p = nullptr;
-  mcfftStatus st = mcfft_plan_many(&p, 1, &h, batch, kind == 1 ? in_fmt : (mcfftFormat)tmp_fmt, out_fmt, scaling);
+  mcfftStatus st =
+      mcfft_plan_many(&p, rank, cdims, batch, kind == 1 ? in_fmt : (mcfftFormat)tmp_fmt, out_fmt, scaling);
   if (st != MCFFT_SUCCESS) return st;
   p->real_kind = kind;
   p->real_h = h;
+  p->real_rows = rows;
+  p->real_outer[0] = rank > 1 ? dims[0] : 1;
+  p->real_outer[1] = rank > 2 ? dims[1] : 1;
   p->real_tmp_fmt = tmp_fmt;
   p->real_in_fmt = (int)in_fmt;
   int n_cu = 0;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) st = MCFFT_HIP_ERROR;
   p->real_grid = 8 * (n_cu > 0 ? n_cu : 1);
-  if (st == MCFFT_SUCCESS && hipMalloc(&p->real_tmp, (size_t)(h * batch) * (size_t)elem_bytes(tmp_fmt)) != hipSuccess)
+  if (st == MCFFT_SUCCESS &&
+      hipMalloc(&p->real_tmp, (size_t)(rows * h * batch) * (size_t)elem_bytes(tmp_fmt)) != hipSuccess)
     st = MCFFT_ALLOC_FAILED;
   // W_N^k, k = 0..H, generated in FP64 (forward and inverse signs)
   TwSpec* dspec = nullptr;
@@ -523,12 +541,22 @@ static mcfftStatus plan_real(mcfftPlan* plan, int kind, long long n, long long b
 
 mcfftStatus mcfft_plan_r2c_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt,
                               mcfftScaling scaling) {
-  return plan_real(plan, 1, n, batch, in_fmt, out_fmt, scaling);
+  return plan_real(plan, 1, 1, &n, batch, in_fmt, out_fmt, scaling);
 }
 
 mcfftStatus mcfft_plan_c2r_1d(mcfftPlan* plan, long long n, long long batch, mcfftFormat in_fmt, mcfftFormat out_fmt,
                               mcfftScaling scaling) {
-  return plan_real(plan, 2, n, batch, in_fmt, out_fmt, scaling);
+  return plan_real(plan, 2, 1, &n, batch, in_fmt, out_fmt, scaling);
+}
+
+mcfftStatus mcfft_plan_r2c(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                           mcfftFormat out_fmt, mcfftScaling scaling) {
+  return plan_real(plan, 1, rank, dims, batch, in_fmt, out_fmt, scaling);
+}
+
+mcfftStatus mcfft_plan_c2r(mcfftPlan* plan, int rank, const long long* dims, long long batch, mcfftFormat in_fmt,
+                           mcfftFormat out_fmt, mcfftScaling scaling) {
+  return plan_real(plan, 2, rank, dims, batch, in_fmt, out_fmt, scaling);
 }
 
 mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* dims, long long batch,
@@ -894,8 +922,11 @@ static mcfftStatus exec_real(mcfftPlan plan, const void* in, void* out, int dire
   RealArgs ra;
   ra.h = plan->real_h;
   ra.batch = plan->core.batch;
+  ra.rows = plan->real_rows;
+  ra.ox = plan->real_outer[0];
+  ra.oy = plan->real_outer[1];
   ra.tw = plan->real_tw[direction == MCFFT_FORWARD ? 0 : 1];
-  const long long total = plan->real_kind == 1 ? (ra.h / 2 + 1) * ra.batch : ra.h * ra.batch;
+  const long long total = (plan->real_kind == 1 ? (ra.h / 2 + 1) : ra.h) * ra.rows * ra.batch;
   const long long need = (total + 1023) / 1024;
   const unsigned grid = (unsigned)(need < plan->real_grid ? need : plan->real_grid);
   void* args[] = {&ra};
@@ -1303,7 +1334,8 @@ mcfftStatus mcfft_get_workspace_size(mcfftPlan plan, size_t* bytes) {
   if (!plan) return MCFFT_INVALID_PLAN;
   if (!bytes) return MCFFT_INVALID_VALUE;
   size_t b = plan->ws_bytes;
-  if (plan->real_kind) b += (size_t)(plan->real_h * plan->core.batch) * (size_t)elem_bytes(plan->real_tmp_fmt);
+  if (plan->real_kind)
+    b += (size_t)(plan->real_rows * plan->real_h * plan->core.batch) * (size_t)elem_bytes(plan->real_tmp_fmt);
   b += plan->lay_bytes;
   for (const auto& bk : plan->bulk) {
     size_t sb = 0;

@@ -341,6 +341,41 @@ def real_exec(kind: str, data, n: int, batch: int, in_fmt: int, out_fmt: int, sc
     return r
 
 
+def real_exec_nd(kind: str, data, dims, batch: int, in_fmt: int, out_fmt: int, scaling: int = 1):
+    """Real transforms of shape `dims` (last dimension n): kind "r2c": data =
+    batch x dims reals -> batch x outer x (n/2+1) complex; kind "c2r": data =
+    batch x outer x (n/2+1) complex -> batch x dims reals.  True units (output
+    exponents applied)."""
+    dims = list(dims)
+    n, h = dims[-1], dims[-1] // 2
+    outer = int(np.prod(dims[:-1])) if len(dims) > 1 else 1
+    if kind == "r2c":
+        x = np.asarray(data, dtype=np.float64).reshape(batch * outer, n)
+        src = to_storage((x[:, 0::2] + 1j * x[:, 1::2]).reshape(-1), in_fmt)
+        out = np.zeros(outer * (h + 1) * batch * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    else:
+        src = to_storage(np.asarray(data, dtype=np.complex128).reshape(-1), in_fmt)
+        out = np.zeros(outer * h * batch * (4 if out_fmt == FMT_SPLIT else 2), dtype=np.uint16)
+    e = ctypes.c_int(0)
+    L = lib()
+    L.emu_real_exec_nd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_int)]
+    d = (ctypes.c_longlong * len(dims))(*dims)
+    rc = L.emu_real_exec_nd(1 if kind == "r2c" else 2, len(dims), d, batch, in_fmt, out_fmt, scaling,
+                            src.ctypes.data, out.ctypes.data, ctypes.byref(e))
+    if rc != 0:
+        raise RuntimeError(f"emu_real_exec_nd status {rc}")
+    y, te = _scaled(from_storage(out, out_fmt), batch)
+    _check_static(te, e.value, scaling)
+    if kind == "r2c":
+        return y.reshape([batch] + dims[:-1] + [h + 1])
+    z = y.reshape(batch * outer, h)
+    r = np.empty((batch * outer, n))
+    r[:, 0::2], r[:, 1::2] = z.real, z.imag
+    return r.reshape([batch] + dims)
+
+
 def layout_encode(x, fmt: int, planar: int, positions, extent: int, fill: int = 0x7E00):
     """Place complex values x[k] at element positions[k] of a caller-side buffer
     of `extent` elements: interleaved -> one uint16 array (2 or 4 components per

@@ -427,13 +427,23 @@ extern "C" void mcfft_c2r_pre_0(RealArgs a);
 extern "C" void mcfft_c2r_pre_1(RealArgs a);
 extern "C" void mcfft_c2r_pre_2(RealArgs a);
 
-extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt, int out_fmt, int scaling,
-                             const void* in, void* out, int* out_exponent) {
-  const long long h = n / 2;
+// rank 1..3, dims: the real shape (last dimension n even); the complex plan runs
+// over (outer dims, n/2)
+extern "C" int emu_real_exec_nd(int kind, int rank, const long long* dims, long long batch, int in_fmt, int out_fmt,
+                                int scaling, const void* in, void* out, int* out_exponent) {
+  if (rank < 1 || rank > 3) return ST_INVALID_VALUE;
+  const long long n = dims[rank - 1], h = n / 2;
+  long long rows = 1, cdims[3] = {1, 1, 1};
+  for (int d = 0; d < rank; ++d) {
+    cdims[d] = dims[d];
+    if (d < rank - 1) rows *= dims[d];
+  }
+  cdims[rank - 1] = h;
   const int cm = compute_mode(in_fmt, out_fmt);
   if (cm < 0 || n < 32 || (n & (n - 1))) return ST_UNSUPPORTED;
   const int tmp_fmt = kind == 1 ? out_fmt : (cm == 2 ? 2 : in_fmt);
-  std::vector<unsigned char> tmp((size_t)(h * batch * elem_bytes(tmp_fmt)) + 16);
+  const long long tmp_elems = rows * h * batch;
+  std::vector<unsigned char> tmp((size_t)(tmp_elems * elem_bytes(tmp_fmt)) + 16);
   const int sign = kind == 1 ? -1 : 1;
   std::vector<f32x2> tw((size_t)h + 1);
   for (long long k = 0; k <= h; ++k) twiddle_host(TwSpec{k, n, 1.0f}, sign, tw[(size_t)k]);
@@ -442,8 +452,11 @@ extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt,
   RealArgs ra;
   ra.h = h;
   ra.batch = batch;
+  ra.rows = rows;
+  ra.ox = rank > 1 ? dims[0] : 1;
+  ra.oy = rank > 2 ? dims[1] : 1;
   ra.tw = tw.data();
-  const long long total = kind == 1 ? (h / 2 + 1) * batch : h * batch;
+  const long long total = (kind == 1 ? (h / 2 + 1) : h) * rows * batch;
   const int grid = (int)std::min<long long>((total + 1023) / 1024, 3);
   int e = 0;
   auto regs = [&](const void* a, size_t abytes, const void* b, size_t bbytes) {
@@ -452,27 +465,34 @@ extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt,
     emu::register_range(b, bbytes);
     emu::register_range(tw.data(), tw.size() * sizeof(f32x2));
   };
+  const size_t spec_bytes_in = (size_t)(rows * (h + 1) * batch * elem_bytes(in_fmt));
+  const size_t spec_bytes_out = (size_t)(rows * (h + 1) * batch * elem_bytes(out_fmt));
   if (kind == 1) {
-    int st = emu_exec(1, &h, batch, in_fmt, out_fmt, scaling, 0, in, tmp.data(), 0, &e);
+    int st = emu_exec(rank, cdims, batch, in_fmt, out_fmt, scaling, 0, in, tmp.data(), 0, &e);
     if (st) return st;
-    regs(tmp.data(), (size_t)(h * batch * elem_bytes(tmp_fmt)), out, (size_t)((h + 1) * batch * elem_bytes(out_fmt)));
+    regs(tmp.data(), (size_t)(tmp_elems * elem_bytes(tmp_fmt)), out, spec_bytes_out);
     ra.in = tmp.data();
     ra.out = out;
     ra.in_fmt = out_fmt;
     emu::launch(grid, 1024, 0, [&] { post[cm](ra); });
   } else {
-    regs(in, (size_t)((h + 1) * batch * elem_bytes(in_fmt)), tmp.data(), (size_t)(h * batch * elem_bytes(tmp_fmt)));
+    regs(in, spec_bytes_in, tmp.data(), (size_t)(tmp_elems * elem_bytes(tmp_fmt)));
     ra.in = in;
     ra.out = tmp.data();
     ra.in_fmt = in_fmt;
     emu::launch(grid, 1024, 0, [&] { pre[cm](ra); });
-    int st = emu_exec(1, &h, batch, tmp_fmt, out_fmt, scaling, 1, tmp.data(), out, 0, &e);
+    int st = emu_exec(rank, cdims, batch, tmp_fmt, out_fmt, scaling, 1, tmp.data(), out, 0, &e);
     if (st) return st;
     e += 1;
     for (int& v : g_texps) v += 1;  // the pre-pass halves its input
   }
   if (out_exponent) *out_exponent = e;
   return 0;
+}
+
+extern "C" int emu_real_exec(int kind, long long n, long long batch, int in_fmt, int out_fmt, int scaling,
+                             const void* in, void* out, int* out_exponent) {
+  return emu_real_exec_nd(kind, 1, &n, batch, in_fmt, out_fmt, scaling, in, out, out_exponent);
 }
 
 // ---- advanced layouts: gather -> transform -> scatter, as the runtime's

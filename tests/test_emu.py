@@ -275,6 +275,35 @@ def test_c2r(n, batch, fi, fo):
     assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (n, err)
 
 
+@pytest.mark.parametrize("dims,batch", [([16, 64], 2), ([64, 32], 1), ([16, 16, 64], 2), ([16, 32, 32], 1),
+                                        ([512, 256], 1)])
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_real_nd(dims, batch, fi, fo):
+    """2D/3D R2C against the oracle's complex FFT of the real input (the first
+    n/2 + 1 bins of the last dimension), and C2R of that quantised half spectrum
+    against numpy's irfftn (the inverse DFT of its Hermitian extension): the
+    mirror-row pairing (-m, n/2 - k) over every outer dimension, odd and even
+    rows, DC and Nyquist bins."""
+    n, h = dims[-1], dims[-1] // 2
+    axes = tuple(range(1, len(dims) + 1))
+    rng = np.random.default_rng(sum(dims) + batch)
+    x = rng.standard_normal([batch] + dims)
+    pairs = (x.reshape(-1, n)[:, 0::2] + 1j * x.reshape(-1, n)[:, 1::2]).reshape(-1)
+    zq = emu.from_storage(emu.to_storage(pairs, fi), fi).reshape(-1, h)
+    xq = np.empty((zq.shape[0], n))
+    xq[:, 0::2], xq[:, 1::2] = zq.real, zq.imag
+    xq = xq.reshape([batch] + dims)
+    y = emu.real_exec_nd("r2c", x, dims, batch, fi, fo)
+    ref = oracle.fft(xq.reshape(-1).astype(np.complex128), tuple(dims)).reshape([batch] + dims)[..., : h + 1]
+    tol = TOL[fo if fo == emu.FMT_SPLIT else fi]
+    assert oracle.rel_l2(y.reshape(-1), ref.reshape(-1)) <= tol
+    spec = np.fft.rfftn(x, axes=axes) / np.sqrt(np.prod(dims))
+    sq = emu.from_storage(emu.to_storage(spec.reshape(-1), fi), fi).reshape(spec.shape)
+    back = emu.real_exec_nd("c2r", spec, dims, batch, fi, fo)
+    refc = np.fft.irfftn(sq, s=dims, axes=axes) * np.prod(dims)
+    assert oracle.rel_l2(back.reshape(-1).astype(np.complex128), refc.reshape(-1).astype(np.complex128)) <= tol
+
+
 # ---- advanced layouts (strided / planar sides staged by gather and scatter) ----
 
 def _dense(dims):

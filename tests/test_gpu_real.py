@@ -1,5 +1,5 @@
-"""GPU: real-input forward (R2C) and real-output inverse (C2R) 1D transforms
-against the FP64 oracle (the complex FFT of the real input / of the
+"""GPU: real-input forward (R2C) and real-output inverse (C2R) transforms, 1D,
+2D and 3D, against the FP64 oracle (the complex FFT of the real input / of the
 Hermitian-extended spectrum)."""
 import numpy as np
 import pytest
@@ -92,3 +92,77 @@ def test_real_plan_exponent_queries_agree(torch_cuda):
         assert p.exponent == q.exponent + (1 if kind == "c2r" else 0)
         p.destroy()
         q.destroy()
+
+
+ND = [([1024, 1024], 2), ([64, 64, 128], 2), ([4096, 512], 1), ([32, 4096], 3)]
+
+
+@pytest.mark.parametrize("dims,batch", ND)
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_r2c_nd(torch_cuda, dims, batch, fi, fo):
+    """2D/3D R2C against the FP64 oracle's complex FFT of the real input (the
+    non-redundant half of the last dimension)."""
+    torch = torch_cuda
+    n, h = dims[-1], dims[-1] // 2
+    rng = np.random.default_rng(sum(dims) + batch)
+    x = rng.standard_normal([batch] + dims)
+    z = mc.real_to_pairs(x)
+    xq = mc.pairs_to_real(mc.decode(mc.encode(z, fi), fi), n).reshape([batch] + dims)
+    p = mc.RealPlan(dims, batch, "r2c", fi, fo)
+    d_in = mc.to_device(z, fi)
+    d_out = mc.empty(p.output_elems_per_transform() * batch, fo)
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    y = p.result(d_out)
+    p.destroy()
+    ref = oracle.fft(xq.reshape(-1).astype(np.complex128), tuple(dims)).reshape([batch] + dims)[..., : h + 1]
+    err = oracle.rel_l2(y, ref.reshape(-1))
+    print(f"r2c {dims} b={batch} {fi}->{fo} rel_l2={err:.3e}")
+    assert err <= TOL[fo if fo == "split" else fi]
+
+
+@pytest.mark.parametrize("dims,batch", ND)
+@pytest.mark.parametrize("fi,fo", FMTS)
+def test_c2r_nd(torch_cuda, dims, batch, fi, fo):
+    """2D/3D C2R of a Hermitian spectrum (the half spectrum of a real array)
+    against the oracle's inverse complex FFT of the full quantised spectrum."""
+    torch = torch_cuda
+    n, h = dims[-1], dims[-1] // 2
+    axes = tuple(range(1, len(dims) + 1))
+    rng = np.random.default_rng(sum(dims) + 7)
+    spec = np.fft.rfftn(rng.standard_normal([batch] + dims), axes=axes) / np.sqrt(np.prod(dims))
+    sq = mc.decode(mc.encode(spec.reshape(-1), fi), fi).reshape(spec.shape)
+    ref = np.fft.irfftn(sq, s=dims, axes=axes) * np.prod(dims)   # the Hermitian extension's inverse DFT
+    p = mc.RealPlan(dims, batch, "c2r", fi, fo)
+    d_in = mc.to_device(spec.reshape(-1), fi)
+    d_out = mc.empty(p.output_elems_per_transform() * batch, fo)
+    p.exec(d_in, d_out, mc.INVERSE)
+    torch.cuda.synchronize()
+    y = mc.pairs_to_real(p.result(d_out), n)
+    p.destroy()
+    err = oracle.rel_l2(y.reshape(-1).astype(np.complex128), ref.reshape(-1).astype(np.complex128))
+    print(f"c2r {dims} b={batch} {fi}->{fo} rel_l2={err:.3e}")
+    assert err <= TOL[fo if fo == "split" else fi]
+
+
+def test_r2c_c2r_nd_round_trip(torch_cuda):
+    """c2r(r2c(x)) = prod(dims) x for a 2D real array, split both ways, exponents
+    carried from the forward plan."""
+    torch = torch_cuda
+    dims, batch = [2048, 1024], 2
+    n = dims[-1]
+    x = np.random.default_rng(12).standard_normal([batch] + dims)
+    z = mc.real_to_pairs(x)
+    xq = mc.pairs_to_real(mc.decode(mc.encode(z, "fp16"), "fp16"), n)
+    pf = mc.RealPlan(dims, batch, "r2c", "fp16", "split")
+    pi = mc.RealPlan(dims, batch, "c2r", "split", "split")
+    d_in = mc.to_device(z, "fp16")
+    d_mid = mc.empty(pf.output_elems_per_transform() * batch, "split")
+    d_out = mc.empty(pi.output_elems_per_transform() * batch, "split")
+    pf.exec(d_in, d_mid, mc.FORWARD)
+    pi.exec(d_mid, d_out, mc.INVERSE, in_exp=pf)
+    torch.cuda.synchronize()
+    y = mc.pairs_to_real(pi.result(d_out), n) / np.prod(dims)
+    pf.destroy()
+    pi.destroy()
+    assert oracle.rel_l2(y.reshape(-1).astype(np.complex128), xq.reshape(-1).astype(np.complex128)) <= 1e-6
