@@ -186,18 +186,25 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
 mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
                               int direction, void* stream);
 
-/* Ragged batch: n_items transforms of any supported shapes (rank 1..3; 1D
- * lengths 16..2^18, sides 16..4096), each at its own element offset in the
- * exec input and output buffers, run by one persistent kernel launch per pass
- * level (at most three: every item's first passes, then second, then third)
- * over a device-side work queue, plus a tiny queue-reset kernel; the whole batch
- * is one graph-capturable exec.  Back-to-back 1D items of one length up to
- * 4096 are processed as one multi-row unit.  A run of consecutive items of one
- * shape lying back to back in both buffers with at least 2^23 elements in all
- * (enough to fill the GPU; environment MCFFT_RAGGED_BULK=<elements> changes
- * the bound, 0 turns it off) is run as one batched plan instead, launched on
- * the same stream before the queue.  Items may have different output
- * exponents: see mcfft_get_item_exponents.  No reference counterpart. */
+/* Ragged batch: n_items transforms of any shapes mcfft_plan_many accepts, each
+ * at its own element offset in the exec input and output buffers; the whole
+ * batch is one graph-capturable exec on one stream.
+ * - Batched runs: a run of consecutive items of one shape lying back to back in
+ *   both buffers with at least 2^23 elements in all (enough to fill the GPU;
+ *   environment MCFFT_RAGGED_BULK=<elements> changes the bound, 0 turns it off)
+ *   runs as one batched plan.  So do the other items of one shape wherever they
+ *   lie (a gathered batch: the plan locates its transforms through per-item
+ *   offset tables) when they reach the bound together -- 2D/3D items and 1D
+ *   items of 4096 points or more -- and, whatever their size, items of shapes
+ *   the queue below cannot run (1D lengths above 2^18, outer axes narrower than
+ *   the queue's column tiles).
+ * - The rest (1D lengths 16..2^18, sides 16..4096) run by one persistent kernel
+ *   launch per pass level (at most three: every item's first passes, then
+ *   second, then third) over a device-side work queue, plus a tiny queue-reset
+ *   kernel.  Back-to-back 1D items of one length up to 4096 are processed as one
+ *   multi-row unit.
+ * Items may have different output exponents: see mcfft_get_item_exponents.  No
+ * reference counterpart. */
 typedef struct mcfftRaggedItem_t {
   int rank;
   long long dims[3];

@@ -707,8 +707,10 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
   // least bulk_min elements together, run as one batched plan whose transforms
   // are located through offset tables -- 2D/3D items and 1D items of >= 4096
   // points (whose row tiles hold one transform; shorter rows share tiles, so
-  // they stay in the queue unless they lie back to back).
-  if (bulk_min > 0 && !spec.empty()) {
+  // they stay in the queue unless they lie back to back).  Items of a shape the
+  // queue's pass variants cannot run (1D above 2^18 points, outer axes narrower
+  // than a variant's column tile) run this way whatever their size.
+  if (!spec.empty()) {
     long long ext[2] = {0, 0};
     for (int i = 0; i < n_items; ++i) {
       long long n = 1;
@@ -725,7 +727,8 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
       const RaggedItemSpec& a = spec[kv.second.front()];
       const long long cnt = (long long)kv.second.size();
       const long long n = kv.first[1] * kv.first[2] * kv.first[3];
-      if ((a.rank == 1 && a.dims[0] < 4096) || cnt * n < bulk_min) continue;
+      const bool queue_ok = ragged_shape_supported(a.rank, a.dims, (int)in_fmt, (int)out_fmt, (int)scaling) == ST_OK;
+      if (queue_ok && ((a.rank == 1 && a.dims[0] < 4096) || bulk_min <= 0 || cnt * n < bulk_min)) continue;
       PlanOpts o;
       o.gathered = true;
       const long long dims[3] = {a.dims[0], a.dims[1], a.dims[2]};

@@ -272,6 +272,20 @@ int ragged_build(RaggedPlanCore& r, int n_items, const RaggedItemSpec* items, in
   return ST_OK;
 }
 
+int ragged_shape_supported(int rank, const long long* dims, int in_fmt, int out_fmt, int scaling) {
+  RaggedPlanCore r;
+  r.cm = compute_mode(in_fmt, out_fmt);
+  if (r.cm < 0 || rank < 1 || rank > 3) return ST_UNSUPPORTED;
+  r.in_fmt = in_fmt;
+  r.out_fmt = out_fmt;
+  r.scaling = scaling;
+  const long long d[3] = {dims[0], rank > 1 ? dims[1] : 1, rank > 2 ? dims[2] : 1};
+  for (int i = 0; i < rank; ++i)
+    if (ilog2(d[i]) < 0) return ST_UNSUPPORTED;
+  ShapePasses sp;
+  return build_shape(r, rank, d, 1, sp);
+}
+
 std::string RaggedPlanCore::describe() const {
   std::ostringstream o;
   o << "ragged items=" << n_items << " passes=" << passes.size() << " tasks=" << task_pass.size()

@@ -42,5 +42,9 @@ struct RaggedPlanCore {
 // Items whose 1D length is at most 4096 and that sit back to back in both
 // buffers with the same length are fused into one multi-row pass.
 int ragged_build(RaggedPlanCore& r, int n_items, const RaggedItemSpec* items, int in_fmt, int out_fmt, int scaling);
+// ST_OK if the queue's pass variants can run a transform of this shape (the
+// others -- 1D above 2^18 points, outer axes narrower than a variant's column
+// tile -- need a batched plan)
+int ragged_shape_supported(int rank, const long long* dims, int in_fmt, int out_fmt, int scaling);
 
 }  // namespace mcfft

@@ -219,3 +219,29 @@ def test_ragged_gathered_bulk_runs(torch_cuda, monkeypatch, fmt):
     print(f"ragged gathered {fmt}: {len(shapes)} items, worst rel_l2 {worst:.3e}")
     assert worst <= TOL[fmt]
     assert np.all(y[~covered] == 7.0)
+
+
+def test_ragged_shapes_outside_the_queue(torch_cuda):
+    """Shapes the queue's pass variants cannot run -- narrow outer axes (256 x 16,
+    512 x 16, 128 x 64 x 16) and a 1D length above 2^18 -- run as gathered
+    batched plans inside the same ragged exec, beside queue items."""
+    torch = torch_cuda
+    shapes = [[256, 16], [1024], [512, 16], [1 << 20], [256, 16], [128, 64, 16], [4096], [64]]
+    sizes = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    p = mc.RaggedPlan([(s, o, o) for s, o in zip(shapes, offs)], "fp16", "fp16")
+    desc = p.describe()
+    assert desc.count("bulk gathered") == 4, desc
+    d_in = mc.to_device(x, "fp16")
+    d_out = mc.empty(total, "fp16")
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, "fp16")
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    worst = max(oracle.rel_l2(y[o:o + n] * 2.0 ** e, oracle.fft(xq[o:o + n], tuple(s)))
+                for s, o, n, e in zip(shapes, offs, sizes, p.exponents))
+    assert worst <= TOL["fp16"], worst
