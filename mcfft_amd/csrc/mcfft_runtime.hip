@@ -1388,6 +1388,6 @@ const char* mcfft_status_string(mcfftStatus s) {
   return "unknown status";
 }
 
-int mcfft_version(void) { return 100; }
+int mcfft_version(void) { return 200; }
 
 }  // extern "C"

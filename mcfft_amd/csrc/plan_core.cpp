@@ -141,14 +141,15 @@ std::vector<int> add_stage_tables(std::vector<TwSpec>& tw, PassPlan& pp, int S, 
 // (4 or 8 bytes) columns: under 32 (FP16/BF16) or 16 (split) columns, i.e.
 // row segments under 128 bytes, from 1024 (FP16/BF16) or 2048 (split) points.
 // Measured on MI355X (NX, NY in 1024..4096, all formats): R = 4 is the fastest
-// or within 4 % of it everywhere except 4096-point rows, whose 4-row tiles hold
-// 164-180 VGPRs (FP16/BF16) or 139 KiB of LDS (split): R = 2 there (4096^2 x 4
-// FP16 0.290 -> 0.281 ms, split 0.587 -> 0.533).  ROWX rows need at least 512
-// points.  `forced` >= 0 (a tuned or requested structure) replaces
+// or within 4 % of it everywhere except NX = NY = 4096, where the 4-row tiles of
+// 4096-point rows (164-194 VGPRs FP16/BF16, 139 KiB of LDS split: one
+// workgroup per CU) meet the 1024-point column pass (128 KiB tiles): R = 2 there
+// (4096^2 x 4 FP16 0.290 -> 0.281 ms, split 0.587 -> 0.533; 1024 x 4096 stays at
+// R = 4, 0.259 against 0.305 ms).  ROWX rows need at least 512 points.  `forced` >= 0 (a tuned or requested structure) replaces
 // the choice; MCFFT_2D_OUTER_RADIX overrides both.
 static int outer_radix(long long NX, long long NY, int forced) {
   int r = 0;
-  if (NX >= kOuterSplitMin && NY >= 512) r = NY >= 4096 ? 2 : 4;
+  if (NX >= kOuterSplitMin && NY >= 512) r = (NX >= 4096 && NY >= 4096) ? 2 : 4;
   if (forced >= 0) r = forced;
   if (const char* e = getenv("MCFFT_2D_OUTER_RADIX")) {
     const int v = atoi(e);

@@ -108,7 +108,7 @@ def test_rowx_default_structure(monkeypatch):
     monkeypatch.delenv("MCFFT_2D_OUTER_RADIX", raising=False)
     F16, SPL = emu.FMT_F16, emu.FMT_SPLIT
     cases = [((4096, 4096), F16, F16, 2), ((4096, 4096), F16, SPL, 2), ((2048, 2048), F16, SPL, 4),
-             ((1024, 1024), F16, F16, 4), ((1024, 4096), F16, F16, 2), ((512, 512), F16, F16, 0),
+             ((1024, 1024), F16, F16, 4), ((1024, 4096), F16, F16, 4), ((512, 512), F16, F16, 0),
              ((4096, 256), F16, F16, 0)]
     for dims, fi, fo, r in cases:
         d = emu.describe(list(dims), 1, fi, fo, AUTO)
