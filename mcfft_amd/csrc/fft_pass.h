@@ -64,9 +64,9 @@ enum { KIND_ROW = 0, KIND_ROWT = 1, KIND_COL = 2, KIND_COLF = 3, KIND_COLM = 4, 
 // r1, and before the row FFT it takes the R-point DFT across them (the outer
 // axis's first factor) times W_NX^(r1*k2), storing result k2 as row R*r1 + k2.
 // The column pass then has NX/R-point columns at stride R rows.  Q = R and the
-// waves cover exactly one row per MFMA tile set (NW = L/512), so a thread's R
-// MFMA tiles are the R rows at the same positions and the butterfly needs no
-// exchange.
+// waves' MFMA tiles cover a row in C = L/(512 NW) sets of 32 columns per wave,
+// so a thread's tiles k = C*q + c are row q at the same positions for every c
+// and the butterfly needs no exchange.
 struct PassArgs {
   const void* in;
   void* out;
@@ -371,8 +371,10 @@ struct PassCfg {
   static_assert(S >= 1, "at least one radix-16 stage");
   static_assert((Q * L) % 512 == 0, "tile must hold a multiple of 32 sixteen-point columns");
   static_assert(T == 1 || T == 2 || T == 4 || T == 8, "tail radix");
-  // ROWX: wave w's MFMA tile k is instance k at columns 32w..32w+31 (one row per NW tiles)
-  static_assert(!XROWS || ((Q == 2 || Q == 4 || Q == 8) && NW * 512 == L), "ROWX tile shape");
+  // ROWX: wave w's MFMA tile k = XC*q + c is row q at columns 32 (w + NW c) .. +31
+  static constexpr int XC = L / (512 * NW);
+  static_assert(!XROWS || ((Q == 2 || Q == 4 || Q == 8) && XC >= 1 && XC * 512 * NW == L && TPW == Q * XC),
+                "ROWX tile shape");
 
   static MCFFT_DEV int lidx(int q, int m) {
     if constexpr (QFAST) return m * Q + q;
@@ -501,8 +503,8 @@ MCFFT_DEV void rescale_stage0(const PassArgs& a, int wave, int lane, int grp, in
 // ROWX, after the stage-0 loads: the radix-Q DFT across the tile's rows (the
 // outer axis's first factor, in FP32), result k2 times W_NX^(r1*k2), the
 // butterfly's stage scale and the AUTO normalisation `nsc`, rounded once.  The
-// thread's MFMA tile k holds row k at the same positions (PassCfg::XROWS), so
-// the butterfly stays in registers.  Row r1 = the tile's index in its group;
+// thread's MFMA tiles XC*k + c hold row k at the same positions (PassCfg::XROWS),
+// so the butterfly stays in registers.  Row r1 = the tile's index in its group;
 // NX = Q * tiles_per_group.
 template <class C, class Tr>
 MCFFT_DEV void cross_rows(const PassArgs& a, int r1, bool inv, float nsc, typename C::El (*x)[8]) {
@@ -516,12 +518,13 @@ MCFFT_DEV void cross_rows(const PassArgs& a, int r1, bool inv, float nsc, typena
     const float inv_nx = 1.0f / (float)(R * a.tiles_per_group);
     for (int k = 0; k < R; ++k) w[k] = twiddle_rev(r1 * k, inv_nx, inv, sc);
   }
-  for (int t = 0; t < 8; ++t) {
-    f32x2 v[R];
-    for (int k = 0; k < R; ++k) v[k] = Tr::unpack(x[k][t]);
-    dft_small<R>(v, inv);
-    for (int k = 0; k < R; ++k) x[k][t] = Tr::pack(cmul(v[k], w[k]));
-  }
+  for (int c = 0; c < C::XC; ++c)
+    for (int t = 0; t < 8; ++t) {
+      f32x2 v[R];
+      for (int k = 0; k < R; ++k) v[k] = Tr::unpack(x[C::XC * k + c][t]);
+      dft_small<R>(v, inv);
+      for (int k = 0; k < R; ++k) x[C::XC * k + c][t] = Tr::pack(cmul(v[k], w[k]));
+    }
 }
 
 // The input view of `tile`: its group/tile base and, for ROW tiles (contiguous

@@ -163,7 +163,7 @@
   MCFFT_COLM_LIST_CM(X, 1, 32)     \
   MCFFT_COLM_LIST_CM(X, 2, 16)
 
-// ---- ROWX: Q = R rows, NW = L / 512 waves (one MFMA tile set per row) ----
+// ---- ROWX: Q = R rows on NW <= L / 512 waves ----
 #define MCFFT_ROWX_L_CM(X, S, T, NW, CM) \
   X(S, T, 2, NW, CM, 5)                  \
   X(S, T, 4, NW, CM, 5)
@@ -174,11 +174,16 @@
   MCFFT_ROWX_L_CM(X, 2, 8, 4, CM)     \
   MCFFT_ROWX_L_CM(X, 3, 1, 8, CM)
 
-// R = 8 where the tile fits the LDS (<= 139 KiB)
+// R = 8 where the tile fits the LDS (<= 139 KiB); 4096-point rows also on 4
+// waves (two column sets per wave: a tuner candidate, measured 9-17 % slower
+// than 8 waves on 4096^2)
 #define MCFFT_ROWX_LIST(X) \
   MCFFT_ROWX_CM(X, 0)      \
   MCFFT_ROWX_CM(X, 1)      \
   MCFFT_ROWX_CM(X, 2)      \
+  X(3, 1, 2, 4, 0, 5)      \
+  X(3, 1, 2, 4, 1, 5)      \
+  X(3, 1, 2, 4, 2, 5)      \
   X(2, 8, 8, 4, 0, 5)      \
   X(3, 1, 8, 8, 0, 5)      \
   X(2, 8, 8, 4, 1, 5)      \

@@ -199,3 +199,23 @@ def test_gathered_batch(dims, count, scaling, fi, fo, monkeypatch):
         err = oracle.rel_l2(y, oracle.fft(xq, tuple(dims)))
         assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], (t, err)
     assert np.all(buf_out[~written] == sentinel)
+
+
+@pytest.mark.parametrize("name", ["mcfft_pass_3_1_2_4_0_5", "mcfft_pass_3_1_2_4_1_5", "mcfft_pass_3_1_2_4_2_5"])
+def test_rowx_two_column_sets_per_wave(name, monkeypatch):
+    """ROWX tiles of 4096-point rows on 4 waves (each wave takes two 32-column
+    sets of every row and pairs the rows' sets for its butterfly): 64 x 4096
+    forward and inverse against the oracle."""
+    monkeypatch.setenv("MCFFT_2D_OUTER_RADIX", "2")
+    cm = int(name.split("_")[-2])
+    fi, fo = (emu.FMT_F16, emu.FMT_SPLIT) if cm == 2 else (cm, cm)
+    rng = np.random.default_rng(len(name))
+    x = rng.uniform(-1, 1, 64 * 4096) + 1j * rng.uniform(-1, 1, 64 * 4096)
+    emu.set_kernel_override(name)
+    try:
+        for inverse in (False, True):
+            err, _ = run((64, 4096), 1, fi, fo, x, inverse=inverse)
+            assert kernels()[0] == name, kernels()
+            assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], err
+    finally:
+        emu.set_kernel_override(None)
