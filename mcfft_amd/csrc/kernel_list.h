@@ -43,6 +43,9 @@
   X(2, 4, 4, 2, CM, 0)           \
   X(2, 8, 4, 4, CM, 0)
 
+// 8192-point rows, 2 per tile (70 KiB of LDS)
+#define MCFFT_ROW_8K2_CM(X, CM) X(3, 2, 2, 16, CM, 0)
+
 // split-mode alternatives for short rows (latency-bound small batches)
 #define MCFFT_ROW_TUNE_SPLIT(X) \
   X(2, 1, 4, 2, 2, 0)           \
@@ -63,6 +66,8 @@
   MCFFT_ROW_WIDE_CM(X, 0)   \
   MCFFT_ROW_WIDE_CM(X, 1)   \
   MCFFT_ROW_WIDE_CM(X, 2)   \
+  MCFFT_ROW_8K2_CM(X, 0)    \
+  MCFFT_ROW_8K2_CM(X, 1)    \
   MCFFT_ROW_TUNE_SPLIT(X)
 
 // ---- ROWT: Q rows stored transposed (segments of Q elements) ----

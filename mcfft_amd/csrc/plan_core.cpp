@@ -48,6 +48,8 @@ struct PreferredShape {
 };
 static const PreferredShape kPreferred[] = {
     {KIND_ROWT, 2, 2, 0, 16, 8}, {KIND_ROWT, 2, 2, 1, 16, 8}, {KIND_ROWT, 2, 2, 2, 16, 8},
+    // 256-point rows, FP16/BF16: 16 rows on 4 waves (1D 256 x 131072 0.060 -> 0.049 ms)
+    {KIND_ROW, 2, 1, 0, 16, 4},  {KIND_ROW, 2, 1, 1, 16, 4},
     // 512-point rows: 8 rows on 2 waves (1D 512 x 131072 FP16 0.117 -> 0.106 ms, split 0.209 -> 0.186)
     {KIND_ROW, 2, 2, 0, 8, 2},   {KIND_ROW, 2, 2, 1, 8, 2},   {KIND_ROW, 2, 2, 2, 8, 2},
     // 1024/2048-point rows: four MFMA tiles per wave (2048^2 x 8 row pass 256 -> ~110 us)
@@ -57,6 +59,8 @@ static const PreferredShape kPreferred[] = {
     {KIND_COL, 2, 4, 0, 32, 16}, {KIND_COL, 2, 4, 1, 32, 16},  // 1024 points: 128-byte segments
     // 4096-point rows, FP16/BF16: 2 rows per tile (1D 4096 x 16384: 0.139 -> 0.118 ms)
     {KIND_ROW, 3, 1, 0, 2, 8},   {KIND_ROW, 3, 1, 1, 2, 8},
+    // 8192-point rows, FP16/BF16: 2 rows per tile (1D 8192 x 4096: 0.101 -> 0.084 ms)
+    {KIND_ROW, 3, 2, 0, 2, 16},  {KIND_ROW, 3, 2, 1, 2, 16},
     {KIND_ROW, 4, 1, 0, 1, 16},  {KIND_ROW, 4, 1, 1, 1, 16},   // single-pass 65536: 16 waves
 };
 
@@ -371,7 +375,9 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     const long long N = p.dims[0];
     const int lg = ilog2(N);
     int S0, T0;
-    if (split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW, scaling)) {  // single pass: 2^15, 2^16 in FP16/BF16
+    // single pass (up to 2^15 / 2^16 in FP16/BF16), unless a two-pass split is requested
+    const bool two_pass = opt.fourstep_lg2 > 0 && lg >= 12;
+    if (!two_pass && split_len(N, S0, T0) && find_kernel(S0, T0, cm, KIND_ROW, scaling)) {
       p.chunk = batch;
       p.n_chunks = 1;
       st = add_row(N, opt.gathered ? 1 : batch, BUF_IN, BUF_OUT, in_fmt, opt.gathered);
@@ -588,7 +594,8 @@ std::vector<PlanOpts> plan_structures(int rank, const long long* dims, long long
       tries.push_back(o);
     }
   } else if (rank == 1 && dims[0] > 0) {
-    // the default split and its neighbours (N1 = N/N2 one factor of 2 either way)
+    // the default split and its neighbours (N1 = N/N2 one factor of 2 either way);
+    // from 2^12 up, lengths with a single-pass kernel may run as two passes too
     const int lg = ilog2(dims[0]), d = (lg + 1) / 2 > 11 ? 11 : (lg + 1) / 2;
     for (int l2 = d - 1; l2 <= d + 1; ++l2) {
       PlanOpts o;

@@ -70,7 +70,8 @@ struct PassPlan {
 // records; -1 = the planner's default rule.
 struct PlanOpts {
   int outer_radix = -1;   // 2D: radix R of the outer-axis split (0: none; 2, 4, 8)
-  int fourstep_lg2 = -1;  // 1D two-pass four-step: log2 N2 (points of the second pass's rows)
+  int fourstep_lg2 = -1;  // 1D two-pass four-step: log2 N2 (points of the second pass's rows);
+                          // > 0 on a length with a single-pass kernel (>= 2^12): run it as two passes
   // gathered batch (transforms at arbitrary offsets, PassArgs::gather_*): every
   // pass has per-transform groups (a 1D single-pass plan one row per group)
   bool gathered = false;

@@ -127,13 +127,15 @@ def test_structures_enumerated():
     assert emu.structures([4096, 4096], 4, F16, SPL, AUTO) == ["R=2;L2=0", "R=0;L2=0", "R=4;L2=0"]
     assert emu.structures([1 << 20], 2, F16, F16, AUTO) == ["R=0;L2=10", "R=0;L2=9", "R=0;L2=11"]
     assert emu.structures([1 << 23], 1, F16, F16, AUTO) == ["R=0;L2=11"]   # N1 <= 2^12, N2 <= 2^11
-    assert emu.structures([4096], 8, F16, F16, AUTO) == ["R=0;L2=0"]
+    # single-pass lengths from 2^12 up may also run as two passes
+    assert emu.structures([4096], 8, F16, F16, AUTO) == ["R=0;L2=0", "R=0;L2=5", "R=0;L2=6", "R=0;L2=7"]
+    assert emu.structures([2048], 8, F16, F16, AUTO) == ["R=0;L2=0"]
     assert emu.structures([64, 64, 64], 1, F16, F16, AUTO) == ["R=0;L2=0"]
     d = emu.describe_opts([1 << 20], 2, F16, F16, AUTO, fourstep_lg2=9)
     assert "struct=R=0;L2=9" in d and "stride(in,out)=(512,512)" in d, d
 
 
-@pytest.mark.parametrize("n,lg2", [(1 << 17, 8), (1 << 17, 10), (1 << 18, 10)])
+@pytest.mark.parametrize("n,lg2", [(1 << 17, 8), (1 << 17, 10), (1 << 18, 10), (1 << 13, 6), (1 << 16, 8)])
 @pytest.mark.parametrize("fi,fo", FMTS)
 def test_fourstep_split_choices(n, lg2, fi, fo):
     """Two-pass 1D plans with a non-default N1 x N2 split (as the autotuner may
