@@ -37,9 +37,9 @@
  * Sizes: every length a power of two; 1D 16 <= N <= 2^27 (one pass up to 2^14,
  * 2^16 for FP16/BF16 output; two passes up to 2^23; three passes from 2^24,
  * one transform at a time, with a workspace of one transform); 2D/3D outer sides
- * 16..4096 and innermost side up to 2^16 for FP16/BF16 output, 2^14 for split
- * output (the innermost axis runs as one single-pass row kernel, and a split
- * row of 2^15 points does not fit the LDS).  The number of
+ * 16..4096 and innermost side 16..2^23 (one single-pass row kernel up to 2^16
+ * for FP16/BF16 output and 2^14 for split output; longer rows run a two-pass
+ * four-step over every row, with a workspace of one batch chunk).  The number of
  * columns an outer axis has (the product of the sides inside it) must be a
  * multiple of the width of a compiled column tile for that axis length (4..64;
  * 16 or more for every axis but the 16-point one, which needs 32: 16 x 16 is

@@ -49,7 +49,9 @@ inline void fill_bfp(PassArgs& a, const PlanCore& c, int pi, long long t0, long 
   a.bfp_nt = (int)nt;
   a.bfp_gpt = pp.groups_per_transform > 0 ? (int)pp.groups_per_transform : 1;
   a.bfp_q0_lg = ilog2(c.passes[0].k->Q);
-  a.bfp_tpg0 = c.passes[0].tiles_per_group;
+  // first-pass tiles per transform (its groups per transform times tiles per group)
+  a.bfp_tpg0 = c.passes[0].tiles_per_group *
+               (c.passes[0].groups_per_transform > 1 ? (int)c.passes[0].groups_per_transform : 1);
   a.bfp_rg = pp.rc_g < 0 ? (int)pp.L : pp.rc_g;
   a.bfp_rm = pp.rc_m;
   a.bfp_rq = pp.rc_q;

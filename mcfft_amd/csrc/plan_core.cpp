@@ -370,6 +370,86 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     return ST_OK;
   };
 
+  // Two-pass four-step FFT of every row (length N) of the chunk's transforms,
+  // `rows` rows per transform (1D: 1; 2D/3D rows too long for one pass: the
+  // product of the outer sides): columns of length N1 = N/N2 (stride N2) with
+  // twiddle W_N^(k1*n2) (in -> ws), then rows of length N2 = 2^lg2 stored
+  // transposed X[k1 + N1*k2] (ws -> out).  The caller sets the chunk and a
+  // workspace of one chunk.
+  auto add_fourstep = [&](long long N, long long rows, int lg2) -> int {
+    const int lg = ilog2(N), lg1 = lg - lg2;
+    p.opts.fourstep_lg2 = lg2;
+    if (lg1 > 12 || lg2 > 12) return ST_UNSUPPORTED;
+    const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
+    int S1, T1, S2, T2;
+    if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
+    const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF, scaling, N2);
+    const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT, scaling, N1);
+    if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
+    // pass A (columns + four-step twiddle): in -> ws, one group per row
+    PassPlan a;
+    a.k = kc;
+    a.kind = KIND_COLF;
+    a.L = N1;
+    a.src = BUF_IN;
+    a.dst = BUF_WS;
+    a.src_fmt = in_fmt;
+    a.stride_in = a.stride_out = (int)N2;
+    a.in_sg = a.out_sg = N;
+    a.in_st = a.out_st = kc->Q;
+    a.n_inst = N2;
+    a.tiles_per_group = (int)(N2 / kc->Q);
+    a.groups = p.chunk * rows;
+    a.groups_per_transform = rows;
+    a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
+    a.fourstep = true;
+    // four-step twiddle W_N^(k1*n2) times the last stage's scale: computed in the
+    // kernel for FP16/BF16, a direct table (k1 < N1, n2 < N2) for split
+    const float sl = pow2(-a.shifts.back());
+    a.out_scale = sl;
+    a.fs_cols = (int)N2;
+    a.fs_tw_off = (long long)p.tw.size();
+    if (cm == 2)
+      for (long long k1 = 0; k1 < N1; ++k1)
+        for (long long n2 = 0; n2 < N2; ++n2) p.tw.push_back({(k1 * n2) % N, N, sl});
+    for (int sh : a.shifts) total_shift += sh;
+    p.passes.push_back(a);
+    // pass B (rows, transposed store): ws -> out; AUTO reconcile: element (q, m)
+    // of row g came from pass A's instance m of that row ((g % rows) * N2 + m
+    // of the transform)
+    PassPlan b;
+    b.k = kr;
+    b.kind = KIND_ROWT;
+    b.L = N2;
+    b.src = BUF_WS;
+    b.dst = BUF_OUT;
+    b.src_fmt = ws_fmt;
+    b.stride_out = (int)N1;
+    b.in_sg = b.out_sg = N;
+    b.in_st = (long long)kr->Q * N2;
+    b.out_st = kr->Q;
+    b.n_inst = N1;
+    b.tiles_per_group = (int)(N1 / kr->Q);
+    b.groups = p.chunk * rows;
+    b.groups_per_transform = rows;
+    b.rc_g = (int)N2;
+    b.shifts = add_stage_tables(p.tw, b, S2, T2, scaling);
+    b.out_scale = pow2(-b.shifts.back());
+    for (int sh : b.shifts) total_shift += sh;
+    p.passes.push_back(b);
+    return ST_OK;
+  };
+  // default split of a two-pass row FFT of length N: N2 = 2^ceil(lg/2), at most 2^11
+  auto fourstep_lg2 = [](long long N) {
+    const int lg = ilog2(N);
+    return (lg + 1) / 2 > 11 ? 11 : (lg + 1) / 2;
+  };
+  // whether rows of length L run in one pass (a ROW kernel exists)
+  auto row_single = [&](long long L) {
+    int S, T;
+    return split_len(L, S, T) && find_kernel(S, T, cm, KIND_ROW, scaling) != nullptr;
+  };
+
   int st = ST_OK;
   if (rank == 1) {
     const long long N = p.dims[0];
@@ -386,77 +466,19 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
     } else {
       // four-step: N = N1 * N2; columns of length N1 (stride N2) with twiddle
       // W_N^(k1*n2), then rows of length N2 stored transposed X[k1 + N1*k2]
-      int lg2 = (lg + 1) / 2;
-      if (lg2 > 11) lg2 = 11;
+      int lg2 = fourstep_lg2(N);
       // a requested split: N2 = 2^4 .. 2^11 (ROWT), N1 = 2^4 .. 2^12 (COLF)
       if (opt.fourstep_lg2 >= 4 && opt.fourstep_lg2 <= 11 && lg - opt.fourstep_lg2 >= 4 &&
           lg - opt.fourstep_lg2 <= 12)
         lg2 = opt.fourstep_lg2;
-      const int lg1 = lg - lg2;
-      p.opts.fourstep_lg2 = lg2;
-      if (lg1 > 12 || lg2 > 12) return ST_UNSUPPORTED;
-      const long long N1 = 1LL << lg1, N2 = 1LL << lg2;
-      int S1, T1, S2, T2;
-      if (!split_len(N1, S1, T1) || !split_len(N2, S2, T2)) return ST_UNSUPPORTED;
-      const KernelInfo* kc = find_kernel(S1, T1, cm, KIND_COLF, scaling, N2);
-      const KernelInfo* kr = find_kernel(S2, T2, cm, KIND_ROWT, scaling, N1);
-      if (!kc || !kr || N2 % kc->Q != 0 || N1 % kr->Q != 0) return ST_UNSUPPORTED;
       long long limit = ws_limit_bytes > 0 ? ws_limit_bytes : (256LL << 20);
-      long long per = N * elem_bytes(ws_fmt);
-      long long chunk = limit / per;
+      long long chunk = limit / (N * elem_bytes(ws_fmt));
       if (chunk < 1) chunk = 1;
       if (chunk > batch) chunk = batch;
       p.chunk = chunk;
       p.n_chunks = (batch + chunk - 1) / chunk;
       p.ws_elems = chunk * N;
-      // pass A (columns + four-step twiddle): in -> ws
-      PassPlan a;
-      a.k = kc;
-      a.kind = KIND_COLF;
-      a.L = N1;
-      a.src = BUF_IN;
-      a.dst = BUF_WS;
-      a.src_fmt = in_fmt;
-      a.stride_in = a.stride_out = (int)N2;
-      a.in_sg = a.out_sg = N;
-      a.in_st = a.out_st = kc->Q;
-      a.n_inst = N2;
-      a.tiles_per_group = (int)(N2 / kc->Q);
-      a.groups = chunk;
-      a.groups_per_transform = 1;
-      a.shifts = add_stage_tables(p.tw, a, S1, T1, scaling);
-      a.fourstep = true;
-      // four-step twiddle W_N^(k1*n2) times the last stage's scale: computed in the
-      // kernel for FP16/BF16, a direct table (k1 < N1, n2 < N2) for split
-      const float sl = pow2(-a.shifts.back());
-      a.out_scale = sl;
-      a.fs_cols = (int)N2;
-      a.fs_tw_off = (long long)p.tw.size();
-      if (cm == 2)
-        for (long long k1 = 0; k1 < N1; ++k1)
-          for (long long n2 = 0; n2 < N2; ++n2) p.tw.push_back({(k1 * n2) % N, N, sl});
-      for (int s : a.shifts) total_shift += s;
-      p.passes.push_back(a);
-      // pass B (rows, transposed store): ws -> out
-      PassPlan b;
-      b.k = kr;
-      b.kind = KIND_ROWT;
-      b.L = N2;
-      b.src = BUF_WS;
-      b.dst = BUF_OUT;
-      b.src_fmt = ws_fmt;
-      b.stride_out = (int)N1;
-      b.in_sg = b.out_sg = N;
-      b.in_st = (long long)kr->Q * N2;
-      b.out_st = kr->Q;
-      b.n_inst = N1;
-      b.tiles_per_group = (int)(N1 / kr->Q);
-      b.groups = chunk;
-      b.groups_per_transform = 1;
-      b.shifts = add_stage_tables(p.tw, b, S2, T2, scaling);
-      b.out_scale = pow2(-b.shifts.back());
-      for (int s : b.shifts) total_shift += s;
-      p.passes.push_back(b);
+      st = add_fourstep(N, 1, lg2);
     }
   } else if (rank == 2) {
     // transforms run in chunks whose data fits the Infinity Cache, so the
@@ -522,15 +544,26 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
         c.rc_m = R;
         c.rc_q = 0;
       }
-    } else {
+    } else if (row_single(NY)) {
       st = add_row(NY, NX, BUF_IN, BUF_OUT, in_fmt, true);
+      if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
+    } else {
+      // rows too long for one pass (split 2^15, 2^16; FP16/BF16 above 2^16): a
+      // two-pass four-step over every row, then the columns
+      p.ws_elems = chunk * p.n;
+      st = add_fourstep(NY, NX, fourstep_lg2(NY));
       if (st == ST_OK) st = add_col(NX, NY, NY, batch, NX * NY, BUF_OUT, BUF_OUT, p.out_fmt);
     }
   } else {
     const long long NX = p.dims[0], NY = p.dims[1], NZ = p.dims[2];
     p.chunk = batch;
     p.n_chunks = 1;
-    st = add_row(NZ, NX * NY, BUF_IN, BUF_OUT, in_fmt, true);
+    if (row_single(NZ)) {
+      st = add_row(NZ, NX * NY, BUF_IN, BUF_OUT, in_fmt, true);
+    } else {  // innermost rows too long for one pass: four-step over every row
+      p.ws_elems = batch * p.n;
+      st = add_fourstep(NZ, NX * NY, fourstep_lg2(NZ));
+    }
     if (st == ST_OK) st = add_col(NY, NZ, NZ, batch * NX, NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
     if (st == ST_OK) st = add_col(NX, NY * NZ, NY * NZ, batch, NX * NY * NZ, BUF_OUT, BUF_OUT, p.out_fmt);
   }

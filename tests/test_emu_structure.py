@@ -221,3 +221,23 @@ def test_rowx_two_column_sets_per_wave(name, monkeypatch):
             assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], err
     finally:
         emu.set_kernel_override(None)
+
+
+# ---------------- 2D/3D rows longer than one pass ----------------
+LONG_ROWS = [((16, 32768), emu.FMT_F16, emu.FMT_SPLIT, 1), ((16, 32768), emu.FMT_F16, emu.FMT_SPLIT, AUTO)]
+
+
+@pytest.mark.parametrize("dims,fi,fo,scaling", LONG_ROWS)
+def test_long_rows_two_pass(dims, fi, fo, scaling):
+    """Innermost sides above the single-pass limit (split 2^15): a two-pass
+    four-step over every row, then the column pass; AUTO with rows 2^8 apart in
+    magnitude (first-pass tiles that disagree).  The GPU tests run the larger
+    shapes (split 2^16 rows, FP16/BF16 2^17 rows, 3D)."""
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n)
+    x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).reshape(-1, dims[-1])
+    x[::2] *= 256.0
+    d = emu.describe(list(dims), 1, fi, fo, scaling)
+    assert "fourstep" in d and len(d.splitlines()) == len(dims) + 2, d
+    err, _ = run(dims, 1, fi, fo, x.reshape(-1), scaling=scaling, inverse=(scaling == 1))
+    assert err <= TOL[fo if fo == emu.FMT_SPLIT else fi], err
