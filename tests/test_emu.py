@@ -172,7 +172,9 @@ def test_plan_describe_and_errors():
     with pytest.raises(RuntimeError, match="status 2"):
         emu.describe([256], 1, 2, 0)         # split in, fp16 out
     with pytest.raises(RuntimeError, match="status 2"):
-        emu.describe([16, 65536], 1, 0, 2)   # split rows of 65536 (single-pass kernel is FP16/BF16 only)
+        emu.describe([16, 1 << 24], 1, 0, 0)  # innermost side above 2^23
+    # split rows of 65536 (the single-pass kernel is FP16/BF16 only): four-step rows, then columns
+    assert emu.describe([16, 65536], 1, 0, 2).count("pass ") == 3
     with pytest.raises(RuntimeError, match="status 1"):
         emu.describe([256], 0, 0, 0)         # batch 0
 
