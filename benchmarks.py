@@ -395,6 +395,22 @@ def bench_layout(torch, reps, n=131072, batch=256):
            "staging_gbs": round(2 * 8 * n * batch / (stage_ms * 1e-3) / 1e9, 1) if stage_ms > 0 else None,
            "bit_identical": same}
     p.destroy()
+    # spaced interleaved transforms (64 elements between them): no staging pass
+    dist = n + 64
+    sp_in = torch.zeros(((batch - 1) * dist + n, 2), device="cuda", dtype=torch.float16)
+    for t in range(batch):
+        sp_in[t * dist:t * dist + n] = inter[t * n:(t + 1) * n]
+    sp_out = torch.empty_like(sp_in)
+    lay = mc.Layout.make([1], dist)
+    ps = mc.LayoutPlan([n], batch, lay, lay, "fp16", "fp16")
+    ms_s = _timer(torch, lambda i: ps.exec_layout(sp_in, None, sp_out, None, mc.FORWARD), reps)
+    ps.exec_layout(sp_in, None, sp_out, None, mc.FORWARD)
+    torch.cuda.synchronize()
+    same_s = all(bool(torch.equal(sp_out[t * dist:t * dist + n].view(torch.int32), out[t * n:(t + 1) * n].view(torch.int32)))
+                 for t in range(batch))
+    res["spaced"] = {"dist": dist, "ms": round(ms_s, 4), "staging_bytes": ps.workspace_bytes() - q.workspace_bytes(),
+                     "bit_identical": same_s}
+    ps.destroy()
     q.destroy()
     return res
 

@@ -168,12 +168,18 @@ typedef struct mcfftLayout_t {
   int planar;
 } mcfftLayout;
 
-/* A plan whose exec reads and writes the given layouts.  A side that is not
+/* A plan whose exec reads and writes the given layouts.  When every side is
+ * interleaved and dense inside each transform (only `dist` differs from the
+ * contiguous layout), the passes read and write the caller's buffers directly,
+ * locating transform t at t * dist -- for 2D/3D plans and 1D plans of more
+ * than one pass or of one-row tiles (lengths 16384, 32768, 65536 and from 2^17);
+ * such a plan is not tunable (MCFFT_UNSUPPORTED).  Otherwise a side that is not
  * the contiguous interleaved layout is staged through a plan-owned contiguous
  * buffer by a gather (input) or scatter (output) kernel: one extra read and
  * write of that side's data; a contiguous side costs nothing extra.  Tuning
- * (mcfft_plan_tune) and mcfft_exec_passes act on the contiguous transform, with
- * contiguous buffers; the tuning cache entry of the same shape applies. */
+ * (mcfft_plan_tune) and mcfft_exec_passes of a staged plan act on the
+ * contiguous transform, with contiguous buffers; the tuning cache entry of the
+ * same shape applies. */
 mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* dims, long long batch,
                                    const mcfftLayout* in_layout, const mcfftLayout* out_layout,
                                    mcfftFormat in_fmt, mcfftFormat out_fmt, mcfftScaling scaling);
@@ -182,7 +188,9 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
  * layout plan equals mcfft_exec_layout(plan, in, NULL, out, NULL, ...).
  * Input and output may overlap in any way (the staging orders the accesses),
  * except that an interleaved-contiguous input may alias a contiguous output
- * only exactly (in place). */
+ * only exactly (in place), and the sides of a plan that stages neither side
+ * (above) may alias only exactly, with equal layouts and formats; a partial
+ * overlap of those returns MCFFT_INVALID_VALUE. */
 mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_im, void* out_re, void* out_im,
                               int direction, void* stream);
 

@@ -200,6 +200,18 @@ inline int layout_validate(int rank, const long long* dims, long long batch, con
 }
 
 // True when the side is the contiguous interleaved layout (no staging pass).
+// interleaved and dense inside each transform (the contiguous layout with any
+// distance >= the transform's size between transforms)
+inline bool layout_is_spaced(int rank, const long long* dims, const long long* strides, long long dist, int planar) {
+  if (planar) return false;
+  long long s = 1;
+  for (int d = rank - 1; d >= 0; --d) {
+    if (strides[d] != s) return false;
+    s *= dims[d];
+  }
+  return dist >= s;
+}
+
 inline bool layout_is_dense(int rank, const long long* dims, const long long* strides, long long dist, int planar) {
   if (planar) return false;
   long long s = 1;

@@ -571,8 +571,49 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
       const int v = layout_validate(rank, dims, batch, lay[sd]->strides, lay[sd]->dist, lay[sd]->planar);
       if (v != ST_OK) return (mcfftStatus)v;
     }
+  // Sides that only space whole transforms apart (interleaved, dense inside a
+  // transform) need no staging: the plan runs as a gathered batch whose offset
+  // tables are t * dist -- when its rows do not share tiles (2D/3D, or 1D plans
+  // of more than one pass or one-row tiles; shorter 1D rows keep staging).
+  long long dist[2] = {0, 0};
+  bool spaced = true;
+  for (int sd = 0; sd < 2; ++sd) {
+    long long s = 1;
+    for (int d = rank - 1; d >= 0; --d) s *= dims[d];
+    dist[sd] = s;
+    if (lay[sd]) {
+      spaced = spaced && layout_is_spaced(rank, dims, lay[sd]->strides, lay[sd]->dist, lay[sd]->planar);
+      dist[sd] = lay[sd]->dist;
+    }
+  }
+  if (spaced && rank == 1) {
+    PlanCore probe;
+    spaced = plan_build(probe, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, 0) == ST_OK &&
+             (probe.passes.size() > 1 || probe.passes[0].k->Q == 1);
+  }
   mcfftPlan p = nullptr;
-  mcfftStatus st = mcfft_plan_many(&p, rank, dims, batch, in_fmt, out_fmt, scaling);
+  mcfftStatus st;
+  if (spaced) {
+    PlanOpts o;
+    o.gathered = true;
+    st = create_plan(&p, rank, dims, batch, in_fmt, out_fmt, scaling, &o);
+    for (int sd = 0; sd < 2 && st == MCFFT_SUCCESS; ++sd) {
+      std::vector<long long> off((size_t)batch);
+      for (long long t = 0; t < batch; ++t) off[(size_t)t] = t * dist[sd];
+      if (hipMalloc(&p->gather[sd], sizeof(long long) * (size_t)batch) != hipSuccess)
+        st = MCFFT_ALLOC_FAILED;
+      else if (hipMemcpy(p->gather[sd], off.data(), sizeof(long long) * (size_t)batch, hipMemcpyHostToDevice) !=
+               hipSuccess)
+        st = MCFFT_HIP_ERROR;
+      p->gather_ext[sd] = (batch - 1) * dist[sd] + p->core.n;
+    }
+    if (st != MCFFT_SUCCESS && p) {
+      free_plan(p);
+      return st;
+    }
+  } else {
+    st = mcfft_plan_many(&p, rank, dims, batch, in_fmt, out_fmt, scaling);
+  }
   if (st != MCFFT_SUCCESS) return st;
   p->layout = true;
   p->lay_rank = rank;
@@ -594,7 +635,9 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
       }
       p->lay_dist[sd] = n;
     }
-    p->lay_dense[sd] = layout_is_dense(rank, dims, p->lay_strides[sd], p->lay_dist[sd], p->lay_planar[sd]);
+    // a gathered plan reads and writes the caller's spaced sides directly
+    p->lay_dense[sd] =
+        spaced || layout_is_dense(rank, dims, p->lay_strides[sd], p->lay_dist[sd], p->lay_planar[sd]);
   }
   // staging: one buffer serves both sides when both stage and the formats match
   // (the transform then runs in place on it)
@@ -965,6 +1008,16 @@ static mcfftStatus exec_layout(mcfftPlan plan, const void* in_re, const void* in
       if ((size_t)ptr[sd][c] % al) return MCFFT_MISALIGNED;
   }
   const int np = (int)plan->core.passes.size();
+  if (plan->gather[0]) {
+    // no staging: the sides may alias only exactly
+    const char* a0 = (const char*)in_re;
+    const char* b0 = (const char*)out_re;
+    const char* a1 = a0 + plan->gather_ext[0] * elem_bytes(fmt[0]);
+    const char* b1 = b0 + plan->gather_ext[1] * elem_bytes(fmt[1]);
+    const bool overlap = a0 < b1 && b0 < a1;
+    if (overlap && !(a0 == b0 && plan->lay_dist[0] == plan->lay_dist[1] && fmt[0] == fmt[1]))
+      return MCFFT_INVALID_VALUE;
+  }
   auto launch = [&](int sd, const void* k, LayoutArgs& la) -> mcfftStatus {
     layout_fill(la, plan->lay_rank, plan->lay_dims, plan->core.batch, plan->lay_strides[sd], plan->lay_dist[sd],
                 plan->lay_planar[sd], fmt[sd]);
@@ -1085,6 +1138,7 @@ static mcfftStatus time_exec(mcfftPlan p, const void* in, void* out, int directi
 
 mcfftStatus mcfft_plan_tune(mcfftPlan plan, const void* in, void* out, int direction, void* stream, int reps) {
   if (!plan || plan->ragged || plan->real_kind) return MCFFT_INVALID_PLAN;
+  if (plan->gather[0]) return MCFFT_UNSUPPORTED;  // its buffers are laid out for its offset tables
   if (!in || !out || reps < 1) return MCFFT_INVALID_VALUE;
   int n_cu = 0;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, plan->device) != hipSuccess)

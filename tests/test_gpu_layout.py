@@ -36,7 +36,14 @@ CASES = [
     ([131072], 3, ([1], 131072, 1), ([1], 131072, 1), "fp16", "fp16"),          # four-step, planar
     ([4096, 64], 2, ([1, 4096], 4096 * 64, 0), None, "bf16", "bf16"),              # column-major input, 2D
     ([65536], 2, ([1], 65536, 1), ([1], 65536, 1), "split", "split"),            # split four-step, shared staging
+    # spaced transforms (dense inside, a gap between): no staging, the passes
+    # locate transform t at t * dist
+    ([1024, 64], 2, ([64, 1], 1024 * 64 + 96, 0), ([64, 1], 1024 * 64 + 32, 0), "fp16", "fp16"),
+    ([131072], 3, ([1], 131072 + 64, 0), ([1], 131072 + 8, 0), "fp16", "split"),   # four-step
+    ([16384], 4, ([1], 16384 + 7, 0), None, "bf16", "bf16"),                        # one-row tiles
+    ([1024], 3, ([1], 1030, 0), None, "fp16", "fp16"),                              # 4-row tiles: staged
 ]
+SPACED = {9: True, 10: True, 11: True, 12: False}
 
 
 def _to_dev(torch, a):
@@ -57,6 +64,7 @@ def run_layout(torch, x, dims, batch, li, lo, fi, fo, direction):
     out_re, out_im = emu.layout_encode(np.zeros(0), FMT_ID[fo], lo[2], np.zeros(0, dtype=np.int64),
                                        int(opos.max()) + 1, fill=FILL)
     p = mc.LayoutPlan(dims, batch, mc.Layout.make(*li), mc.Layout.make(*lo), fi, fo)
+    run_layout.staging_bytes = p.workspace_bytes() - mc.Plan(dims, batch, fi, fo).workspace_bytes()
     d = [_to_dev(torch, a) for a in (in_re, in_im, out_re, out_im)]
     p.exec_layout(d[0], d[1], d[2], d[3], direction)
     torch.cuda.synchronize()
@@ -78,6 +86,8 @@ def test_layout_exec(torch_cuda, case, direction):
     rng = np.random.default_rng(case)
     x = rng.standard_normal(n * batch) + 1j * rng.standard_normal(n * batch)
     y, (ore, oim), opos = run_layout(torch, x, dims, batch, li, lo, fi, fo, direction)
+    if case in SPACED:
+        assert (run_layout.staging_bytes == 0) == SPACED[case], run_layout.staging_bytes
     # contiguous exec of the same values
     p = mc.Plan(dims, batch, fi, fo)
     d_in = mc.to_device(x, fi)
@@ -131,5 +141,30 @@ def test_layout_invalid(torch_cuda):
     buf = torch.zeros((256, 2), dtype=torch.float16, device="cuda")
     with pytest.raises(mc.McfftError) as e:   # planar input without its imaginary array
         p.exec_layout(buf, None, buf, None)
+    assert e.value.status == 1
+    p.destroy()
+
+
+def test_spaced_layout_in_place_and_overlap(torch_cuda):
+    """A spaced layout (no staging) in place gives the out-of-place result bit
+    for bit; sides that overlap other than exactly are refused."""
+    torch = torch_cuda
+    dims, batch, dist = [2048, 32], 3, 2048 * 32 + 64
+    n = 2048 * 32
+    lay = mc.Layout.make([32, 1], dist)
+    p = mc.LayoutPlan(dims, batch, lay, lay, "fp16", "fp16")
+    g = torch.Generator(device="cuda").manual_seed(8)
+    ext = (batch - 1) * dist + n
+    big = torch.randn((ext + 64, 2), device="cuda", generator=g).half()
+    buf = big[:ext]
+    out = torch.empty_like(buf)
+    p.exec_layout(buf, None, out, None, mc.FORWARD)
+    p.exec_layout(buf, None, buf, None, mc.FORWARD)   # in place
+    torch.cuda.synchronize()
+    for t in range(batch):
+        sl = slice(t * dist, t * dist + n)
+        assert torch.equal(buf[sl].view(torch.int32), out[sl].view(torch.int32))
+    with pytest.raises(mc.McfftError) as e:
+        p.exec_layout(buf, None, big[64:], None, mc.FORWARD)   # overlaps, not exactly
     assert e.value.status == 1
     p.destroy()
