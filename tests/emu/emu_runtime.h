@@ -85,30 +85,48 @@ inline void launch_concurrent(int nblocks, int nt, size_t lds, const std::functi
   }
 }
 
+// workgroups of a launch() that run at the same time (EMU_BLOCKS, default 4):
+// the pass kernels' workgroups only meet through atomics, which are real ones
+inline int concurrent_blocks() {
+  static const int n = [] {
+    const char* e = getenv("EMU_BLOCKS");
+    const int v = e ? atoi(e) : 4;
+    return v < 1 ? 1 : v;
+  }();
+  return n;
+}
+
 // launch `body` as a grid of nblocks workgroups of nt threads with `lds` bytes of
-// LDS, running only workgroups [b0, b1) of it (default: all)
+// LDS, running only workgroups [b0, b1) of it (default: all), a few at a time
 inline void launch(int nblocks, int nt, size_t lds, const std::function<void()>& body, int b0 = 0, int b1 = -1) {
   if (b1 < 0 || b1 > nblocks) b1 = nblocks;
-  for (int b = b0; b < b1; ++b) {
-    Block blk;
-    blk.nt = nt;
-    blk.bid = b;
-    blk.nblocks = nblocks;
-    blk.lds.assign(lds, 0xCD);
-    std::barrier<> bar(nt);
-    blk.bar = &bar;
-    int nw = nt / 64;
-    std::vector<std::barrier<>*> wb;
-    for (int w = 0; w < nw; ++w) wb.push_back(new std::barrier<>(64));
-    blk.wbar = wb;
-    blk.xa.assign((size_t)nw * 64 * 8, 0.f);
-    blk.xb.assign((size_t)nw * 64 * 8, 0.f);
-    blk.xs.assign((size_t)nt, 0.f);
+  const int nw = nt / 64;
+  for (int g0 = b0; g0 < b1; g0 += concurrent_blocks()) {
+    const int g1 = g0 + concurrent_blocks() < b1 ? g0 + concurrent_blocks() : b1;
+    std::vector<Block> blks((size_t)(g1 - g0));
     std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&, t] { tl_blk = &blk; tl_tid = t; body(); });
+    for (int b = g0; b < g1; ++b) {
+      Block& blk = blks[(size_t)(b - g0)];
+      blk.nt = nt;
+      blk.bid = b;
+      blk.nblocks = nblocks;
+      blk.lds.assign(lds, 0xCD);
+      blk.bar = new std::barrier<>(nt);
+      for (int w = 0; w < nw; ++w) blk.wbar.push_back(new std::barrier<>(64));
+      blk.xa.assign((size_t)nw * 64 * 8, 0.f);
+      blk.xb.assign((size_t)nw * 64 * 8, 0.f);
+      blk.xs.assign((size_t)nt, 0.f);
+    }
+    for (int b = g0; b < g1; ++b)
+      for (int t = 0; t < nt; ++t) {
+        Block* bp = &blks[(size_t)(b - g0)];
+        th.emplace_back([&body, bp, t] { tl_blk = bp; tl_tid = t; body(); });
+      }
     for (auto& x : th) x.join();
-    for (auto* p : wb) delete p;
+    for (Block& blk : blks) {
+      delete blk.bar;
+      for (auto* p : blk.wbar) delete p;
+    }
   }
 }
 }  // namespace emu

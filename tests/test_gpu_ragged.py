@@ -219,6 +219,22 @@ def test_ragged_gathered_bulk_runs(torch_cuda, monkeypatch, fmt):
     print(f"ragged gathered {fmt}: {len(shapes)} items, worst rel_l2 {worst:.3e}")
     assert worst <= TOL[fmt]
     assert np.all(y[~covered] == 7.0)
+    # the exec (gathered and one-item batched plans + queue launches) replays from
+    # a hipGraph to the same bits, eager work and synchronisations in between
+    o2 = mc.to_device(np.full(total, 7.0 + 0j), out_fmt)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            p.exec(d_in, o2, mc.FORWARD, stream=s)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        o2.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        got = mc.from_device(o2, out_fmt)
+        assert np.array_equal(got[covered], y[covered])
 
 
 def test_ragged_shapes_outside_the_queue(torch_cuda):
