@@ -142,6 +142,11 @@ MCFFT_DEV unsigned wave_max(unsigned v) {
 MCFFT_DEV void release_dev() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 MCFFT_DEV void acquire_dev() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 MCFFT_DEV void nap() { __builtin_amdgcn_s_sleep(4); }
+// wait n ticks of the constant 100 MHz clock (s_memrealtime); ends by itself
+MCFFT_DEV void stall_ticks(int n) {
+  const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+  while (__builtin_amdgcn_s_memrealtime() - t0 < (unsigned long long)n) __builtin_amdgcn_s_sleep(8);
+}
 
 // Work done for the whole workgroup by one wave inside a loop that also holds
 // barriers: branch on wave_id() (wave-uniform, a scalar branch) and let every
@@ -156,6 +161,8 @@ MCFFT_DEV unsigned wave_first(unsigned v) { return (unsigned)__builtin_amdgcn_re
 // pack the low halves / high halves of two packed pairs into one pair (v_perm_b32)
 MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x05040100u); }
 MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x07060302u); }
+// either of the two above, chosen per lane by sel (0x05040100 or 0x07060302)
+MCFFT_DEV uint32_t sel_halves(uint32_t e0, uint32_t e1, uint32_t sel) { return __builtin_amdgcn_perm(e1, e0, sel); }
 
 // D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
 // Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and

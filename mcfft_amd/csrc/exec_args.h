@@ -68,7 +68,7 @@ inline void fill_bfp(PassArgs& a, const PlanCore& c, int pi, long long t0, long 
 // the direction and `aops` its DFT-matrix operands.
 inline void fill_pass_args(PassArgs& a, const PlanCore& c, const PassPlan& pp, const void* in, void* out,
                            long long in_elems, long long out_elems, int inverse, const f32x2* tw,
-                           const u32x4* aops) {
+                           const u32x4* aops, const uint32_t* dft256) {
   memset(&a, 0, sizeof(a));
   a.in = in;
   a.out = out;
@@ -89,6 +89,7 @@ inline void fill_pass_args(PassArgs& a, const PlanCore& c, const PassPlan& pp, c
   for (size_t si = 0; si < 6; ++si) a.st_scale[si] = si < pp.shifts.size() ? pow2(-pp.shifts[si]) : 1.f;
   a.stage_tw = tw + pp.stage_tw_off;
   a.aops = aops;
+  a.dft256 = dft256;
   if (pp.fourstep) {
     a.fs_tw = tw + pp.fs_tw_off;
     a.fs_tw_entries = (int)((long long)c.tw.size() - pp.fs_tw_off);

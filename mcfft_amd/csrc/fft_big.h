@@ -14,12 +14,55 @@
 // live data per lane: 1.25x the lane's share (8 waves: 128 + 32 packed
 // elements, 2 waves per SIMD; 16 waves: 64 + 16, 4 waves per SIMD).
 //
+// Twiddles.  Between stages the element of output row rr (0..15) of column j
+// needs W_M^(r2 * (j % NS)) * W_256^(r2 * rr), M = 256 NS, r2 = j >> 8.  A
+// thread's MFMA tile k holds 32 columns with one r2, so the second factor scales
+// the DFT matrix's rows: the kernel multiplies by a DFT matrix with rows scaled
+// by W_256^(r2 rr) (16 of them, read from a table in LDS), and only the first
+// factor, the same for all outputs of a lane's column, is applied to the FP32
+// accumulators (stage 0: none, only the stage's scale).
+//
 // No reference counterpart: the reference has no 65536-point plan
 // (source/mcfft_1d_device.cpp:695-775 falls to `default:` for it).
 #pragma once
 #include "fft_pass.h"
 
 namespace mcfft {
+
+// Row-scaled DFT matrices, one 32-bit word (re, im halves in the plan's FP16 or
+// BF16 format) per entry: word (r2 * 16 + i) * 16 + k holds
+// W_256^(i * (16 k + r2)) = W_16^(i k) * W_256^(r2 i), direction's sign; r2 = 0 is
+// the plain 16-point DFT matrix.
+constexpr int kDft256Words = 16 * 16 * 16;
+MCFFT_DEV int dft256_exponent(int word) {
+  const int r2 = word >> 8, i = (word >> 4) & 15, k = word & 15;
+  return (i * (16 * k + r2)) & 255;
+}
+
+// MFMA A operands of lane `lane` (see aops_words) for the DFT matrix with rows
+// scaled by W_256^(r2 rr), from the table in LDS: the lane's row i, entries
+// k = 8h..8h+7.  Rows i < 16 (real parts) take (re, -im), rows 16 + i (imaginary
+// parts) take (im, re).
+MCFFT_DEV AOps dft256_aops(const uint32_t* tab, int r2, int lane) {
+  const int i = lane & 15, h = lane >> 5;
+  const bool top = (lane & 16) == 0;
+  const int w0 = ((r2 * 16 + i) * 16 + 8 * h) >> 2;
+  const u32x4 g0 = lds_ld((const u32x4*)tab, w0), g1 = lds_ld((const u32x4*)tab, w0 + 1);
+  const uint32_t s1 = top ? 0x05040100u : 0x07060302u, s2 = top ? 0x07060302u : 0x05040100u;
+  const uint32_t neg = top ? 0x80008000u : 0u;
+  AOps A;
+  A.a1[0] = sel_halves(g0[0], g0[1], s1);
+  A.a1[1] = sel_halves(g0[2], g0[3], s1);
+  A.a1[2] = sel_halves(g1[0], g1[1], s1);
+  A.a1[3] = sel_halves(g1[2], g1[3], s1);
+  A.a2[0] = sel_halves(g0[0], g0[1], s2) ^ neg;
+  A.a2[1] = sel_halves(g0[2], g0[3], s2) ^ neg;
+  A.a2[2] = sel_halves(g1[0], g1[1], s2) ^ neg;
+  A.a2[3] = sel_halves(g1[2], g1[3], s2) ^ neg;
+  A.a1lo = A.a1;
+  A.a2lo = A.a2;
+  return A;
+}
 
 template <int NW_>
 struct BigCfg {
@@ -28,7 +71,8 @@ struct BigCfg {
   static constexpr int HALF = 32768;
   static constexpr int LDS_ELEMS = HALF + HALF / 16;
   static constexpr int RED_OFF = LDS_ELEMS * 4;  // then one word per wave for block reductions
-  static constexpr int LDS_BYTES = RED_OFF + 64;
+  static constexpr int TAB_OFF = RED_OFF + 64;   // then the row-scaled DFT matrices (kDft256Words)
+  static constexpr int LDS_BYTES = TAB_OFF + kDft256Words * 4;
   // LDS slot of position m inside its round: drop bit 13, pad every 16 elements
   //   lidx(m) = c + (c >> 4),  c = (m & 0x1fff) | ((m >> 14) << 13).
   // Column j of MFMA tile k is low + 32 NW k (low = 32 wave + lane%32 < 32 NW).
@@ -68,13 +112,14 @@ MCFFT_DEV GBuf big_view(const void* p, const long long* off, int gpt, long long 
 // AUTO: the kernel serves MCFFT_SCALE_AUTO plans (first-pass normalisation and
 // output exponents); otherwise statically scaled plans (output exponents).
 template <int CM, int NW, bool AUTO>
-MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A,
+MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next,
                             typename Traits<CM>::T (&x)[BigCfg<NW>::TPW][8], SlotAcc& sacc) {
   static_assert(CM == CM_F16 || CM == CM_BF16, "4-byte elements only");
   typedef Traits<CM> Tr;
   typedef typename Tr::T El;
   typedef BigCfg<NW> B;
   MCFFT_DYN_LDS(El, lds);
+  const uint32_t* tab = (const uint32_t*)((const unsigned char*)lds + B::TAB_OFF);
   // The per-lane bases below do not depend on the tile; an opaque copy of the
   // thread index keeps the compiler from hoisting them (and everything derived
   // from them) out of the workgroup's tile loop.
@@ -115,43 +160,46 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
                              : hi8 * 4352 + lo8 + (lo8 >> 4) + 272 * 4 * h;
     constexpr int wmul = P == 0 ? 1 : P == 1 ? 17 : 272;
     const int vout = (low + 4 * h * B::LQ) * 4;   // last stage: element j + rr * 4096
+    // The element written at m' = (j / NS) * 16 NS + j % NS + rr * NS is input
+    // r2 = m' >> 12 = j >> 8 of the next stage's column m' & 4095, and its
+    // twiddle W_{256 NS}^(r2 * (m' % 16 NS)) splits into W_256^(r2 rr) (in the
+    // tile's DFT matrix) and the column factor cf = W_{256 NS}^(r2 * (j % NS)).
+    // Tile k's r2 is r20 + DR * k, so cf advances by W_{256 NS}^(DR * (j % NS))
+    // per tile.  The stage's scale rides on cf (stage 0: on the accumulators);
+    // the last stage's scale rides on stage 2's.
+    constexpr int NS = ipow16(P < 3 ? P : 0);
+    constexpr float INV_M = 1.0f / (float)(256 * NS);
+    constexpr int DR = (32 * NW) >> 8;
+    const int r20 = low >> 8;
+    const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : P == 0 ? a.st_scale[0] * nsc : a.st_scale[P];
+    f32x2 cf = {sc, 0.0f}, cf_step = {1.0f, 0.0f};
+    if constexpr (P == 1 || P == 2) {
+      const int jm = low & (NS - 1);
+      cf = twiddle_rev(r20 * jm, INV_M, a.inverse != 0, sc);
+      cf_step = twiddle_rev(DR * jm, INV_M, a.inverse != 0, 1.0f);
+    }
+    AOps A0;
+    if constexpr (P == 3) A0 = dft256_aops(tab, 0, lane);
 #pragma unroll
     for (int k = 0; k < B::TPW; ++k) {
-      const f32x16 y = dft16<CM>(x[k], A);
+      f32x16 y;
+      if constexpr (P < 3) y = dft16<CM>(x[k], dft256_aops(tab, r20 + DR * k, lane));
+      else y = dft16<CM>(x[k], A0);
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        f32x2 tw[4];
-        if constexpr (P < 3) {
-          // element written at m' = jb + rr * NS is input m' >> 12 of the next
-          // stage's column m' & 4095: twiddle W_{256 NS}^((m' >> 12) * (m' % 16NS))
-          // r2 = m' >> 12 is the same for the lane's outputs and m' % 16NS grows
-          // by NS per u: W^E (one sin/cos) times powers of W^(r2 NS).  The last
-          // stage's scale rides on the twiddles of stage 2.
-          constexpr int NS = ipow16(P);
-          constexpr float INV_M = 1.0f / (float)(256 * NS);
-          const int j = low + 32 * NW * k;
-          const int mo0 = (int)(((unsigned)j / NS) * NS * 16 + ((unsigned)j % NS)) + (4 * h + 8 * g) * NS;
-          const int r2 = mo0 >> 12;
-          const float sc = P == 2 ? a.st_scale[2] * a.st_scale[3] : P == 0 ? a.st_scale[0] * nsc : a.st_scale[P];
-          const f32x2 d = twiddle_rev(r2 * NS, INV_M, a.inverse != 0, 1.0f);
-          tw[0] = twiddle_rev(r2 * (mo0 & (16 * NS - 1)), INV_M, a.inverse != 0, sc);
-#pragma unroll
-          for (int u = 1; u < 4; ++u) tw[u] = cmul(tw[u - 1], d);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int s = 4 * g + u;
-          f32x2 v;
-          v.x = y[s];
-          v.y = y[8 + s];
-          if constexpr (P < 3) {
-            x[k][s] = Tr::pack(cmul(v, tw[u]));
-          } else {
-            const El e = Tr::pack(v);
-            buf_st32s(gout, vout, (k * 32 * NW + B::rrc(s) * B::LQ) * 4, e);
-          }
+      for (int s = 0; s < 8; ++s) {
+        f32x2 v;
+        v.x = y[s];
+        v.y = y[8 + s];
+        if constexpr (P == 0) {
+          x[k][s] = Tr::pack(v * cf.x);
+        } else if constexpr (P < 3) {
+          x[k][s] = Tr::pack(cmul(v, cf));
+        } else {
+          const El e = Tr::pack(v);
+          buf_st32s(gout, vout, (k * 32 * NW + B::rrc(s) * B::LQ) * 4, e);
         }
       }
+      if constexpr (P == 1 || P == 2) cf = cmul(cf, cf_step);
       if constexpr (P == 3) {
         if (next >= 0) {
 #pragma unroll
@@ -210,10 +258,21 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next, const AOps& A
 template <int CM, int NW, bool AUTO>
 MCFFT_DEV void fft_big_body(const PassArgs& a) {
   typedef BigCfg<NW> B;
-  const AOps A = load_aops(a.aops, lane_id());
   typename Traits<CM>::T x[B::TPW][8];
+  {
+    // the row-scaled DFT matrices into LDS (read by every stage; the first
+    // barrier of stage 0 comes after its first use, hence the one here)
+    MCFFT_DYN_LDS(uint32_t, lds);
+    u32x4* tab = (u32x4*)(lds + B::TAB_OFF / 4);
+    for (int i = thread_id(); i < kDft256Words / 4; i += B::NT) lds_st(tab, i, g_ld((const u32x4*)a.dft256, (long long)i));
+    block_barrier();
+  }
   const int g = grid_size();
   int tile = block_id();
+  // start stagger (PassArgs::stagger_*), only when every workgroup has several
+  // transforms: with one each, the late half's wait would add to the kernel time
+  if (a.stagger_ticks > 0 && a.stagger_phases > 1 && a.n_tiles >= 4 * g)
+    stall_ticks(((tile >> 3) % a.stagger_phases) * a.stagger_ticks);
   if (tile < a.n_tiles) {
     const int tid = thread_id(), lane = tid & 63, low = 32 * (tid >> 6) + (lane & 31);
     const int vin = (low + 8 * (lane >> 5) * B::LQ) * 4;
@@ -227,7 +286,7 @@ MCFFT_DEV void fft_big_body(const PassArgs& a) {
   if (AUTO && (a.bfp_flags & BFP_FIRST)) sacc.epoch = (unsigned long long)uniform64(g_ld(&a.bfp_state->epoch, 0));
   for (; tile < a.n_tiles; tile += g) {
     const int next = tile + g < a.n_tiles ? tile + g : -1;
-    fft_big_tile<CM, NW, AUTO>(a, tile, next, A, x, sacc);
+    fft_big_tile<CM, NW, AUTO>(a, tile, next, x, sacc);
   }
   if constexpr (AUTO) slot_flush(sacc, a.bfp_slots);
 }

@@ -80,6 +80,7 @@ struct PassArgs {
   float out_scale;            // last-stage scale when there is no four-step table
   const f32x2* stage_tw;      // post-twiddle tables [stage][j][16] (see plan_core)
   const u32x4* aops;          // DFT-matrix MFMA operands [64 lanes][4]: A1, A2, A1lo, A2lo
+  const uint32_t* dft256;     // 65536-point kernel: 16-point DFT rows scaled by W_256 powers (fft_big.h)
   const f32x2* fs_tw;         // four-step twiddles W_N^(m*column), N1 x fs_cols entries, or (fs_lo_bits > 0)
                               // (split ROWX: W_NX^(r1*k2) at r1*R + k2, unscaled)
                               // W_N^(m*(column % 2^b)) [N1][2^b] then W_N^(m*(column - column % 2^b)) [N1][fs_cols >> b]
@@ -112,13 +113,22 @@ struct PassArgs {
   const long long* gather_in;
   const long long* gather_out;
   int gather_gpt;
+  // start stagger of the single-pass 65536-point kernel: workgroup b waits
+  // ((b / 8) % stagger_phases) * stagger_ticks ticks of the 100 MHz clock before
+  // its first load, so that its memory phases fall into other workgroups'
+  // compute phases instead of all workgroups loading, then computing, together
+  // (0: no wait)
+  int stagger_ticks, stagger_phases;
 };
 
-// first element of group `grp` of a side with group stride sg (see PassArgs::gather_*)
+// first element of group `grp` of a side with group stride sg (see PassArgs::gather_*).
+// grp is wave-uniform at every call, so the offset is read back as a scalar: a
+// buffer descriptor built from a vector register would wrap every load and store
+// of the tile in a waterfall loop.
 MCFFT_DEV long long group_base(const long long* off, int gpt, long long sg, int grp) {
   if (!off) return (long long)grp * sg;
   const int t = grp / gpt;
-  return g_ld(off, (long long)t) + (long long)(grp - t * gpt) * sg;
+  return uniform64((unsigned long long)g_ld(off, (long long)t)) + (long long)(grp - t * gpt) * sg;
 }
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:

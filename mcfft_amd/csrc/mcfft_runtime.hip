@@ -19,6 +19,7 @@
 
 #include "../../include/mcfft.h"
 #include "fft_pass.h"
+#include "fft_big.h"
 #include "plan_core.h"
 #include "ragged.h"
 #include "real_pass.h"
@@ -35,7 +36,11 @@ struct mcfftPlan_st {
   f32x2* tw[2] = {nullptr, nullptr};      // [0] forward, [1] inverse
   uint16_t* dft[2] = {nullptr, nullptr};
   u32x4* aops[2] = {nullptr, nullptr};    // per-lane DFT-matrix MFMA operands
+  uint32_t* dft256[2] = {nullptr, nullptr};  // row-scaled DFT matrices of the 65536-point kernel
   std::vector<int> grid_cap;              // resident workgroups per pass (occupancy x CUs)
+  // PassArgs::stagger_* of the 65536-point kernel: half its workgroups start
+  // 15 us late (4096 x 65536 FP16: 0.585 -> 0.561 ms); MCFFT_BIG_STAGGER=ticks[,phases]
+  int stagger_ticks = 1500, stagger_phases = 2;
   void* ws = nullptr;
   size_t ws_bytes = 0;
   // exec bookkeeping (bfp.h): status word and last-launch counter, per-transform
@@ -126,6 +131,25 @@ __global__ void gen_dft_table(int sign, uint16_t* t) {
   t[48 + e] = __builtin_bit_cast(uint16_t, (_Float16)((im - (double)ih) * 2048.0));
   t[64 + e] = __builtin_bit_cast(uint16_t, (__bf16)(float)re);
   t[80 + e] = __builtin_bit_cast(uint16_t, (__bf16)(float)im);
+}
+
+// the 65536-point kernel's row-scaled DFT matrices (fft_big.h): FP64, rounded
+// once to the plan's format as the 16-point table is (r2 = 0 gives its values)
+__global__ void gen_dft256(int sign, int cm, uint32_t* t) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= kDft256Words) return;
+  double sn, cs;
+  sincospi(2.0 * dft256_exponent(w) / 256.0, &sn, &cs);
+  const double im = sign * sn;
+  uint32_t re_b, im_b;
+  if (cm == CM_BF16) {
+    re_b = __builtin_bit_cast(uint16_t, (__bf16)(float)cs);
+    im_b = __builtin_bit_cast(uint16_t, (__bf16)(float)im);
+  } else {
+    re_b = __builtin_bit_cast(uint16_t, (_Float16)cs);
+    im_b = __builtin_bit_cast(uint16_t, (_Float16)im);
+  }
+  t[w] = re_b | (im_b << 16);
 }
 
 // per-lane MFMA operand words of the DFT matrix (64 lanes x 16 words)
@@ -332,6 +356,7 @@ void free_plan(mcfftPlan p) {
     if (p->tw[d]) (void)hipFree(p->tw[d]);
     if (p->dft[d]) (void)hipFree(p->dft[d]);
     if (p->aops[d]) (void)hipFree(p->aops[d]);
+    if (p->dft256[d]) (void)hipFree(p->dft256[d]);
   }
   if (p->ws) (void)hipFree(p->ws);
   if (p->state) (void)hipFree(p->state);
@@ -364,6 +389,8 @@ mcfftStatus make_tables(mcfftPlan p, const std::vector<TwSpec>& tw, int cm) {
     if (rs == MCFFT_SUCCESS && hipMalloc(&p->dft[d], sizeof(uint16_t) * kDftTableEntries) != hipSuccess)
       rs = MCFFT_ALLOC_FAILED;
     if (rs == MCFFT_SUCCESS && hipMalloc(&p->aops[d], sizeof(u32x4) * 64 * 4) != hipSuccess) rs = MCFFT_ALLOC_FAILED;
+    if (rs == MCFFT_SUCCESS && hipMalloc(&p->dft256[d], sizeof(uint32_t) * kDft256Words) != hipSuccess)
+      rs = MCFFT_ALLOC_FAILED;
   }
   if (rs == MCFFT_SUCCESS && ntw > 0 &&
       hipMemcpy(dspec, tw.data(), sizeof(TwSpec) * ntw, hipMemcpyHostToDevice) != hipSuccess)
@@ -373,6 +400,7 @@ mcfftStatus make_tables(mcfftPlan p, const std::vector<TwSpec>& tw, int cm) {
     if (ntw > 0) gen_twiddles<<<(unsigned)((ntw + 255) / 256), 256>>>(dspec, ntw, sign, p->tw[d]);
     gen_dft_table<<<1, 64>>>(sign, p->dft[d]);
     gen_aops<<<1, 64>>>(p->dft[d], cm, (uint32_t*)p->aops[d]);
+    gen_dft256<<<kDft256Words / 256, 256>>>(sign, cm, p->dft256[d]);
     if (hipGetLastError() != hipSuccess) rs = MCFFT_HIP_ERROR;
   }
   if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;
@@ -433,6 +461,11 @@ mcfftStatus create_plan(mcfftPlan* plan, int rank, const long long* dims, long l
     apply_kernels(p, tc.kernels);
   }
   apply_forced(p);
+  if (const char* env = getenv("MCFFT_BIG_STAGGER")) {  // ticks of the 100 MHz clock[,phases]; 0: none
+    p->stagger_ticks = atoi(env);
+    const char* c = strchr(env, ',');
+    p->stagger_phases = c ? atoi(c + 1) : 2;
+  }
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);
     int per_cu = 0;
@@ -1352,11 +1385,13 @@ static mcfftStatus exec_complex(mcfftPlan plan, const void* in, void* out, int d
       const int dst_fmt = (pp.dst == BUF_OUT) ? c.out_fmt : ws_fmt;
       PassArgs a;
       fill_pass_args(a, c, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
-                     view_elems(pp.dst), d, plan->tw[d], plan->aops[d]);
+                     view_elems(pp.dst), d, plan->tw[d], plan->aops[d], plan->dft256[d]);
       if (gathered) {
         const long long* const tab[2] = {plan->gather[0], plan->gather[1]};
         gather_args(a, pp, tab, t0);
       }
+      a.stagger_ticks = plan->stagger_ticks;
+      a.stagger_phases = plan->stagger_phases;
       fill_bfp(a, c, pi, t0, nt, plan->real_kind == 2, in_exp,
                BfpBuffers{plan->state, plan->slots, plan->tile_e, plan->out_exp});
       const long long groups = pp.groups_per_transform > 0 ? nt * pp.groups_per_transform : pp.groups;

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "fft_big.h"
 #include "fft_pass.h"
 #include "plan_core.h"
 #include "ragged.h"
@@ -53,6 +54,23 @@ static void dft_table_host(int sign, uint16_t* t) {
     t[48 + e] = f16b((im - (double)__builtin_bit_cast(_Float16, ih)) * 2048.0);
     t[64 + e] = bf16b(re);
     t[80 + e] = bf16b(im);
+  }
+}
+
+// the 65536-point kernel's row-scaled DFT matrices (fft_big.h), as the runtime
+// makes them (FP64 values rounded once; exact at quarter turns)
+static void dft256_table_host(int sign, int cm, uint32_t* t) {
+  for (int w = 0; w < kDft256Words; ++w) {
+    const int e = dft256_exponent(w);
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * e / 256.0L;
+    double re = (double)cosl(a), im = (double)(sign * sinl(a));
+    if (e % 64 == 0) {
+      const int q = e / 64;
+      re = (q == 0) ? 1 : (q == 2) ? -1 : 0;
+      im = sign * ((q == 1) ? 1 : (q == 3) ? -1 : 0);
+    }
+    const uint32_t rb = cm == 1 ? bf16b(re) : f16b(re), ib = cm == 1 ? bf16b(im) : f16b(im);
+    t[w] = rb | (ib << 16);
   }
 }
 
@@ -175,6 +193,8 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   std::vector<u32x4> aops(64 * 4);
   for (int lane = 0; lane < 64; ++lane)
     aops_words(dft.data(), p.cm, lane, [&](int k, uint32_t w) { aops[lane * 4 + k / 4][k % 4] = w; });
+  std::vector<uint32_t> dft256(kDft256Words);
+  dft256_table_host(sign, p.cm, dft256.data());
   const int ws_fmt = (p.cm == 2) ? 2 : in_fmt;
   std::vector<unsigned char> ws((size_t)(p.ws_elems * elem_bytes(ws_fmt)) + 16);
   const size_t in_bytes = (size_t)((gathered ? g_gather_ext[0] : p.n * batch) * elem_bytes(in_fmt));
@@ -203,6 +223,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
   emu::register_range(ws.data(), (size_t)(p.ws_elems * elem_bytes(ws_fmt)));
   emu::register_range(tw.data(), p.tw.size() * sizeof(f32x2));
   emu::register_range(aops.data(), aops.size() * sizeof(u32x4));
+  emu::register_range(dft256.data(), dft256.size() * sizeof(uint32_t));
   emu::register_range(&state, sizeof(state));
   emu::register_range(out_exp.data(), out_exp.size() * sizeof(int));
   emu::register_range(slots.data(), bfp_slot_count(p) * sizeof(unsigned long long));
@@ -228,7 +249,7 @@ extern "C" int emu_exec(int rank, const long long* dims, long long batch, int in
       const int dst_fmt = (pp.dst == BUF_OUT) ? out_fmt : ws_fmt;
       PassArgs a;
       fill_pass_args(a, p, pp, bufptr(pp.src, pp.src_fmt), bufptr(pp.dst, dst_fmt), view_elems(pp.src),
-                     view_elems(pp.dst), inverse, tw.data(), aops.data());
+                     view_elems(pp.dst), inverse, tw.data(), aops.data(), dft256.data());
       if (gathered) {
         const long long* const tab[2] = {g_gather[0].data(), g_gather[1].data()};
         gather_args(a, pp, tab, t0);

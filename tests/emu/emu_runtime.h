@@ -243,6 +243,7 @@ inline unsigned wave_max(unsigned v) {
 inline void release_dev() { __atomic_thread_fence(__ATOMIC_RELEASE); }
 inline void acquire_dev() { __atomic_thread_fence(__ATOMIC_ACQUIRE); }
 inline void nap() { std::this_thread::yield(); }
+inline void stall_ticks(int) {}
 inline int wave_id() { return emu::tl_tid / 64; }
 // lane 0's value, for every lane of the wave (all 64 lanes take part)
 inline unsigned wave_first(unsigned v) {
@@ -256,6 +257,11 @@ inline unsigned wave_first(unsigned v) {
 }
 
 inline uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
+inline uint32_t sel_halves(uint32_t e0, uint32_t e1, uint32_t sel) {
+  if (sel == 0x05040100u) return (e0 & 0xffffu) | (e1 << 16);
+  if (sel == 0x07060302u) return (e0 >> 16) | (e1 & 0xffff0000u);
+  emu::fail("sel_halves selector", (long long)sel);
+}
 inline uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }
 
 template <class Dec>
