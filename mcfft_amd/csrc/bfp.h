@@ -106,15 +106,15 @@ MCFFT_DEV int bfp_norm(uint32_t bits, int cm, int quantum) {
 }
 
 // Largest |component| over the workgroup (each wave's lane 0 writes its wave's
-// value to red[wave]; every thread gets the maximum, as a wave-uniform value).
-// Ends after a barrier.
-MCFFT_DEV uint32_t block_max(uint32_t v, unsigned* red, int nw, int wave, int lane) {
+// value to red[wave * stride]; every thread gets the maximum, as a wave-uniform
+// value).  Ends after a barrier.
+MCFFT_DEV uint32_t block_max(uint32_t v, unsigned* red, int nw, int wave, int lane, int stride = 1) {
   v = wave_max(v);
-  if (lane == 0) lds_st(red, wave, v);
+  if (lane == 0) lds_st(red, wave * stride, v);
   block_barrier();
   uint32_t m = 0;
   for (int w = 0; w < nw; ++w) {
-    const uint32_t x = lds_ld(red, w);
+    const uint32_t x = lds_ld(red, w * stride);
     m = x > m ? x : m;
   }
   return (uint32_t)uniform((int)m);

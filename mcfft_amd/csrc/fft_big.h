@@ -29,36 +29,38 @@
 
 namespace mcfft {
 
-// Row-scaled DFT matrices, one 32-bit word (re, im halves in the plan's FP16 or
-// BF16 format) per entry: word (r2 * 16 + i) * 16 + k holds
-// W_256^(i * (16 k + r2)) = W_16^(i k) * W_256^(r2 i), direction's sign; r2 = 0 is
-// the plain 16-point DFT matrix.
-constexpr int kDft256Words = 16 * 16 * 16;
-MCFFT_DEV int dft256_exponent(int word) {
-  const int r2 = word >> 8, i = (word >> 4) & 15, k = word & 15;
-  return (i * (16 * k + r2)) & 255;
+// Row-scaled DFT matrices G_r2[i][k] = W_256^(i (16 k + r2)) = W_16^(i k) W_256^(r2 i)
+// (direction's sign; r2 = 0 is the plain 16-point DFT matrix), stored as MFMA A
+// operand words (see aops_words): 16-byte group ((b * 256 + r2 * 16 + i) * 2 + h)
+// holds entries k = 8h..8h+7 of row i, two per word (k even in the low half), as
+// their real parts (block b = 0), imaginary parts (b = 1) or negated imaginary
+// parts (b = 2), in the plan's FP16 or BF16 format.
+constexpr int kDft256Words = 3 * 256 * 2 * 4;
+// the two entries of word w: row i, columns k and k + 1, block b, scale r2
+struct Dft256Word {
+  int b, r2, i, k;
+};
+MCFFT_DEV Dft256Word dft256_word(int w) {
+  const int q = w >> 2, rem = q & 511;
+  Dft256Word d;
+  d.b = q >> 9;
+  d.r2 = rem >> 5;
+  d.i = (rem >> 1) & 15;
+  d.k = 8 * (rem & 1) + 2 * (w & 3);
+  return d;
 }
+MCFFT_DEV int dft256_exponent(int r2, int i, int k) { return (i * (16 * k + r2)) & 255; }
 
-// MFMA A operands of lane `lane` (see aops_words) for the DFT matrix with rows
-// scaled by W_256^(r2 rr), from the table in LDS: the lane's row i, entries
-// k = 8h..8h+7.  Rows i < 16 (real parts) take (re, -im), rows 16 + i (imaginary
-// parts) take (im, re).
+// MFMA A operands of lane `lane` for the DFT matrix with rows scaled by
+// W_256^(r2 rr), read from the table in LDS: rows < 16 (real parts of the
+// outputs) take (re, -im), rows >= 16 (imaginary parts) take (im, re).
 MCFFT_DEV AOps dft256_aops(const uint32_t* tab, int r2, int lane) {
   const int i = lane & 15, h = lane >> 5;
   const bool top = (lane & 16) == 0;
-  const int w0 = ((r2 * 16 + i) * 16 + 8 * h) >> 2;
-  const u32x4 g0 = lds_ld((const u32x4*)tab, w0), g1 = lds_ld((const u32x4*)tab, w0 + 1);
-  const uint32_t s1 = top ? 0x05040100u : 0x07060302u, s2 = top ? 0x07060302u : 0x05040100u;
-  const uint32_t neg = top ? 0x80008000u : 0u;
+  const int b1 = top ? 0 : 1, b2 = top ? 2 : 0;
   AOps A;
-  A.a1[0] = sel_halves(g0[0], g0[1], s1);
-  A.a1[1] = sel_halves(g0[2], g0[3], s1);
-  A.a1[2] = sel_halves(g1[0], g1[1], s1);
-  A.a1[3] = sel_halves(g1[2], g1[3], s1);
-  A.a2[0] = sel_halves(g0[0], g0[1], s2) ^ neg;
-  A.a2[1] = sel_halves(g0[2], g0[3], s2) ^ neg;
-  A.a2[2] = sel_halves(g1[0], g1[1], s2) ^ neg;
-  A.a2[3] = sel_halves(g1[2], g1[3], s2) ^ neg;
+  A.a1 = lds_ld((const u32x4*)tab, (b1 * 256 + r2 * 16 + i) * 2 + h);
+  A.a2 = lds_ld((const u32x4*)tab, (b2 * 256 + r2 * 16 + i) * 2 + h);
   A.a1lo = A.a1;
   A.a2lo = A.a2;
   return A;
@@ -70,9 +72,12 @@ struct BigCfg {
   static_assert(NW_ == 8 || NW_ == 16, "8 or 16 waves");
   static constexpr int HALF = 32768;
   static constexpr int LDS_ELEMS = HALF + HALF / 16;
-  static constexpr int RED_OFF = LDS_ELEMS * 4;  // then one word per wave for block reductions
-  static constexpr int TAB_OFF = RED_OFF + 64;   // then the row-scaled DFT matrices (kDft256Words)
+  // block reductions: one word per wave in the exchange's padding slots
+  // 17 q + 16, which no exchange access touches (see lidx below)
+  static constexpr int RED_SLOT0 = 16, RED_STRIDE = 17;
+  static constexpr int TAB_OFF = LDS_ELEMS * 4;  // then the row-scaled DFT matrices (kDft256Words)
   static constexpr int LDS_BYTES = TAB_OFF + kDft256Words * 4;
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS of one workgroup");
   // LDS slot of position m inside its round: drop bit 13, pad every 16 elements
   //   lidx(m) = c + (c >> 4),  c = (m & 0x1fff) | ((m >> 14) << 13).
   // Column j of MFMA tile k is low + 32 NW k (low = 32 wave + lane%32 < 32 NW).
@@ -147,8 +152,8 @@ MCFFT_DEV void fft_big_tile(const PassArgs& a, int tile, int next,
 #pragma unroll
       for (int t = 0; t < 8; ++t) acc = pk_absmax(acc, x[k][t]);
     const int grp = tile / a.tiles_per_group;
-    nsc = bfp_first_tile(a, acc, CM, (unsigned*)((unsigned char*)lds + B::RED_OFF), NW, tile,
-                         a.bfp_gpt > 0 ? grp / a.bfp_gpt : 0, e_out, sacc);
+    nsc = bfp_first_tile(a, acc, CM, (unsigned*)lds + B::RED_SLOT0, NW, tile,
+                         a.bfp_gpt > 0 ? grp / a.bfp_gpt : 0, e_out, sacc, B::RED_STRIDE);
   }
 
   For<0, 4>::run([&](auto pc) __attribute__((always_inline)) {

@@ -435,11 +435,12 @@ MCFFT_DEV void load_stage0(const PassArgs& a, GBuf gin, int wave, int lane, type
 // e_out (the stored output's exponent) and, unless the pass is also the last,
 // records the tile's exponent (byte) and folds it into its transform's slots.
 // An all-zero tile keeps 2^0 and records no exponent; an Inf/NaN input sets the
-// status bit and is treated like it.  Ends after a barrier (block_max).
+// status bit and is treated like it.  Ends after a barrier (block_max; its
+// per-wave words at red[wave * red_stride]).
 MCFFT_DEV float bfp_first_tile(const PassArgs& a, uint32_t acc, int cm, unsigned* red, int nw, int tile, int t_loc,
-                               int& e_out, SlotAcc& sacc) {
+                               int& e_out, SlotAcc& sacc, int red_stride = 1) {
   const int tid = thread_id();
-  const uint32_t mb = block_max(pk_fold(acc), red, nw, tid >> 6, tid & 63);
+  const uint32_t mb = block_max(pk_fold(acc), red, nw, tid >> 6, tid & 63, red_stride);
   const bool nonfinite = bits_nonfinite(mb, cm), zero = nonfinite || mb == 0;
   if (nonfinite && tid == 0) atomic_or_dev(&a.bfp_state->status, kStatusNonFinite);
   const int n = zero ? 0 : bfp_norm(mb, cm, a.bfp_quantum);

@@ -138,18 +138,17 @@ __global__ void gen_dft_table(int sign, uint16_t* t) {
 __global__ void gen_dft256(int sign, int cm, uint32_t* t) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= kDft256Words) return;
-  double sn, cs;
-  sincospi(2.0 * dft256_exponent(w) / 256.0, &sn, &cs);
-  const double im = sign * sn;
-  uint32_t re_b, im_b;
-  if (cm == CM_BF16) {
-    re_b = __builtin_bit_cast(uint16_t, (__bf16)(float)cs);
-    im_b = __builtin_bit_cast(uint16_t, (__bf16)(float)im);
-  } else {
-    re_b = __builtin_bit_cast(uint16_t, (_Float16)cs);
-    im_b = __builtin_bit_cast(uint16_t, (_Float16)im);
+  const Dft256Word d = dft256_word(w);
+  uint32_t word = 0;
+  for (int j = 0; j < 2; ++j) {
+    double sn, cs;
+    sincospi(2.0 * dft256_exponent(d.r2, d.i, d.k + j) / 256.0, &sn, &cs);
+    const double v = d.b == 0 ? cs : d.b == 1 ? sign * sn : -(sign * sn);
+    const uint32_t bits = cm == CM_BF16 ? __builtin_bit_cast(uint16_t, (__bf16)(float)v)
+                                        : __builtin_bit_cast(uint16_t, (_Float16)v);
+    word |= bits << (16 * j);
   }
-  t[w] = re_b | (im_b << 16);
+  t[w] = word;
 }
 
 // per-lane MFMA operand words of the DFT matrix (64 lanes x 16 words)
@@ -400,7 +399,7 @@ mcfftStatus make_tables(mcfftPlan p, const std::vector<TwSpec>& tw, int cm) {
     if (ntw > 0) gen_twiddles<<<(unsigned)((ntw + 255) / 256), 256>>>(dspec, ntw, sign, p->tw[d]);
     gen_dft_table<<<1, 64>>>(sign, p->dft[d]);
     gen_aops<<<1, 64>>>(p->dft[d], cm, (uint32_t*)p->aops[d]);
-    gen_dft256<<<kDft256Words / 256, 256>>>(sign, cm, p->dft256[d]);
+    gen_dft256<<<(kDft256Words + 255) / 256, 256>>>(sign, cm, p->dft256[d]);
     if (hipGetLastError() != hipSuccess) rs = MCFFT_HIP_ERROR;
   }
   if (rs == MCFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) rs = MCFFT_HIP_ERROR;

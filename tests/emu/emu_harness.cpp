@@ -61,16 +61,21 @@ static void dft_table_host(int sign, uint16_t* t) {
 // makes them (FP64 values rounded once; exact at quarter turns)
 static void dft256_table_host(int sign, int cm, uint32_t* t) {
   for (int w = 0; w < kDft256Words; ++w) {
-    const int e = dft256_exponent(w);
-    long double a = 2.0L * 3.14159265358979323846264338327950288L * e / 256.0L;
-    double re = (double)cosl(a), im = (double)(sign * sinl(a));
-    if (e % 64 == 0) {
-      const int q = e / 64;
-      re = (q == 0) ? 1 : (q == 2) ? -1 : 0;
-      im = sign * ((q == 1) ? 1 : (q == 3) ? -1 : 0);
+    const Dft256Word d = dft256_word(w);
+    uint32_t word = 0;
+    for (int j = 0; j < 2; ++j) {
+      const int e = dft256_exponent(d.r2, d.i, d.k + j);
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * e / 256.0L;
+      double re = (double)cosl(a), im = (double)(sign * sinl(a));
+      if (e % 64 == 0) {
+        const int q = e / 64;
+        re = (q == 0) ? 1 : (q == 2) ? -1 : 0;
+        im = sign * ((q == 1) ? 1 : (q == 3) ? -1 : 0);
+      }
+      const double v = d.b == 0 ? re : d.b == 1 ? im : -im;
+      word |= (uint32_t)(cm == 1 ? bf16b(v) : f16b(v)) << (16 * j);
     }
-    const uint32_t rb = cm == 1 ? bf16b(re) : f16b(re), ib = cm == 1 ? bf16b(im) : f16b(im);
-    t[w] = rb | (ib << 16);
+    t[w] = word;
   }
 }
 
