@@ -73,6 +73,14 @@ MCFFT_DEV void mem_order_point() { asm volatile("" ::: "memory"); }
 template <class T> MCFFT_DEV T lds_ld(const T* p, int i) { return p[i]; }
 template <class T> MCFFT_DEV void lds_st(T* p, int i, T v) { p[i] = v; }
 template <class T> MCFFT_DEV T g_ld(const T* p, long long i) { return p[i]; }
+// wave-uniform read of memory that no kernel of the launch writes, through the
+// scalar cache (constant address space: s_load): its wait covers earlier scalar
+// loads only, where a vector load's wait would also cover the wave's pending
+// vector loads and stores
+template <class T> MCFFT_DEV T s_ld(const T* p, long long i) {
+  typedef const __attribute__((address_space(4))) T* ConstPtr;
+  return ((ConstPtr)p)[i];
+}
 template <class T> MCFFT_DEV void g_st(T* p, long long i, T v) { p[i] = v; }
 
 // Buffer-resource addressing: a 64-bit base in scalar registers plus 32-bit byte

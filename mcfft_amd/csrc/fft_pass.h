@@ -122,13 +122,14 @@ struct PassArgs {
 };
 
 // first element of group `grp` of a side with group stride sg (see PassArgs::gather_*).
-// grp is wave-uniform at every call, so the offset is read back as a scalar: a
-// buffer descriptor built from a vector register would wrap every load and store
-// of the tile in a waterfall loop.
+// grp is wave-uniform at every call and the offset table is read-only, so the
+// offset is a scalar load: a buffer descriptor built from a vector register would
+// wrap every load and store of the tile in a waterfall loop, and a vector load's
+// wait would also wait for the previous tile's stores.
 MCFFT_DEV long long group_base(const long long* off, int gpt, long long sg, int grp) {
   if (!off) return (long long)grp * sg;
   const int t = grp / gpt;
-  return uniform64((unsigned long long)g_ld(off, (long long)t)) + (long long)(grp - t * gpt) * sg;
+  return s_ld(off, (long long)t) + (long long)(grp - t * gpt) * sg;
 }
 
 // Layout of the 16-entry DFT root tables (uint16 bit patterns), per direction:

@@ -606,7 +606,8 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
   // Sides that only space whole transforms apart (interleaved, dense inside a
   // transform) need no staging: the plan runs as a gathered batch whose offset
   // tables are t * dist -- when its rows do not share tiles (2D/3D, or 1D plans
-  // of more than one pass or one-row tiles; shorter 1D rows keep staging).
+  // of more than one pass or with a one-row tile kernel, 2048 points and up;
+  // shorter 1D rows keep staging).
   long long dist[2] = {0, 0};
   bool spaced = true;
   for (int sd = 0; sd < 2; ++sd) {
@@ -620,7 +621,9 @@ mcfftStatus mcfft_plan_many_layout(mcfftPlan* plan, int rank, const long long* d
   }
   if (spaced && rank == 1) {
     PlanCore probe;
-    spaced = plan_build(probe, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, 0) == ST_OK &&
+    PlanOpts po;
+    po.gathered = true;
+    spaced = plan_build(probe, rank, dims, batch, (int)in_fmt, (int)out_fmt, (int)scaling, 0, po) == ST_OK &&
              (probe.passes.size() > 1 || probe.passes[0].k->Q == 1);
   }
   mcfftPlan p = nullptr;

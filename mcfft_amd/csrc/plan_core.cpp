@@ -320,7 +320,10 @@ int plan_build(PlanCore& p, int rank, const long long* dims, long long batch, in
   auto add_row = [&](long long L, long long n_rows, int src, int dst, int src_fmt, bool per_transform) -> int {
     int S, T;
     if (!split_len(L, S, T)) return ST_UNSUPPORTED;
-    const KernelInfo* k = find_kernel(S, T, cm, KIND_ROW, scaling);
+    // one row per group (a gathered batch: each transform its own group, or a
+    // single transform): a tile of several rows would run mostly empty
+    const KernelInfo* k = n_rows == 1 ? find_kernel(S, T, cm, KIND_ROW, scaling, 0, 1) : nullptr;
+    if (!k) k = find_kernel(S, T, cm, KIND_ROW, scaling);
     if (!k) return ST_UNSUPPORTED;
     PassPlan pp;
     pp.k = k;

@@ -175,6 +175,7 @@ template <class T> inline void lds_st(T* p, int i, T v) {
   p[i] = v;
 }
 template <class T> inline T g_ld(const T* p, long long i) { emu::check_global(p + i, sizeof(T), i); return p[i]; }
+template <class T> inline T s_ld(const T* p, long long i) { return g_ld(p, i); }
 template <class T> inline void g_st(T* p, long long i, T v) { emu::check_global(p + i, sizeof(T), i); p[i] = v; }
 
 // buffer-resource emulation with the hardware's range check: an access outside

@@ -207,6 +207,17 @@ def test_gathered_batch(dims, count, scaling, fi, fo, monkeypatch):
     assert np.all(buf_out[~written] == sentinel)
 
 
+@pytest.mark.parametrize("n", [2048, 4096, 8192, 16384, 32768])
+def test_one_row_groups_take_one_row_tiles(n):
+    """A row pass with one row per group (a single transform here; every
+    transform of a gathered batch) runs a kernel whose tile holds one row, where
+    one is compiled, instead of a tile of several rows run mostly empty."""
+    for fmt in (emu.FMT_F16, emu.FMT_BF16):
+        d = emu.describe([n], 1, fmt, fmt)
+        name = [l.split()[1] for l in d.splitlines() if l.strip().startswith("pass")][0]
+        assert name.split("_")[4] == "1", d
+
+
 @pytest.mark.parametrize("name", ["mcfft_pass_3_1_2_4_0_5", "mcfft_pass_3_1_2_4_1_5", "mcfft_pass_3_1_2_4_2_5"])
 def test_rowx_two_column_sets_per_wave(name, monkeypatch):
     """ROWX tiles of 4096-point rows on 4 waves (each wave takes two 32-column

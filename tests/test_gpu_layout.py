@@ -42,8 +42,10 @@ CASES = [
     ([131072], 3, ([1], 131072 + 64, 0), ([1], 131072 + 8, 0), "fp16", "split"),   # four-step
     ([16384], 4, ([1], 16384 + 7, 0), None, "bf16", "bf16"),                        # one-row tiles
     ([1024], 3, ([1], 1030, 0), None, "fp16", "fp16"),                              # 4-row tiles: staged
+    ([4096], 5, ([1], 4096 + 64, 0), ([1], 4096 + 3, 0), "fp16", "fp16"),          # one-row tile kernel when gathered
+    ([8192], 3, None, ([1], 8192 + 16, 0), "bf16", "bf16"),
 ]
-SPACED = {9: True, 10: True, 11: True, 12: False}
+SPACED = {9: True, 10: True, 11: True, 12: False, 13: True, 14: True}
 
 
 def _to_dev(torch, a):
