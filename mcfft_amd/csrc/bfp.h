@@ -149,9 +149,11 @@ MCFFT_DEV void slot_note(SlotAcc& s, unsigned long long* slots, long long t, int
   s.vmin = lo > s.vmin ? lo : s.vmin;
 }
 // exponents a later pass reads from transform t's slots; emax = emin = 0 for a
-// transform whose first-pass tiles all held zeros
+// transform whose first-pass tiles all held zeros.  t is wave-uniform and the
+// slots were written by an earlier launch, so they are scalar loads (whose wait
+// does not cover the previous tile's stores, unlike a vector load's)
 MCFFT_DEV void slot_read(const unsigned long long* slots, long long t, int& emax, int& emin) {
-  const int vmax = (int)(g_ld(slots, 2 * t) & 0xffffu), vmin = (int)(g_ld(slots, 2 * t + 1) & 0xffffu);
+  const int vmax = (int)(s_ld(slots, 2 * t) & 0xffffu), vmin = (int)(s_ld(slots, 2 * t + 1) & 0xffffu);
   emax = vmax - kBfpBias;
   emin = kBfpBias - vmin;
   if (emax <= -kBfpZeroSentinel) emax = emin = 0;
