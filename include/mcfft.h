@@ -202,10 +202,10 @@ mcfftStatus mcfft_exec_layout(mcfftPlan plan, const void* in_re, const void* in_
  *   environment MCFFT_RAGGED_BULK=<elements> changes the bound, 0 turns it off)
  *   runs as one batched plan.  So do the other items of one shape wherever they
  *   lie (a gathered batch: the plan locates its transforms through per-item
- *   offset tables) when they reach the bound together -- 2D/3D items and 1D
- *   items of 4096 points or more -- and, whatever their size, items of shapes
- *   the queue below cannot run (1D lengths above 2^18, outer axes narrower than
- *   the queue's column tiles).
+ *   offset tables) when they reach the bound or 1024 items together -- 2D/3D
+ *   items and 1D items of 2048 points or more -- and, whatever their size,
+ *   items of shapes the queue below cannot run (1D lengths above 2^18, outer
+ *   axes narrower than the queue's column tiles).
  * - The rest (1D lengths 16..2^18, sides 16..4096) run by one persistent kernel
  *   launch per pass level (at most three: every item's first passes, then
  *   second, then third) over a device-side work queue, plus a tiny queue-reset

@@ -463,7 +463,12 @@ MCFFT_DEV void bfp_last_tile(const PassArgs& a, int Q, int inst0, int grp, int t
   const bool single = (a.bfp_flags & BFP_SINGLE) != 0;
   if (single ? (tid < Q && inst0 + tid < a.n_inst) : (tid == 0 && til == 0 && grp == t_loc * a.bfp_gpt)) {
     const long long t = (long long)a.bfp_tr0 + (single ? inst0 + tid : t_loc);
-    g_st(a.bfp_out_exp, t, e_out + (a.bfp_in_exp ? g_ld(a.bfp_in_exp, t) : 0));
+    // the caller's input exponent: a scalar load where t is wave-uniform (a
+    // vector load here would wait for all of the tile's stores); each entry is
+    // read before its one writer (this thread) may overwrite it in place
+    int e_in = 0;
+    if (a.bfp_in_exp) e_in = (!single || Q == 1) ? s_ld(a.bfp_in_exp, t) : g_ld(a.bfp_in_exp, t);
+    g_st(a.bfp_out_exp, t, e_out + e_in);
   }
   if ((a.bfp_flags & BFP_CHECK) && stored_nonfinite) atomic_or_dev(&a.bfp_state->status, kStatusNonFinite);
 }

@@ -782,12 +782,15 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
     i = j;
   }
   // Gathered bulk runs: the remaining items of one shape wherever they lie, at
-  // least bulk_min elements together, run as one batched plan whose transforms
-  // are located through offset tables -- 2D/3D items and 1D items of >= 4096
-  // points (whose row tiles hold one transform; shorter rows share tiles, so
-  // they stay in the queue unless they lie back to back).  Items of a shape the
-  // queue's pass variants cannot run (1D above 2^18 points, outer axes narrower
-  // than a variant's column tile) run this way whatever their size.
+  // least bulk_min elements or kRaggedGatherItems items together (enough
+  // one-transform tiles to fill the GPU several times), run as one batched plan
+  // whose transforms are located through offset tables -- 2D/3D items and 1D
+  // items whose row pass has a one-row tile kernel (2048 points and up; shorter
+  // rows share tiles, so they stay in the queue unless they lie back to back).
+  // Items of a shape the queue's pass variants cannot run (1D above 2^18 points,
+  // outer axes narrower than a variant's column tile) run this way whatever
+  // their size.
+  constexpr long long kRaggedGatherItems = 1024;
   if (!spec.empty()) {
     long long ext[2] = {0, 0};
     for (int i = 0; i < n_items; ++i) {
@@ -806,12 +809,18 @@ mcfftStatus mcfft_plan_ragged(mcfftPlan* plan, int n_items, const mcfftRaggedIte
       const long long cnt = (long long)kv.second.size();
       const long long n = kv.first[1] * kv.first[2] * kv.first[3];
       const bool queue_ok = ragged_shape_supported(a.rank, a.dims, (int)in_fmt, (int)out_fmt, (int)scaling) == ST_OK;
-      if (queue_ok && ((a.rank == 1 && a.dims[0] < 4096) || bulk_min <= 0 || cnt * n < bulk_min)) continue;
+      const bool enough = bulk_min > 0 && (cnt * n >= bulk_min || cnt >= kRaggedGatherItems);
+      if (queue_ok && ((a.rank == 1 && a.dims[0] < 2048) || !enough)) continue;
       PlanOpts o;
       o.gathered = true;
       const long long dims[3] = {a.dims[0], a.dims[1], a.dims[2]};
       mcfftPlan sub = nullptr;
       if (create_plan(&sub, a.rank, dims, cnt, in_fmt, out_fmt, scaling, &o) != MCFFT_SUCCESS) continue;
+      if (queue_ok && sub->core.passes.size() == 1 && sub->core.passes[0].kind == KIND_ROW &&
+          sub->core.passes[0].k->Q != 1) {  // rows would share tiles: the queue runs them
+        free_plan(sub);
+        continue;
+      }
       std::vector<long long> off[2];
       for (size_t k : kv.second) {
         off[0].push_back(spec[k].in_off);

@@ -237,6 +237,50 @@ def test_ragged_gathered_bulk_runs(torch_cuda, monkeypatch, fmt):
         assert np.array_equal(got[covered], y[covered])
 
 
+def test_ragged_many_short_items_gathered(torch_cuda):
+    """1024 or more items of one 1D length whose row pass has a one-row tile
+    (2048 and 4096 points here, shuffled among 1024-point items, with gaps) run
+    as gathered batched plans although they hold fewer than 2^23 elements;
+    1024-point items, whose tiles hold several rows, stay in the queue.  A
+    sample of every shape against the oracle; nothing between items written."""
+    torch = torch_cuda
+    rng = np.random.default_rng(31)
+    shapes = [[2048]] * 1100 + [[4096]] * 1030 + [[1024]] * 1200
+    rng.shuffle(shapes)
+    sizes = [int(np.prod(s)) for s in shapes]
+    gaps = rng.integers(0, 16, len(shapes))
+    offs, o = [], 0
+    for n, g in zip(sizes, gaps):
+        o += int(g)
+        offs.append(o)
+        o += n
+    total = o + 16
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    p = mc.RaggedPlan([(s, q, q) for s, q in zip(shapes, offs)], "fp16", "fp16")
+    desc = p.describe()
+    assert desc.count("bulk gathered") == 2, desc
+    assert "dims=2048 batch=1100" in desc and "dims=4096 batch=1030" in desc, desc
+    d_in = mc.to_device(x, "fp16")
+    d_out = mc.to_device(np.full(total, 7.0 + 0j), "fp16")
+    p.exec(d_in, d_out, mc.FORWARD)
+    torch.cuda.synchronize()
+    assert p.status_flags() == 0
+    y = mc.from_device(d_out, "fp16")
+    xq = mc.decode(mc.encode(x, "fp16"), "fp16")
+    covered = np.zeros(total, dtype=bool)
+    for s, q in zip(shapes, offs):
+        covered[q:q + int(np.prod(s))] = True
+    assert np.all(y[~covered] == 7.0)
+    worst = 0.0
+    for n in (1024, 2048, 4096):
+        idx = [i for i, s in enumerate(shapes) if s[0] == n]
+        for i in rng.choice(idx, 12, replace=False):
+            q, e = offs[i], p.exponents[i]
+            worst = max(worst, oracle.rel_l2(y[q:q + n] * 2.0 ** e, oracle.fft(xq[q:q + n], (n,))))
+    print(f"ragged short items: {len(shapes)} items, worst sampled rel_l2 {worst:.3e}")
+    assert worst <= TOL["fp16"]
+
+
 def test_ragged_shapes_outside_the_queue(torch_cuda):
     """Shapes the queue's pass variants cannot run -- narrow outer axes (256 x 16,
     512 x 16, 128 x 64 x 16) and a 1D length above 2^18 -- run as gathered
