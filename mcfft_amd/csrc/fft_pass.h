@@ -391,6 +391,14 @@ struct PassCfg {
     if constexpr (QFAST) return m * Q + q;
     else return q * ROW + m + (m >> 4);
   }
+  // lidx(q, mb + d) where d is a multiple of 16, or mb is and 0 <= d < 16: the
+  // padding term then splits, (mb + d) >> 4 == (mb >> 4) + (d >> 4), so a lane's
+  // slots are one per-lane base plus offsets the compiler can fold into the LDS
+  // instructions instead of a shift and add per element
+  static MCFFT_DEV int lidx_sum(int q, int mb, int d) {
+    if constexpr (QFAST) return (mb + d) * Q + q;
+    else return q * ROW + mb + (mb >> 4) + d + (d >> 4);
+  }
   template <int R>
   static MCFFT_DEV void colmap(int c, int& q, int& j) {  // c >= 0: unsigned division is a shift
     const unsigned u = (unsigned)c;
@@ -584,6 +592,13 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
   constexpr int L = C::L;
   constexpr bool STAGED = C::STAGED, FOURSTEP = C::FOURSTEP;
   MCFFT_DYN_LDS(El, lds);
+  // LDS slot of (q, mb + d), see lidx_sum.  The ragged kernel's split variants
+  // (BFP off, capped at 128 VGPRs) keep the per-element form: the per-lane bases
+  // of the split one spilled 10 VGPRs there.
+  auto lds_slot = [](int q, int mb, int d) {
+    if constexpr (BFP || CM != CM_SPLIT) return C::lidx_sum(q, mb, d);
+    else return C::lidx(q, mb + d);
+  };
 
   // opaque per tile: index math is redone for each tile instead of being hoisted
   // out of the tile loop and held in registers (or spilled) across it
@@ -728,7 +743,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
           if (ti < NCOLT) {
             int q, j;
             stage_col(k, q, j);
-            for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, C::lidx(q, j + (8 * h + t) * (L / 16)));
+            for (int t = 0; t < 8; ++t) x[k][t] = lds_ld(lds, lds_slot(q, j, (8 * h + t) * (L / 16)));
           }
         }
       }
@@ -769,8 +784,9 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
               const f32x2 w1 = cmul(v1, tw[u]);
               const f32x2 lo = finish(w0 + w1, q, m), hi = finish(w0 - w1, q, m + 256);
               if constexpr (STAGED) {
-                lds_st(lds, C::lidx(q, m), Tr::pack(lo));
-                lds_st(lds, C::lidx(q, m + 256), Tr::pack(hi));
+                // m = j0 + 16 (4h + 8g + u)
+                lds_st(lds, lds_slot(q, j0, m - j0), Tr::pack(lo));
+                lds_st(lds, lds_slot(q, j0, m - j0 + 256), Tr::pack(hi));
               } else {
                 store_out(q, m, Tr::pack(lo));
                 store_out(q, m + 256, Tr::pack(hi));
@@ -829,6 +845,8 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
             for (int u = 0; u < 4; ++u) {
               const int s = 4 * g + u;
               const int mo = mo0 + u * NS;
+              // mo = jb + rr * NS: jb a multiple of 16 when NS = 1 (rr < 16), rr * NS one when NS >= 16
+              const int rr = 4 * h + 8 * g + u;
               f32x2 v;
               v.x = y[s];
               v.y = y[8 + s];
@@ -843,12 +861,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
                 f32x2 r;
                 r.x = __builtin_fmaf(sg, w.x, o.x);
                 r.y = __builtin_fmaf(sg, w.y, o.y);
-                if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(finish(r, q, mo)));
+                if constexpr (STAGED) lds_st(lds, lds_slot(q, jb, rr * NS), Tr::pack(finish(r, q, mo)));
                 else store_out(q, mo, Tr::pack(finish(r, q, mo)));
               } else if constexpr (!LAST) {
-                lds_st(lds, C::lidx(q, mo), Tr::pack(cmul(v, tw[u])));
+                lds_st(lds, lds_slot(q, jb, rr * NS), Tr::pack(cmul(v, tw[u])));
               } else if constexpr (STAGED) {
-                lds_st(lds, C::lidx(q, mo), Tr::pack(finish(v, q, mo)));
+                lds_st(lds, lds_slot(q, jb, rr * NS), Tr::pack(finish(v, q, mo)));
               }
             }
             if constexpr (LAST && !STAGED) {
@@ -890,12 +908,12 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
           int q, j;
           C::template colmap<T>(c, q, j);
           f32x2 v[T];
-          for (int r = 0; r < T; ++r) v[r] = Tr::unpack(lds_ld(lds, C::lidx(q, j + r * (L / T))));
+          for (int r = 0; r < T; ++r) v[r] = Tr::unpack(lds_ld(lds, lds_slot(q, j, r * (L / T))));
           dft_small<T>(v, inv);
           for (int r = 0; r < T; ++r) {
             const int mo = j + r * (L / T);
             const f32x2 w = finish(v[r], q, mo);
-            if constexpr (STAGED) lds_st(lds, C::lidx(q, mo), Tr::pack(w));
+            if constexpr (STAGED) lds_st(lds, lds_slot(q, j, r * (L / T)), Tr::pack(w));
             else store_out(q, mo, Tr::pack(w));
           }
         }
@@ -913,7 +931,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
       const int vo = C::out_off(q, m0, sout);
       for (int i = 0; i < Q * L / C::NT; ++i)
       {
-        const El e = lds_ld(lds, C::lidx(q, m0 + i * MSTEP));
+        const El e = lds_ld(lds, (C::QFAST || MSTEP % 16 == 0) ? lds_slot(q, m0, i * MSTEP) : C::lidx(q, m0 + i * MSTEP));
         if (check) watch = nonfinite_fold(watch, e, CM);
         Tr::store_s(gout, vo, i * MSTEP * sout, e);
       }
