@@ -635,12 +635,6 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
     int e_out = BFP ? a.bfp_e_base : 0;  // exponent of the stored output (true = stored * 2^e_out)
     bool rescale = false;    // AUTO reconcile pass: the transform's first-pass tiles disagree
     uint32_t watch = ~0u;    // static last pass: Inf/NaN watch over this thread's stores (bfp.h)
-    if (BFP && !FIRST_KIND && (bf & BFP_AUTO)) {
-      int emax, emin;
-      slot_read(a.bfp_slots, (long long)a.bfp_tr0 + t_loc, emax, emin);
-      e_out = uniform(e_out + emax);
-      rescale = (bf & BFP_RECONCILE) && uniform(emin) != uniform(emax);
-    }
 
     // Scale factors.  With computed twiddles the last stage's power-of-two scale
     // rides on the twiddle the stage before it writes (the stages are linear), so
@@ -720,6 +714,14 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
             for (int t = 0; t < 8; ++t) x[k][t] = pre[k][t];
         } else {
           load_stage0<C, Tr>(a, tile_in_view<C>(a, tile, esz_in), wave, lane, x);
+        }
+        if (BFP && !FIRST_KIND && (bf & BFP_AUTO)) {
+          // AUTO later pass: the transform's exponent slots (scalar loads), read
+          // once the tile's loads are in flight so that the two latencies overlap
+          int emax, emin;
+          slot_read(a.bfp_slots, (long long)a.bfp_tr0 + t_loc, emax, emin);
+          e_out = uniform(e_out + emax);
+          rescale = (bf & BFP_RECONCILE) && uniform(emin) != uniform(emax);
         }
         if (BFP && FIRST_KIND && (bf & BFP_FIRST)) {
           // AUTO first pass: the tile's largest |component| sets its normalisation
