@@ -169,8 +169,6 @@ MCFFT_DEV unsigned wave_first(unsigned v) { return (unsigned)__builtin_amdgcn_re
 // pack the low halves / high halves of two packed pairs into one pair (v_perm_b32)
 MCFFT_DEV uint32_t lo_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x05040100u); }
 MCFFT_DEV uint32_t hi_halves(uint32_t e0, uint32_t e1) { return __builtin_amdgcn_perm(e1, e0, 0x07060302u); }
-// either of the two above, chosen per lane by sel (0x05040100 or 0x07060302)
-MCFFT_DEV uint32_t sel_halves(uint32_t e0, uint32_t e1, uint32_t sel) { return __builtin_amdgcn_perm(e1, e0, sel); }
 
 // D = A(32x16) * B(16x32) + C, f16 operands, fp32 accumulate (v_mfma_f32_32x32x16_f16).
 // Lane l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32] (t = 0..7) and

@@ -461,9 +461,9 @@ mcfftStatus create_plan(mcfftPlan* plan, int rank, const long long* dims, long l
   }
   apply_forced(p);
   if (const char* env = getenv("MCFFT_BIG_STAGGER")) {  // ticks of the 100 MHz clock[,phases]; 0: none
-    p->stagger_ticks = atoi(env);
     const char* c = strchr(env, ',');
-    p->stagger_phases = c ? atoi(c + 1) : 2;
+    p->stagger_ticks = std::min(std::max(atoi(env), 0), 100000);  // at most 1 ms
+    p->stagger_phases = std::min(std::max(c ? atoi(c + 1) : 2, 1), 16);
   }
   for (const PassPlan& pp : p->core.passes) {
     mcfftStatus s = set_lds_attr(pp.k);

@@ -258,11 +258,6 @@ inline unsigned wave_first(unsigned v) {
 }
 
 inline uint32_t lo_halves(uint32_t e0, uint32_t e1) { return (e0 & 0xffffu) | (e1 << 16); }
-inline uint32_t sel_halves(uint32_t e0, uint32_t e1, uint32_t sel) {
-  if (sel == 0x05040100u) return (e0 & 0xffffu) | (e1 << 16);
-  if (sel == 0x07060302u) return (e0 >> 16) | (e1 & 0xffff0000u);
-  emu::fail("sel_halves selector", (long long)sel);
-}
 inline uint32_t hi_halves(uint32_t e0, uint32_t e1) { return (e0 >> 16) | (e1 & 0xffff0000u); }
 
 template <class Dec>
