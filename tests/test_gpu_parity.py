@@ -175,3 +175,39 @@ def test_single_pass_65536_variants(torch_cuda, kernel, fmt, scaling, monkeypatc
     err = oracle.rel_l2(y, oracle.fft(_quantised(x, fmt), (n,)))
     print(f"65536 {kernel} rel_l2={err:.3e}")
     assert err <= TOL[fmt]
+
+
+@pytest.mark.parametrize("scaling", ["auto", "balanced"])
+def test_single_pass_65536_stagger_keeps_bits(torch_cuda, monkeypatch, scaling):
+    """The single-pass 65536 kernel's start stagger (half the workgroups start
+    late when each has four or more transforms: 1024 here) changes timing only:
+    output and exponents bit-identical with it off, and one transform against
+    the oracle."""
+    torch = torch_cuda
+    n, batch = 65536, 1024
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    d_in = torch.randn((n * batch, 2), device="cuda", generator=g).half()
+    outs, exps = [], []
+    for stagger in ("0", None):
+        if stagger is None:
+            monkeypatch.delenv("MCFFT_BIG_STAGGER", raising=False)
+        else:
+            monkeypatch.setenv("MCFFT_BIG_STAGGER", stagger)
+        p = mc.Plan([n], batch, "fp16", "fp16", scaling)
+        assert "mcfft_pass_big" in p.describe()
+        d_out = torch.empty_like(d_in)
+        p.exec(d_in, d_out, mc.FORWARD)
+        torch.cuda.synchronize()
+        outs.append(d_out)
+        exps.append(p.exponents().tolist() if scaling == "auto" else [p.exponent])
+        p.destroy()
+    assert torch.equal(outs[0].view(torch.int16), outs[1].view(torch.int16))
+    assert exps[0] == exps[1]
+    t = 517
+    e = exps[1][t] if scaling == "auto" else exps[1][0]
+    y = mc.from_device(outs[1][t * n:(t + 1) * n], "fp16") * 2.0 ** e
+    ref = oracle.fft(mc.from_device(d_in[t * n:(t + 1) * n], "fp16"), (n,))
+    err = oracle.rel_l2(y, ref)
+    print(f"65536 x {batch} {scaling} stagger: bit-identical, transform {t} rel_l2 {err:.3e}")
+    assert err <= TOL["fp16"]
