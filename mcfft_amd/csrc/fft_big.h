@@ -29,43 +29,6 @@
 
 namespace mcfft {
 
-// Row-scaled DFT matrices G_r2[i][k] = W_256^(i (16 k + r2)) = W_16^(i k) W_256^(r2 i)
-// (direction's sign; r2 = 0 is the plain 16-point DFT matrix), stored as MFMA A
-// operand words (see aops_words): 16-byte group ((b * 256 + r2 * 16 + i) * 2 + h)
-// holds entries k = 8h..8h+7 of row i, two per word (k even in the low half), as
-// their real parts (block b = 0), imaginary parts (b = 1) or negated imaginary
-// parts (b = 2), in the plan's FP16 or BF16 format.
-constexpr int kDft256Words = 3 * 256 * 2 * 4;
-// the two entries of word w: row i, columns k and k + 1, block b, scale r2
-struct Dft256Word {
-  int b, r2, i, k;
-};
-MCFFT_DEV Dft256Word dft256_word(int w) {
-  const int q = w >> 2, rem = q & 511;
-  Dft256Word d;
-  d.b = q >> 9;
-  d.r2 = rem >> 5;
-  d.i = (rem >> 1) & 15;
-  d.k = 8 * (rem & 1) + 2 * (w & 3);
-  return d;
-}
-MCFFT_DEV int dft256_exponent(int r2, int i, int k) { return (i * (16 * k + r2)) & 255; }
-
-// MFMA A operands of lane `lane` for the DFT matrix with rows scaled by
-// W_256^(r2 rr), read from the table in LDS: rows < 16 (real parts of the
-// outputs) take (re, -im), rows >= 16 (imaginary parts) take (im, re).
-MCFFT_DEV AOps dft256_aops(const uint32_t* tab, int r2, int lane) {
-  const int i = lane & 15, h = lane >> 5;
-  const bool top = (lane & 16) == 0;
-  const int b1 = top ? 0 : 1, b2 = top ? 2 : 0;
-  AOps A;
-  A.a1 = lds_ld((const u32x4*)tab, (b1 * 256 + r2 * 16 + i) * 2 + h);
-  A.a2 = lds_ld((const u32x4*)tab, (b2 * 256 + r2 * 16 + i) * 2 + h);
-  A.a1lo = A.a1;
-  A.a2lo = A.a2;
-  return A;
-}
-
 template <int NW_>
 struct BigCfg {
   static constexpr int L = 65536, LQ = 4096, NW = NW_, NT = 64 * NW_, TPW = 128 / NW_;

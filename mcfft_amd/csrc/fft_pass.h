@@ -239,6 +239,51 @@ MCFFT_DEV AOps load_aops(const u32x4* tab, int lane) {
   return A;
 }
 
+// Row-scaled DFT matrices G_r2[i][k] = W_256^(i (16 k + r2)) = W_16^(i k) W_256^(r2 i)
+// (direction's sign; r2 = 0 is the plain 16-point DFT matrix), stored as MFMA A
+// operand words (see aops_words): 16-byte group ((b * 256 + r2 * 16 + i) * 2 + h)
+// holds entries k = 8h..8h+7 of row i, two per word (k even in the low half), as
+// their real parts (block b = 0), imaginary parts (b = 1) or negated imaginary
+// parts (b = 2), in the plan's FP16 or BF16 format.
+constexpr int kDft256Words = 3 * 256 * 2 * 4;
+// the two entries of word w: row i, columns k and k + 1, block b, scale r2
+struct Dft256Word {
+  int b, r2, i, k;
+};
+MCFFT_DEV Dft256Word dft256_word(int w) {
+  const int q = w >> 2, rem = q & 511;
+  Dft256Word d;
+  d.b = q >> 9;
+  d.r2 = rem >> 5;
+  d.i = (rem >> 1) & 15;
+  d.k = 8 * (rem & 1) + 2 * (w & 3);
+  return d;
+}
+MCFFT_DEV int dft256_exponent(int r2, int i, int k) { return (i * (16 * k + r2)) & 255; }
+
+// MFMA A operands of lane `lane` for the DFT matrix with rows scaled by
+// W_256^(r2 rr), read from the table in LDS: rows < 16 (real parts of the
+// outputs) take (re, -im), rows >= 16 (imaginary parts) take (im, re).  NB = 2:
+// a table of blocks 0 and 1 only (where LDS is short), -im made by a sign flip.
+template <int NB = 3>
+MCFFT_DEV AOps dft256_aops(const uint32_t* tab, int r2, int lane) {
+  const int i = lane & 15, h = lane >> 5;
+  const bool top = (lane & 16) == 0;
+  AOps A;
+  A.a1 = lds_ld((const u32x4*)tab, ((top ? 0 : 1) * 256 + r2 * 16 + i) * 2 + h);
+  if constexpr (NB == 3) {
+    A.a2 = lds_ld((const u32x4*)tab, ((top ? 2 : 0) * 256 + r2 * 16 + i) * 2 + h);
+  } else {
+    const u32x4 w = lds_ld((const u32x4*)tab, ((top ? 1 : 0) * 256 + r2 * 16 + i) * 2 + h);
+    const uint32_t neg = top ? 0x80008000u : 0u;
+    for (int c = 0; c < 4; ++c) A.a2[c] = w[c] ^ neg;
+  }
+  A.a1lo = A.a1;
+  A.a2lo = A.a2;
+  return A;
+}
+
+
 // 16-point DFTs of 32 columns: x[t] = input r = 8*(lane/32)+t of this lane's column.
 // Returns the FP32 outputs (re in [0..7], im in [8..15] of the slot order above).
 template <int CM>
@@ -368,7 +413,15 @@ struct PassCfg {
   typedef typename Traits<CM>::T El;
   // the tile's data, then one word per wave for block-wide reductions
   static constexpr int RED_OFF = LDS_ELEMS * (int)sizeof(El);
-  static constexpr int LDS_BYTES = RED_OFF + 64;
+  // Row twiddles folded into the DFT matrix (dft256_aops): FP16/BF16 row tiles
+  // of 8192, 16384 and 32768 points, where the 32 columns of an MFMA tile share
+  // r2 = j >> (lg(L/Rn) - 4) at every twiddled stage; the matrices sit in LDS
+  // after the tile (these kernels run one workgroup per CU, held by registers),
+  // all three blocks where they fit, else the first two (TAB_BLOCKS).
+  static constexpr bool FOLD_TW = KIND == KIND_ROW && CM != CM_SPLIT && S == 3 && T >= 2;
+  static constexpr int TAB_OFF = (RED_OFF + 64 + 15) / 16 * 16;
+  static constexpr int TAB_BLOCKS = TAB_OFF + kDft256Words * 4 <= 160 * 1024 ? 3 : 2;
+  static constexpr int LDS_BYTES = FOLD_TW ? TAB_OFF + TAB_BLOCKS * (kDft256Words / 3) * 4 : RED_OFF + 64;
   static constexpr int NCOLT = Q * L / 512;          // 32-column MFMA tiles of the tile
   static constexpr int TPW = (NCOLT + NW - 1) / NW;  // MFMA tiles per wave
   // L = 512: the radix-2 tail is done right after the last radix-16 stage
@@ -595,6 +648,7 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
   // LDS slot of (q, mb + d), see lidx_sum.  The ragged kernel's split variants
   // (BFP off, capped at 128 VGPRs) keep the per-element form: the per-lane bases
   // of the split one spilled 10 VGPRs there.
+  const uint32_t* tabf = (const uint32_t*)((const unsigned char*)lds + C::TAB_OFF);  // FOLD_TW matrices
   auto lds_slot = [](int q, int mb, int d) {
     if constexpr (BFP || CM != CM_SPLIT) return C::lidx_sum(q, mb, d);
     else return C::lidx(q, mb + d);
@@ -799,6 +853,35 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
       } else
       for (int k = 0; k < TPW; ++k) {
         const int ti = wave + k * NW;
+        if constexpr (C::FOLD_TW && BFP && !LAST) {
+          // twiddle W_M^(r2 (m' % 16Ns)) = W_M^(r2 (j % Ns)) (column factor) *
+          // W_(16 Rn)^(r2 rr) (in the tile's DFT matrix, table entry r2 * 16 / Rn)
+          constexpr int RN = (P + 1 < S) ? 16 : T;
+          constexpr int SH = ilog2c(L / RN) - 4;
+          constexpr float INV_M = 1.0f / (float)(16 * NS * RN);
+          static_assert(SH >= 5 && (L / 16) % 32 == 0, "the 32 columns of an MFMA tile share r2");
+          if (ti < NCOLT) {
+            int q, j;
+            stage_col(k, q, j);
+            const int r2 = j >> SH;
+            const f32x16 y = dft16<CM>(x[k], dft256_aops<C::TAB_BLOCKS>(tabf, r2 * (16 / RN), lane));
+            const int jb = (int)(((unsigned)j / NS) * NS * 16 + ((unsigned)j % NS));
+            f32x2 cf;
+            if constexpr (P == 0) {
+              cf.x = stage_scale(0);
+              cf.y = 0.0f;
+            } else {
+              cf = twiddle_rev(r2 * (j & (NS - 1)), INV_M, inv, stage_scale(P));
+            }
+            for (int s = 0; s < 8; ++s) {
+              const int rr = 4 * h + (s & 3) + 8 * (s >> 2);
+              f32x2 v;
+              v.x = y[s];
+              v.y = y[8 + s];
+              lds_st(lds, lds_slot(q, jb, rr * NS), Tr::pack(P == 0 ? v * cf.x : cmul(v, cf)));
+            }
+          }
+        } else
         if (ti < NCOLT) {
           int q, j;
           stage_col(k, q, j);
@@ -947,6 +1030,15 @@ MCFFT_DEV void fft_tile(const PassArgs& a, int tile, const AOps& A, SlotAcc* sac
 template <int S, int T, int Q, int NW, int CM, int KIND>
 MCFFT_DEV void fft_pass_body(const PassArgs& a) {
   const AOps A = load_aops(a.aops, lane_id());
+  if constexpr (PassCfg<S, T, Q, NW, CM, KIND>::FOLD_TW) {
+    // the row-scaled DFT matrices into LDS, read by every tile's stages
+    typedef PassCfg<S, T, Q, NW, CM, KIND> C;
+    MCFFT_DYN_LDS(uint32_t, lds);
+    u32x4* tab = (u32x4*)(lds + C::TAB_OFF / 4);
+    for (int i = thread_id(); i < C::TAB_BLOCKS * (kDft256Words / 3) / 4; i += C::NT)
+      lds_st(tab, i, g_ld((const u32x4*)a.dft256, (long long)i));
+    block_barrier();
+  }
   // Column tiles narrower than a 128-byte line share lines with their
   // neighbours.  Workgroups are dealt to the 8 XCDs round robin (block b on XCD
   // b % 8), each XCD with its own L2, so within every aligned group of 32 blocks
